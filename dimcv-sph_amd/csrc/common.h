@@ -1,0 +1,242 @@
+// Shared device helpers and host plumbing for libdimcv (gfx950).
+//
+// Every f32 helper below performs the reference's operations in the
+// reference's order (Taichi evaluates left to right, vector ops elementwise,
+// dot/norm as left-to-right sums).  The library is compiled with
+// -ffp-contract=off and IEEE-correct f32 division and sqrt, so the SPH passes
+// reproduce the serial reference arithmetic bit for bit; kernels that trade
+// that for speed use explicit fmaf / reciprocal builtins.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/dimcv.h"
+
+#define DIMCV_BLOCK 256
+#define DIMCV_SORT_TILE 4096  // elements per radix-sort workgroup (256 x 16)
+#define DIMCV_SCAN_TILE 4096  // elements per canonical-scan workgroup (256 x 16)
+#define DIMCV_MAX_SCAN_BLOCKS 4096
+
+namespace dimcv {
+
+// ---------------------------------------------------------------------------
+// error plumbing (host)
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define DIMCV_HIP(call)                                   \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return ::dimcv::hip_fail(e_, #call); \
+  } while (0)
+
+#define DIMCV_LAUNCHED(what)                                  \
+  do {                                                        \
+    hipError_t e_ = hipGetLastError();                        \
+    if (e_ != hipSuccess) return ::dimcv::hip_fail(e_, what); \
+  } while (0)
+
+inline unsigned nblocks(int64_t n, int per = DIMCV_BLOCK) {
+  if (n <= 0) return 0;
+  return (unsigned)((n + per - 1) / per);
+}
+
+// ---------------------------------------------------------------------------
+// workspace layout (all offsets 256-byte aligned)
+struct Workspace {
+  uint32_t* keys[2];
+  uint32_t* vals[2];
+  uint32_t* radix_hist;   // 256 * sort blocks
+  uint32_t* scan_tmp;     // block sums for the u32 scan
+  uint32_t* compact_cnt;  // per-block counts for compaction
+  double* partials;       // per-block FP64 partial sums
+  double* cdf;            // canonical scan output
+  double* cdf_weights;    // scan input
+  double* block_tot;      // canonical scan block totals
+  double* block_pre;      // scan of block totals
+  float* near3;           // vort2vel near field per fluid entry (3 floats)
+  float* scale_s;         // vort2vel S_i per fluid entry
+  float4* samples;        // packed far-field sources: (x, y, z, -) (cx, cy, cz, -)
+  int64_t max_samples;
+  int64_t bytes;
+};
+int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Workspace* ws,
+                         void* base);
+int bind_workspace(const dimcv_state* st, const dimcv_params* p, Workspace* ws, int64_t max_samples);
+
+// ---------------------------------------------------------------------------
+// device math
+struct v3 {
+  float x, y, z;
+};
+__device__ __forceinline__ v3 mk(float a, float b, float c) { return v3{a, b, c}; }
+__device__ __forceinline__ v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 smul(float s, v3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ v3 vmuls(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 vdivs(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float vnorm(v3 a) { return sqrtf(vdot(a, a)); }
+__device__ __forceinline__ v3 vcross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ v3 ld3(const float* __restrict__ a, int64_t i) {
+  return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+}
+__device__ __forceinline__ void st3(float* __restrict__ a, int64_t i, v3 q) {
+  a[3 * i] = q.x;
+  a[3 * i + 1] = q.y;
+  a[3 * i + 2] = q.z;
+}
+
+// Cubic spline kernel, sph_base.py:23-44 ((1-q)^3 evaluated as (t*t)*t).
+__device__ __forceinline__ float cubic_W(const dimcv_params& p, float r_norm) {
+  float res = 0.0f;
+  float q = r_norm * p.inv_support_radius;
+  if (q <= 1.0f) {
+    if (q <= 0.5f) {
+      float q2 = q * q;
+      float q3 = q2 * q;
+      res = p.kernel_k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+    } else {
+      float t = 1.0f - q;
+      res = p.kernel_k * 2.0f * ((t * t) * t);
+    }
+  }
+  return res;
+}
+
+// Kernel gradient, sph_base.py:46-68.
+__device__ __forceinline__ v3 cubic_dW(const dimcv_params& p, v3 r) {
+  float rn = vnorm(r);
+  float q = rn * p.inv_support_radius;
+  v3 res = mk(0.0f, 0.0f, 0.0f);
+  if (rn > 1e-5f && q <= 1.0f) {
+    v3 gq = vmuls(vdivs(r, rn), p.inv_support_radius);
+    if (q <= 0.5f) {
+      res = smul(p.kernel_dk * q * (3.0f * q - 2.0f), gq);
+    } else {
+      float f = 1.0f - q;
+      res = smul(p.kernel_dk * (-f * f), gq);
+    }
+  }
+  return res;
+}
+
+// f32 exp: Cody-Waite reduction x = k ln2 + r, degree-8 Taylor polynomial in
+// r (Horner, no fused operations), scaling by 2^k through exponent bits.
+__device__ __forceinline__ float pow2i(int k) { return __uint_as_float((uint32_t)(k + 127) << 23); }
+__device__ inline float exp_f32(float x) {
+  if (x != x) return x;
+  if (x > 88.72283935546875f) return __int_as_float(0x7f800000);
+  if (x < -103.97208404541015625f) return 0.0f;
+  float kf = floorf(x * 1.44269502162933349609375f + 0.5f);
+  float r = (x - kf * 0.693145751953125f) - kf * 1.428606765330187045e-06f;
+  float q = 2.4801587301587302e-05f;
+  q = q * r + 1.9841269841269841e-04f;
+  q = q * r + 1.3888888888888889e-03f;
+  q = q * r + 8.3333333333333332e-03f;
+  q = q * r + 4.1666666666666664e-02f;
+  q = q * r + 1.6666666666666666e-01f;
+  q = q * r + 0.5f;
+  q = q * r + 1.0f;
+  q = q * r + 1.0f;
+  int k = (int)kf;
+  if (k > 127) return (q * pow2i(127)) * pow2i(k - 127);
+  if (k < -126) return (q * pow2i(k + 64)) * pow2i(-64);
+  return q * pow2i(k);
+}
+
+// f32 log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1)).
+__device__ inline float log_f32(float x) {
+  if (x != x) return x;
+  if (x < 0.0f) return __int_as_float(0x7fc00000);
+  if (x == 0.0f) return __int_as_float((int)0xff800000);
+  if (x == __int_as_float(0x7f800000)) return x;
+  int e = 0;
+  if (x < 1.17549435e-38f) {
+    x = x * 8388608.0f;
+    e = -23;
+  }
+  uint32_t u = __float_as_uint(x);
+  e += (int)((u >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((u & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421353816986083984375f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  float s = (m - 1.0f) / (m + 1.0f);
+  float s2 = s * s;
+  float t = 0.1111111111111111f;
+  t = t * s2 + 0.14285714285714285f;
+  t = t * s2 + 0.2f;
+  t = t * s2 + 0.33333333333333331f;
+  t = t * s2 + 1.0f;
+  float lm = (2.0f * s) * t;
+  float ef = (float)e;
+  return ef * 0.693145751953125f + (ef * 1.428606765330187045e-06f + lm);
+}
+
+__device__ __forceinline__ float pow_f32(float x, float y) { return exp_f32(y * log_f32(x)); }
+
+// dimcv_sph.py:76-84
+__device__ __forceinline__ bool in_vortex_domain(const dimcv_params& p, v3 x) {
+  if (!p.has_vortex_domain) return true;
+  if (x.x < p.vortex_start[0] || x.x > p.vortex_end[0]) return false;
+  if (x.y < p.vortex_start[1] || x.y > p.vortex_end[1]) return false;
+  if (x.z < p.vortex_start[2] || x.z > p.vortex_end[2]) return false;
+  return true;
+}
+
+// dimcv_sph.py:401-405
+__device__ __forceinline__ float transit_weight(const dimcv_params& p, float xpos) {
+  float w = (xpos - p.transit_a) / p.transit_ba;
+  return fminf(fmaxf(w, 0.0f), 1.0f);
+}
+
+// Neighbour-list entry: slot in the low 31 bits, fluid flag in bit 31.
+__device__ __forceinline__ int nb_slot(uint32_t e) { return (int)(e & 0x7fffffffu); }
+__device__ __forceinline__ bool nb_fluid(uint32_t e) { return (e >> 31) != 0u; }
+
+// Relative position x_i - x_j; with a periodic box the minimum image.
+__device__ __forceinline__ v3 rel(const dimcv_params& p, v3 xi, v3 xj) {
+  v3 r = vsub(xi, xj);
+  if (p.periodic) {
+    float hx = 0.5f * p.box_size[0], hy = 0.5f * p.box_size[1], hz = 0.5f * p.box_size[2];
+    if (r.x > hx) r.x -= p.box_size[0];
+    else if (r.x < -hx) r.x += p.box_size[0];
+    if (r.y > hy) r.y -= p.box_size[1];
+    else if (r.y < -hy) r.y += p.box_size[1];
+    if (r.z > hz) r.z -= p.box_size[2];
+    else if (r.z < -hz) r.z += p.box_size[2];
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// canonical FP64 block tree (256 values, a[t] += a[t + s], s = 128..1)
+__device__ inline double block_tree_sum256(double v, double* lds) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = 128; s >= 1; s >>= 1) {
+    if (t < s) lds[t] = lds[t] + lds[t + s];
+    __syncthreads();
+  }
+  double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// host helpers implemented in util.hip
+int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, uint32_t* total,
+                       void* stream);
+
+}  // namespace dimcv

@@ -1,0 +1,509 @@
+// Uniform grid: particle insertion, activity, stable cell-key radix sort,
+// cell table, permutation gather, fluid list and fixed-radius neighbour lists.
+//
+// Reference: particle_system.py:316-586 (add_particle, update_activity,
+// particle_partition, update_grid_id, PrefixSumExecutor, counting_sort,
+// for_all_neighbors), sph_base.py:91-112 (boundary volumes),
+// karman_vortex.py:27-36 (init_cylinder).
+//
+// The reference compacts active slots with an atomic counter and counting-
+// sorts by cell with a reversed atomic_sub loop; under serial execution both
+// are order preserving, so the slot order after a build is "active slots in
+// ascending order, stably sorted by cell".  Here that order is produced by
+// one stable LSD radix sort of (cell key, slot) in which inactive slots carry
+// a key past the last cell, followed by one gather of every persistent field.
+#include "common.h"
+
+namespace dimcv {
+
+// ---------------------------------------------------------------------------
+// particle insertion (particle_system.py:316-380)
+__global__ void k_add_particles(dimcv_state st, dimcv_params p, int32_t obj, int32_t n,
+                                const float* __restrict__ pos, const float* __restrict__ vel,
+                                const float* __restrict__ dens, const float* __restrict__ pres,
+                                const int32_t* __restrict__ mat, const int32_t* __restrict__ dyn,
+                                const int32_t* __restrict__ col) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int64_t pn = st.counters[DIMCV_C_PARTICLE_NUM];
+  int64_t q = pn + k;
+  if (q >= p.n_max) return;
+  st.object_id[q] = obj;
+  for (int d = 0; d < 3; ++d) {
+    st.x[3 * q + d] = pos[3 * k + d];
+    st.x_0[3 * q + d] = pos[3 * k + d];
+    st.v[3 * q + d] = vel[3 * k + d];
+    st.color[3 * q + d] = col[3 * k + d];
+    st.acceleration[3 * q + d] = 0.0f;
+  }
+  st.density[q] = dens[k];
+  st.m_V[q] = p.m_V0;
+  st.m[q] = p.m_V0 * dens[k];
+  st.pressure[q] = pres[k];
+  st.material[q] = mat[k];
+  st.is_dynamic[q] = dyn[k] != 0 ? 1 : 0;
+  st.is_active[q] = 1;
+  st.is_sample[q] = 0;
+  st.life_time[q] = 0.0f;
+  if (mat[k] == 1) atomicAdd(&st.counters[DIMCV_C_FLUID_NUM], 1);
+}
+
+__global__ void k_add_finish(int32_t* counters, int64_t n_max, int32_t n, int32_t* dropped) {
+  int64_t pn = counters[DIMCV_C_PARTICLE_NUM];
+  int64_t want = pn + n;
+  int64_t got = want < n_max ? want : n_max;
+  counters[DIMCV_C_PARTICLE_NUM] = (int32_t)got;
+  *dropped = (int32_t)(want - got);
+}
+
+// karman_vortex.py:27-36
+__global__ void k_init_cylinder(dimcv_state st, int32_t bound, int32_t obj, float cx, float cy, float radius) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bound || i >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.object_id[i] != obj) return;
+  float dx = st.x[3 * i] - cx;
+  float dy = st.x[3 * i + 1] - cy;
+  if (sqrtf(dx * dx + dy * dy) > radius) {
+    st.x[3 * i] = 0.0f;
+    st.x[3 * i + 1] = 0.0f;
+    st.x[3 * i + 2] = 0.0f;
+    st.is_active[i] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pos_to_index / flatten (particle_system.py:382-402): f32 division by the
+// f32 grid size, truncation toward zero, row-major flattening.
+__device__ __forceinline__ void cell_of(const dimcv_params& p, v3 x, int c[3]) {
+  c[0] = (int)(x.x / p.grid_size);
+  c[1] = (int)(x.y / p.grid_size);
+  c[2] = (int)(x.z / p.grid_size);
+}
+
+// update_activity (:413-427) fused with update_grid_id's key computation
+// (:490-498).  Inactive slots get key n_cells (sorted past every cell).
+__global__ void k_activity_keys(dimcv_state st, dimcv_params p, int64_t n_in,
+                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_in) return;
+  const uint32_t sentinel = (uint32_t)p.n_cells;
+  vals[i] = (uint32_t)i;
+  if (i >= st.counters[DIMCV_C_PARTICLE_NUM]) {
+    keys[i] = sentinel;
+    return;
+  }
+  int32_t mat = st.material[i];
+  v3 xi = ld3(st.x, i);
+  uint8_t active = st.is_active[i];
+  if (mat == 0) {
+    active = 1;
+  } else if (mat == 1) {
+    float q[3] = {xi.x, xi.y, xi.z};
+    for (int j = 0; j < 3; ++j) {
+      if (q[j] < p.domain_start[j] || q[j] > p.domain_end[j]) {
+        active = 0;
+        atomicSub(&st.counters[DIMCV_C_FLUID_NUM], 1);
+        int32_t o = st.object_id[i];
+        if (st.n_emitters > 0 && o >= 0 && o < st.n_emitters)
+          atomicSub(&st.num_particles_each_emitter[o], 1);
+        break;
+      }
+    }
+  }
+  st.is_active[i] = active;
+  if (!active) {
+    keys[i] = sentinel;
+    return;
+  }
+  int c[3];
+  cell_of(p, xi, c);
+  bool ok = true;
+  for (int d = 0; d < 3; ++d)
+    if (c[d] < 0 || c[d] >= p.grid_num[d]) ok = false;
+  if (!ok) {
+    atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
+    keys[i] = sentinel;
+    return;
+  }
+  keys[i] = (uint32_t)(c[0] * p.grid_num[1] * p.grid_num[2] + c[1] * p.grid_num[2] + c[2]);
+}
+
+// particle_num = fluid + solid (:467-468), never above the slots scanned.
+__global__ void k_set_particle_num(int32_t* counters, int32_t n_in) {
+  int32_t pn = counters[DIMCV_C_FLUID_NUM] + counters[DIMCV_C_SOLID_NUM];
+  counters[DIMCV_C_PREV_NUM] = counters[DIMCV_C_PARTICLE_NUM];
+  counters[DIMCV_C_PARTICLE_NUM] = pn < n_in ? pn : n_in;
+  counters[DIMCV_C_NBR_MAX] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits, 4096 keys per workgroup
+__global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n,
+                                                    int shift, uint32_t* __restrict__ hist, int nblk) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * DIMCV_SORT_TILE;
+#pragma unroll 4
+  for (int k = 0; k < DIMCV_SORT_TILE / 256; ++k) {
+    int64_t i = base + k * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Each workgroup walks its tile in 16 strips of 256 keys (index order).  In a
+// strip, every wave ranks its lanes among same-digit lanes with 8 ballots;
+// wave counts per digit go to LDS so a key's output position is
+//   base(digit, block) + keys of that digit in earlier strips
+//   + same-digit keys in lower waves + same-digit lanes below it,
+// which is exactly its stable rank.
+__global__ void __launch_bounds__(256) k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int nblk) {
+  __shared__ uint32_t cnt[256];
+  __shared__ uint32_t wc[4][256];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  cnt[t] = offs[(int64_t)t * nblk + blockIdx.x];
+  wc[0][t] = 0;
+  wc[1][t] = 0;
+  wc[2][t] = 0;
+  wc[3][t] = 0;
+  __syncthreads();
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int64_t base = (int64_t)blockIdx.x * DIMCV_SORT_TILE;
+  for (int k = 0; k < DIMCV_SORT_TILE / 256; ++k) {
+    int64_t i = base + k * 256 + t;
+    bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = cnt[d] + rank;
+      for (int ww = 0; ww < w; ++ww) pos += wc[ww][d];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    cnt[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+    wc[0][t] = 0;
+    wc[1][t] = 0;
+    wc[2][t] = 0;
+    wc[3][t] = 0;
+    __syncthreads();
+  }
+}
+
+// Cell table: grid_particles_num[c] = number of sorted keys <= c (the
+// reference's inclusive prefix sum of per-cell counts); the counting sort's
+// remaining per-cell counter is zero for every cell afterwards.
+__global__ void k_cell_end(const uint32_t* __restrict__ keys, int64_t n, int32_t n_cells,
+                           int32_t* __restrict__ gpn, int32_t* __restrict__ gpn_temp) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] <= (uint32_t)c) lo = mid + 1;
+    else hi = mid;
+  }
+  gpn[c] = (int32_t)lo;
+  gpn_temp[c] = 0;
+}
+
+// One gather of every sorted field into its buffer (particle_partition +
+// counting_sort copies, :435-453 and :513-532).
+__global__ void k_gather(dimcv_state st, int64_t n_in, const uint32_t* __restrict__ keys,
+                         const uint32_t* __restrict__ perm) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_in || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  int64_t i = perm[s];
+  st.grid_ids_buffer[s] = (int32_t)keys[s];
+  st.object_id_buffer[s] = st.object_id[i];
+  st3(st.x_buffer, s, ld3(st.x, i));
+  st3(st.x_0_buffer, s, ld3(st.x_0, i));
+  st3(st.v_buffer, s, ld3(st.v, i));
+  st.m_V_buffer[s] = st.m_V[i];
+  st.m_buffer[s] = st.m[i];
+  st.density_buffer[s] = st.density[i];
+  st.pressure_buffer[s] = st.pressure[i];
+  st.material_buffer[s] = st.material[i];
+  st.color_buffer[3 * s] = st.color[3 * i];
+  st.color_buffer[3 * s + 1] = st.color[3 * i + 1];
+  st.color_buffer[3 * s + 2] = st.color[3 * i + 2];
+  st.is_dynamic_buffer[s] = st.is_dynamic[i];
+  st.dfsph_factor_buffer[s] = st.dfsph_factor[i];
+  st.density_adv_buffer[s] = st.density_adv[i];
+  st.is_active_buffer[s] = 1;
+  st.is_sample_buffer[s] = st.is_sample[i];
+  st.life_time_buffer[s] = st.life_time[i];
+}
+
+// ---------------------------------------------------------------------------
+// ordered compaction: out = { i < n : pred(i) } ascending, count -> *total
+struct PredFluid {
+  const int32_t* material;
+  const int32_t* n;
+  __device__ bool operator()(int64_t i) const { return i < *n && material[i] == 1; }
+};
+struct PredSample {
+  const uint8_t* is_sample;
+  const int32_t* n;
+  __device__ bool operator()(int64_t i) const { return i < *n && is_sample[i] == 1; }
+};
+
+template <class Pred>
+__global__ void __launch_bounds__(256) k_compact_count(Pred pred, int64_t n, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t wsum[4];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool f = i < n && pred(i);
+  uint64_t b = __ballot(f);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+template <class Pred>
+__global__ void __launch_bounds__(256) k_compact_write(Pred pred, int64_t n, const uint32_t* __restrict__ off,
+                                                       int32_t* __restrict__ out) {
+  __shared__ uint32_t wsum[4];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool f = i < n && pred(i);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t b = __ballot(f);
+  if (lane == 0) wsum[w] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t pre = off[blockIdx.x];
+  for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (f) out[pre + (uint32_t)__popcll(b & lt)] = (int32_t)i;
+}
+
+template <class Pred>
+static int compact(Pred pred, int64_t n, int32_t* out, int32_t* total, Workspace& ws, hipStream_t s) {
+  unsigned nb = nblocks(n);
+  if (nb == 0) {
+    DIMCV_HIP(hipMemsetAsync(total, 0, sizeof(int32_t), s));
+    return DIMCV_OK;
+  }
+  hipLaunchKernelGGL(k_compact_count<Pred>, dim3(nb), dim3(256), 0, s, pred, n, ws.compact_cnt);
+  int rc = exclusive_scan_u32(ws.compact_cnt, ws.compact_cnt, nb, ws.scan_tmp, (uint32_t*)total, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compact_write<Pred>, dim3(nb), dim3(256), 0, s, pred, n, ws.compact_cnt, out);
+  DIMCV_LAUNCHED("compact");
+  return DIMCV_OK;
+}
+
+int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t s) {
+  PredSample pr{st->is_sample, st->counters + DIMCV_C_PARTICLE_NUM};
+  return compact(pr, n, st->sample_idx, st->counters + DIMCV_C_NUM_SAMPLES, ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// for_all_neighbors (particle_system.py:562-586)
+template <class F>
+__device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* __restrict__ x,
+                                           const int32_t* __restrict__ gpn, int64_t i, v3 xi, F&& f) {
+  int cc[3];
+  cell_of(p, xi, cc);
+  const int ny = p.grid_num[1], nz = p.grid_num[2];
+  for (int o0 = -1; o0 <= 1; ++o0) {
+    int g0 = cc[0] + o0;
+    if (g0 < 0 || g0 >= p.grid_num[0]) continue;
+    for (int o1 = -1; o1 <= 1; ++o1) {
+      int g1 = cc[1] + o1;
+      if (g1 < 0 || g1 >= ny) continue;
+      for (int o2 = -1; o2 <= 1; ++o2) {
+        int g2 = cc[2] + o2;
+        if (g2 < 0 || g2 >= nz) continue;
+        int fl = g0 * ny * nz + g1 * nz + g2;
+        int beg = gpn[fl > 0 ? fl - 1 : 0];
+        int end = gpn[fl];
+        for (int j = beg; j < end; ++j) {
+          if (j == i) continue;
+          v3 xj = ld3(x, j);
+          if (vnorm(vsub(xi, xj)) < p.support_radius) f(j);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  int cnt = 0;
+  const int kmax = p.nbr_kmax;
+  const int64_t stride = p.n_max;
+  scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) {
+    if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
+    ++cnt;
+  });
+  st.nbr_count[f] = cnt;
+  atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+}
+
+// sph_base.py:91-112: Akinci boundary volume over solid neighbours.
+__global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound, int dynamic_sel) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bound || i >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[i] != 0 || (st.is_dynamic[i] != 0) != (dynamic_sel != 0)) return;
+  v3 xi = ld3(st.x, i);
+  float delta = cubic_W(p, 0.0f);
+  scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) {
+    if (st.material[j] == 0) delta += cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))));
+  });
+  st.m_V[i] = 1.0f / delta * 3.0f;
+}
+
+}  // namespace dimcv
+
+using namespace dimcv;
+
+extern "C" {
+
+int dimcv_add_particles(const dimcv_state* st, const dimcv_params* p, int32_t object_id, int32_t n,
+                        const float* pos, const float* vel, const float* density,
+                        const float* pressure, const int32_t* material, const int32_t* is_dynamic,
+                        const int32_t* color, int32_t* dropped, void* stream) {
+  if (!st || !p || n < 0) {
+    set_error("dimcv_add_particles: bad arguments");
+    return DIMCV_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  if (n > 0)
+    hipLaunchKernelGGL(k_add_particles, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, object_id, n, pos,
+                       vel, density, pressure, material, is_dynamic, color);
+  int32_t* dev_drop = (int32_t*)ws.compact_cnt;
+  hipLaunchKernelGGL(k_add_finish, dim3(1), dim3(1), 0, s, st->counters, p->n_max, n, dev_drop);
+  DIMCV_LAUNCHED("k_add_particles");
+  if (dropped) {
+    DIMCV_HIP(hipMemcpyAsync(dropped, dev_drop, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DIMCV_HIP(hipStreamSynchronize(s));
+  }
+  return DIMCV_OK;
+}
+
+int dimcv_init_cylinder(const dimcv_state* st, const dimcv_params* p, int32_t object_id, float cx,
+                        float cy, float radius, void* stream) {
+  (void)p;
+  int32_t bound = (int32_t)p->n_max;
+  hipLaunchKernelGGL(k_init_cylinder, dim3(nblocks(bound)), dim3(256), 0, (hipStream_t)stream, *st,
+                     bound, object_id, cx, cy, radius);
+  DIMCV_LAUNCHED("k_init_cylinder");
+  return DIMCV_OK;
+}
+
+int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_particle_num,
+                     int32_t* host_counters, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  if (prev_particle_num < 0 || prev_particle_num > p->n_max || p->n_cells <= 0) {
+    set_error("dimcv_grid_build: particle bound %d outside [0, %lld]", prev_particle_num,
+              (long long)p->n_max);
+    return DIMCV_ERR_ARG;
+  }
+  const int64_t n_in = prev_particle_num;
+  DIMCV_HIP(hipMemsetAsync(st->counters + DIMCV_C_KEY_ERROR, 0, sizeof(int32_t), s));
+  if (n_in > 0)
+    hipLaunchKernelGGL(k_activity_keys, dim3(nblocks(n_in)), dim3(256), 0, s, *st, *p, n_in, ws.keys[0],
+                       ws.vals[0]);
+  hipLaunchKernelGGL(k_set_particle_num, dim3(1), dim3(1), 0, s, st->counters, (int32_t)n_in);
+  DIMCV_LAUNCHED("k_activity_keys");
+
+  // stable radix sort over the key bits in use (sentinel = n_cells)
+  int nbits = 32 - __builtin_clz((unsigned)p->n_cells);
+  int passes = (nbits + 7) / 8;
+  int cur = 0;
+  if (n_in > 0) {
+    int nblk = (int)((n_in + DIMCV_SORT_TILE - 1) / DIMCV_SORT_TILE);
+    for (int ps = 0; ps < passes; ++ps) {
+      int shift = 8 * ps;
+      hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift,
+                         ws.radix_hist, nblk);
+      rc = exclusive_scan_u32(ws.radix_hist, ws.radix_hist, (int64_t)256 * nblk, ws.scan_tmp, nullptr, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
+                         ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
+      DIMCV_LAUNCHED("k_radix_scatter");
+      cur ^= 1;
+    }
+  }
+  hipLaunchKernelGGL(k_cell_end, dim3(nblocks(p->n_cells)), dim3(256), 0, s, ws.keys[cur], n_in,
+                     p->n_cells, st->grid_particles_num, st->grid_particles_num_temp);
+  if (n_in > 0)
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(n_in)), dim3(256), 0, s, *st, n_in, ws.keys[cur],
+                       ws.vals[cur]);
+  DIMCV_LAUNCHED("k_gather");
+  PredFluid pf{st->material_buffer, st->counters + DIMCV_C_PARTICLE_NUM};
+  rc = compact(pf, n_in, st->fluid_list, st->counters + DIMCV_C_FLUID_LIST, ws, s);
+  if (rc) return rc;
+  if (host_counters) {
+    DIMCV_HIP(hipMemcpyAsync(host_counters, st->counters, sizeof(int32_t) * DIMCV_C_COUNT,
+                             hipMemcpyDeviceToHost, s));
+    DIMCV_HIP(hipStreamSynchronize(s));
+    if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
+      set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
+      return DIMCV_ERR_KEY;
+    }
+  }
+  return DIMCV_OK;
+}
+
+int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                         void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (p->periodic) {
+    set_error("periodic neighbour search is not available");
+    return DIMCV_ERR_ARG;
+  }
+  if (fluid_bound > 0)
+    hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound);
+  DIMCV_LAUNCHED("k_nbr_build");
+  int32_t mx = 0;
+  DIMCV_HIP(hipMemcpyAsync(&mx, st->counters + DIMCV_C_NBR_MAX, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  if (mx > p->nbr_kmax) {
+    set_error("neighbour list width %d exceeds nbr_kmax %d", mx, p->nbr_kmax);
+    return DIMCV_ERR_NBR;
+  }
+  return DIMCV_OK;
+}
+
+int dimcv_compute_static_boundary_volume(const dimcv_state* st, const dimcv_params* p,
+                                         int32_t particle_bound, void* stream) {
+  if (particle_bound > 0)
+    hipLaunchKernelGGL(k_boundary_volume, dim3(nblocks(particle_bound)), dim3(256), 0, (hipStream_t)stream,
+                       *st, *p, particle_bound, 0);
+  DIMCV_LAUNCHED("k_boundary_volume");
+  return DIMCV_OK;
+}
+
+int dimcv_compute_moving_boundary_volume(const dimcv_state* st, const dimcv_params* p,
+                                         int32_t particle_bound, void* stream) {
+  if (particle_bound > 0)
+    hipLaunchKernelGGL(k_boundary_volume, dim3(nblocks(particle_bound)), dim3(256), 0, (hipStream_t)stream,
+                       *st, *p, particle_bound, 1);
+  DIMCV_LAUNCHED("k_boundary_volume");
+  return DIMCV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,640 @@
+// SPH / DFSPH / DIMCV neighbour passes over the per-step neighbour lists.
+//
+// One thread per fluid-list entry f (fluid slot i = fluid_list[f]); the
+// neighbours are read from the k-major list nbr[k * n_max + f] in the order
+// for_all_neighbors visits them (particle_system.py:562-586), so every
+// accumulation below is the reference's serial sum, term by term, in the
+// reference's expression order (dimcv_sph.py, sph_base.py).
+#include "common.h"
+
+namespace dimcv {
+
+#define FLUID_ENTRY()                                                       \
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;        \
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;          \
+  const int64_t i = st.fluid_list[f];                                      \
+  const int nn = min(st.nbr_count[f], p.nbr_kmax);                         \
+  const uint32_t* __restrict__ nl = st.nbr + f;                            \
+  const int64_t ns = p.n_max
+
+#define FOR_NBR(k, e, j)                       \
+  for (int k = 0; k < nn; ++k)                 \
+    if (const uint32_t e = nl[(int64_t)k * ns]; true) \
+      if (const int64_t j = nb_slot(e); true)
+
+// Zero-fill of slots that a "field.fill(0) + fluid writes" kernel leaves at
+// zero: every slot >= particle_num or not fluid.
+__global__ void k_zero_nonfluid(const int32_t* __restrict__ material, const int32_t* __restrict__ counters,
+                                int64_t n_max, float* a, int ca, float* b, int cb, float* c, int cc) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_max) return;
+  if (s < counters[DIMCV_C_PARTICLE_NUM] && material[s] == 1) return;
+  for (int k = 0; k < ca; ++k) a[s * ca + k] = 0.0f;
+  if (b)
+    for (int k = 0; k < cb; ++k) b[s * cb + k] = 0.0f;
+  if (c)
+    for (int k = 0; k < cc; ++k) c[s * cc + k] = 0.0f;
+}
+
+static int zero_nonfluid(const dimcv_state* st, const dimcv_params* p, hipStream_t s, float* a, int ca,
+                         float* b = nullptr, int cb = 0, float* c = nullptr, int cc = 0) {
+  hipLaunchKernelGGL(k_zero_nonfluid, dim3(nblocks(p->n_max)), dim3(256), 0, s, st->material, st->counters,
+                     p->n_max, a, ca, b, cb, c, cc);
+  DIMCV_LAUNCHED("k_zero_nonfluid");
+  return DIMCV_OK;
+}
+
+// dimcv_sph.py:126-148
+__global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  float dens = st.m_V[i] * cubic_W(p, 0.0f);
+  float den = 0.0f;
+  FOR_NBR(k, e, j) { den += st.m_V[j] * cubic_W(p, vnorm(vsub(xi, ld3(st.x, j)))); }
+  dens += den;
+  dens *= p.density0;
+  st.density[i] = dens;
+}
+
+// dimcv_sph.py:86-124
+__global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+  FOR_NBR(k, e, j) {
+    v3 gw = cubic_dW(p, vsub(xi, ld3(st.x, j)));
+    if (nb_fluid(e)) {
+      v3 g = smul(st.m[j], gw);
+      r3 += vdot(g, g);
+      r0 += g.x;
+      r1 += g.y;
+      r2 += g.z;
+    } else {
+      v3 g = smul(st.m_V[j] * p.density0, gw);
+      r0 += g.x;
+      r1 += g.y;
+      r2 += g.z;
+    }
+  }
+  float sum = r3;
+  v3 gp = mk(r0, r1, r2);
+  sum += vdot(gp, gp);
+  float factor = 0.0f;
+  if (sum > 1e-6f) factor = 1.0f / sum;
+  st.dfsph_factor[i] = factor;
+}
+
+// dimcv_sph.py:337-374
+__global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+  float dadv = 0.0f;
+  FOR_NBR(k, e, j) {
+    dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, vsub(xi, ld3(st.x, j))));
+  }
+  float d = fmaxf(dadv, 0.0f);
+  if (st.nbr_count[f] < 20) d = 0.0f;
+  st.density_adv[i] = d;
+}
+
+// dimcv_sph.py:395-399
+__global__ void k_multiply(dimcv_state st, float* field, float scale, int32_t bound) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  field[i] *= scale;
+}
+
+// dimcv_sph.py:407-419: per-block canonical FP64 tree over fluid-list order.
+__global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p, float offset, int32_t bound,
+                                                  double* __restrict__ partials) {
+  __shared__ double lds[256];
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
+    const int64_t i = st.fluid_list[f];
+    float t;
+    if (!p.has_transit) t = st.density_adv[i] - offset;
+    else t = transit_weight(p, st.x[3 * i]) * (st.density_adv[i] - offset);
+    v = (double)t;
+  }
+  double s = block_tree_sum256(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// dimcv_sph.py:421-468
+__global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  const float dt = p.dt;
+  v3 xi = ld3(st.x, i);
+  float b_i = st.density_adv[i];
+  float k_i = b_i * st.dfsph_factor[i] * st.density[i];
+  v3 dv = mk(0.0f, 0.0f, 0.0f);
+  FOR_NBR(k, e, j) {
+    if (nb_fluid(e)) {
+      float b_j = st.density_adv[j];
+      float k_j = b_j * st.dfsph_factor[j] * st.density[j];
+      float k_sum = k_i + k_j;
+      if (fabsf(k_sum) > p.m_eps) {
+        v3 g = smul(st.m[j], cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        dv = vsub(dv, smul(dt * k_sum, g));
+      }
+    } else {
+      if (fabsf(k_i) > p.m_eps) {
+        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 vc = smul(-dt * 1.0f * k_i, g);
+        dv = vadd(dv, vc);
+      }
+    }
+  }
+  v3 vi = ld3(st.v, i);
+  if (!p.has_transit) vi = vadd(vi, dv);
+  else vi = vadd(vi, smul(transit_weight(p, xi.x), dv));
+  st3(st.v, i, vi);
+}
+
+// dimcv_sph.py:150-208, fluid particles (surface tension sigma = 0 and the
+// boundary viscosity 0 contribute exact zeros and are not evaluated).
+__global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+  v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
+  FOR_NBR(k, e, j) {
+    if (nb_fluid(e)) {
+      v3 r = vsub(xi, ld3(st.x, j));
+      float v_xy = vdot(vsub(vi, ld3(st.v, j)), r);
+      float rn = vnorm(r);
+      v3 fv = smul(p.visc_coef * st.m_V[j] * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
+      dv = vadd(dv, fv);
+    }
+  }
+  st3(st.acceleration, i, dv);
+}
+
+// dimcv_sph.py:197-204 for non-fluid slots: static rigid -> 0, otherwise g.
+__global__ void k_nonpressure_other(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[s] == 1) return;
+  if (st.material[s] == 0 && !st.is_dynamic[s]) st3(st.acceleration, s, mk(0.0f, 0.0f, 0.0f));
+  else st3(st.acceleration, s, mk(p.gravity[0], p.gravity[1], p.gravity[2]));
+}
+
+// dimcv_sph.py:220-226
+__global__ void k_predict(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.is_dynamic[s] && st.material[s] == 1)
+    st3(st.v, s, vadd(ld3(st.v, s), smul(p.dt, ld3(st.acceleration, s))));
+}
+
+// dimcv_sph.py:210-218
+__global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (!st.is_dynamic[s]) return;
+  v3 v = ld3(st.v, s);
+  if (st.material[s] == 0) {
+    v = vadd(v, smul(p.dt, ld3(st.acceleration, s)));
+    st3(st.v, s, v);
+  }
+  st3(st.x, s, vadd(ld3(st.x, s), smul(p.dt, v)));
+}
+
+// dimcv_sph.py:470-493
+__global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+  float delta = 0.0f;
+  FOR_NBR(k, e, j) {
+    float dd = vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, vsub(xi, ld3(st.x, j))));
+    if (nb_fluid(e)) delta += st.m[j] * dd;
+    else delta += st.m_V[j] * p.density0 * dd;
+  }
+  float da = st.density[i] + p.dt * delta;
+  st.density_adv[i] = fmaxf(da, p.density0);
+}
+
+// dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour)
+__global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  const float dt = p.dt;
+  v3 xi = ld3(st.x, i);
+  float b_i = st.density_adv[i] - p.density0;
+  float k_i = b_i * st.dfsph_factor[i];
+  v3 vi = ld3(st.v, i);
+  const float w = p.has_transit ? transit_weight(p, xi.x) : 1.0f;
+  FOR_NBR(k, e, j) {
+    if (nb_fluid(e)) {
+      float b_j = st.density_adv[j] - p.density0;
+      float k_j = b_j * st.dfsph_factor[j];
+      float k_sum = k_i + k_j;
+      if (fabsf(k_sum) > p.m_eps) {
+        v3 g = smul(st.m[j], cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, g));
+        else vi = vsub(vi, smul(w * dt * k_sum, g));
+      }
+    } else {
+      if (fabsf(k_i) > p.m_eps) {
+        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 vc = smul(-dt * 1.0f * k_i, g);
+        if (!p.has_transit) vi = vadd(vi, vc);
+        else vi = vadd(vi, smul(w, vc));
+      }
+    }
+  }
+  st3(st.v, i, vi);
+}
+
+// dimcv_sph.py:797-801
+__global__ void k_copy_x_temp(dimcv_state st, int64_t n_max) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_max) return;
+  v3 q = s < st.counters[DIMCV_C_PARTICLE_NUM] ? ld3(st.x, s) : mk(0.0f, 0.0f, 0.0f);
+  st3(st.x_temp, s, q);
+}
+
+// dimcv_sph.py:778-795
+__global__ void __launch_bounds__(256) k_vorticity_vis(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  float w = st.m_V[i] * cubic_W(p, 0.0f);
+  v3 vort = smul(w, ld3(st.vorticity, i));
+  FOR_NBR(k, e, j) {
+    float wj = st.m_V[j] * cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))));
+    vort = vadd(vort, smul(wj, ld3(st.vorticity, j)));
+    w += wj;
+  }
+  st3(st.vorticity_vis, i, vdivs(vort, w + 1e-6f));
+}
+
+// sph_base.py:118-123,149-179 for fluid particles
+__global__ void k_bounce(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[s] != 1 || !st.is_dynamic[s]) return;
+  v3 pos = ld3(st.x, s);
+  float q[3] = {pos.x, pos.y, pos.z};
+  float nrm[3] = {0.0f, 0.0f, 0.0f};
+  for (int d = 0; d < 3; ++d) {
+    if (q[d] > p.bounce_hi[d]) {
+      nrm[d] += 1.0f;
+      st.x[3 * s + d] = p.bounce_hi[d];
+    }
+    if (q[d] <= p.bounce_padding) {
+      nrm[d] += -1.0f;
+      st.x[3 * s + d] = p.bounce_padding;
+    }
+  }
+  v3 cn = mk(nrm[0], nrm[1], nrm[2]);
+  float len = vnorm(cn);
+  if (len > 1e-6f) {
+    v3 vec = vdivs(cn, len);
+    v3 v = ld3(st.v, s);
+    const float c_f = 0.5f;
+    st3(st.v, s, vsub(v, smul((1.0f + c_f) * vdot(v, vec), vec)));
+  }
+}
+
+// ---- DIMCV vorticity passes -------------------------------------------------
+
+// dimcv_sph.py:264-271
+__global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  st3(st.v_star, i, vadd(ld3(st.v, i), vmuls(ld3(st.acceleration, i), p.dt)));
+}
+
+// dimcv_sph.py:228-248,284-293 and 250-262,273-282; MODE 1 = vorticity,
+// 2 = grad_v, 3 = both (+ v*) in one neighbour pass.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+  v3 vort = mk(0.0f, 0.0f, 0.0f);
+  float G[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  FOR_NBR(k, e, j) {
+    if (!nb_fluid(e)) continue;
+    v3 vj = ld3(st.v, j);
+    v3 kd = cubic_dW(p, vsub(xi, ld3(st.x, j)));
+    v3 vji = vsub(vj, vi);
+    float mv = st.m_V[j];
+    if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
+    if (MODE & 2) {
+      v3 a = smul(mv, vji);
+      G[0] = G[0] + a.x * kd.x; G[1] = G[1] + a.x * kd.y; G[2] = G[2] + a.x * kd.z;
+      G[3] = G[3] + a.y * kd.x; G[4] = G[4] + a.y * kd.y; G[5] = G[5] + a.y * kd.z;
+      G[6] = G[6] + a.z * kd.x; G[7] = G[7] + a.z * kd.y; G[8] = G[8] + a.z * kd.z;
+    }
+  }
+  if (MODE & 1) st3(st.vorticity, i, vort);
+  if (MODE & 2)
+    for (int q = 0; q < 9; ++q) st.grad_v[9 * i + q] = G[q];
+  if (MODE == 3) st3(st.v_star, i, vadd(vi, vmuls(ld3(st.acceleration, i), p.dt)));
+}
+
+// dimcv_sph.py:551-583
+__global__ void __launch_bounds__(256) k_d_vorticity(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i), vsi = ld3(st.v_star, i);
+  v3 curl = mk(0.0f, 0.0f, 0.0f);
+  FOR_NBR(k, e, j) {
+    if (!nb_fluid(e)) continue;
+    curl = vadd(curl, smul(st.m_V[j], vcross(vsub(ld3(st.v_star, j), vsi), cubic_dW(p, vsub(xi, ld3(st.x, j))))));
+  }
+  st3(st.d_vort, i, vsub(ld3(st.vorticity, i), curl));
+}
+
+// dimcv_sph.py:660-677
+__global__ void __launch_bounds__(256) k_d_vort_smoothed(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  v3 acc = vmuls(smul(st.m_V[i], ld3(st.d_vort, i)), cubic_W(p, 0.0f));
+  FOR_NBR(k, e, j) {
+    if (!nb_fluid(e)) continue;
+    acc = vadd(acc, vmuls(smul(st.m_V[j], ld3(st.d_vort, j)), cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))))));
+  }
+  st3(st.d_vort_smoothed, i, acc);
+}
+
+// dimcv_sph.py:295-324 with host-fed uniforms u1/u2 per slot.
+__global__ void k_kvn_sample(dimcv_state st, dimcv_params p, const float* __restrict__ u1,
+                             const float* __restrict__ u2, int do_bernoulli, int32_t bound) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  if (!in_vortex_domain(p, ld3(st.x, i))) return;
+  const float* G = st.grad_v + 9 * i;
+  float g[9];
+  for (int q = 0; q < 9; ++q) g[q] = G[q];
+  float Wn = 0.0f, Sn = 0.0f;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      float w = g[3 * r + c] - g[3 * c + r];
+      float s = g[3 * r + c] + g[3 * c + r];
+      if (r == 0 && c == 0) {
+        Wn = w * w;
+        Sn = s * s;
+      } else {
+        Wn = Wn + w * w;
+        Sn = Sn + s * s;
+      }
+    }
+  float kvn = sqrtf(Wn) / (sqrtf(Sn) + 1e-8f);
+  float t0 = fminf(fmaxf(st.life_time[i], 0.0f), 1.0f);
+  float rr = 0.018f * exp_f32(p.neg_gen_a * t0) + 0.004f;
+  float produce = 1.0f - 1.0f / pow_f32(kvn + 1.0f, rr);
+  float del = exp_f32(p.neg_del_b * kvn);
+  st.kinematic_vort_num[i] = kvn;
+  if (!do_bernoulli) return;
+  uint8_t smp = st.is_sample[i];
+  if (smp == 0 && u1[i] < produce) st.is_sample[i] = 1;
+  else if (smp == 1 && kvn < 2.0f && u2[i] < del) st.is_sample[i] = 0;
+}
+
+__device__ __forceinline__ float epan(const dimcv_params& p, v3 r) {
+  float ret = 1.0f;
+  float q0 = fabsf(r.x) * p.inv_support_radius;
+  float q1 = fabsf(r.y) * p.inv_support_radius;
+  float q2 = fabsf(r.z) * p.inv_support_radius;
+  if (q0 < 1.0f) ret *= 0.75f * (1.0f - q0 * q0); else ret *= 0.0f;
+  if (q1 < 1.0f) ret *= 0.75f * (1.0f - q1 * q1); else ret *= 0.0f;
+  if (q2 < 1.0f) ret *= 0.75f * (1.0f - q2 * q2); else ret *= 0.0f;
+  return ret;
+}
+
+// dimcv_sph.py:721-751; kden_sum partials in canonical tree order.
+__global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_params p, int32_t bound,
+                                                        double* __restrict__ partials) {
+  __shared__ double lds[256];
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
+    const int64_t i = st.fluid_list[f];
+    v3 xi = ld3(st.x, i);
+    if (in_vortex_domain(p, xi)) {
+      const int nn = min(st.nbr_count[f], p.nbr_kmax);
+      const uint32_t* __restrict__ nl = st.nbr + f;
+      const int64_t ns = p.n_max;
+      float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
+      for (int k = 0; k < nn; ++k) {
+        uint32_t e = nl[(int64_t)k * ns];
+        if (!nb_fluid(e)) continue;
+        int64_t j = nb_slot(e);
+        kden += st.kinematic_vort_num[j] * epan(p, vsub(xi, ld3(st.x, j)));
+      }
+      st.kernel_density[i] = kden;
+      contrib = (double)kden;
+    }
+  }
+  double s = block_tree_sum256(contrib, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// dimcv_sph.py:753-760
+__global__ void k_life_time(dimcv_state st, dimcv_params p, int32_t bound) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  if (!in_vortex_domain(p, ld3(st.x, i))) return;
+  st.life_time[i] += p.dt;
+}
+
+int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t s);
+
+}  // namespace dimcv
+
+using namespace dimcv;
+
+#define LAUNCH_FLUID(kern, ...)                                                                 \
+  do {                                                                                          \
+    if (fluid_bound > 0)                                                                        \
+      hipLaunchKernelGGL(kern, dim3(nblocks(fluid_bound)), dim3(256), 0, (hipStream_t)stream, \
+                         __VA_ARGS__);                                                          \
+    DIMCV_LAUNCHED(#kern);                                                                      \
+  } while (0)
+
+#define LAUNCH_SLOTS(kern, bound, ...)                                                          \
+  do {                                                                                          \
+    if ((bound) > 0)                                                                            \
+      hipLaunchKernelGGL(kern, dim3(nblocks(bound)), dim3(256), 0, (hipStream_t)stream,        \
+                         __VA_ARGS__);                                                          \
+    DIMCV_LAUNCHED(#kern);                                                                      \
+  } while (0)
+
+extern "C" {
+
+int dimcv_compute_densities(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_densities, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_dfsph_factor, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_multiply_time_step(const dimcv_state* st, const dimcv_params* p, float* field, float scale,
+                             int32_t fluid_bound, void* stream) {
+  (void)p;
+  LAUNCH_FLUID(k_multiply, *st, field, scale, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, float offset,
+                                int32_t fluid_bound, double* host_sum, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  unsigned nb = nblocks(fluid_bound);
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, fluid_bound, ws.partials);
+    DIMCV_LAUNCHED("k_residual");
+  }
+  rc = reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_RESIDUAL, stream);
+  if (rc) return rc;
+  if (host_sum) {
+    DIMCV_HIP(hipMemcpyAsync(host_sum, st->dscal + DIMCV_D_RESIDUAL, sizeof(double), hipMemcpyDeviceToHost, s));
+    DIMCV_HIP(hipStreamSynchronize(s));
+  }
+  return DIMCV_OK;
+}
+
+int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_div_iter, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_non_pressure_forces(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
+                                      int32_t fluid_bound, void* stream) {
+  LAUNCH_SLOTS(k_nonpressure_other, particle_bound, *st, *p, particle_bound);
+  LAUNCH_FLUID(k_nonpressure_fluid, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_predict_velocity(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
+  LAUNCH_SLOTS(k_predict, particle_bound, *st, *p, particle_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
+  LAUNCH_SLOTS(k_advect, particle_bound, *st, *p, particle_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  LAUNCH_FLUID(k_pressure_iter, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_copy_x_temp(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
+  (void)particle_bound;
+  LAUNCH_SLOTS(k_copy_x_temp, p->n_max, *st, p->n_max);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_vorticity_vis(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->vorticity_vis, 3);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_vorticity_vis, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_enforce_boundary_3d(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
+  LAUNCH_SLOTS(k_bounce, particle_bound, *st, *p, particle_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_v_star(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->v_star, 3);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_v_star, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->vorticity, 3);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_vort_gradv<1>, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_grad_v(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->grad_v, 9);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_vort_gradv<2>, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_dimcv_init(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_vort_gradv<3>, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_d_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->d_vort, 3);
+  if (rc) return rc;
+  LAUNCH_FLUID(k_d_vorticity, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_all_d_vort_smoothed(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                                      void* stream) {
+  LAUNCH_FLUID(k_d_vort_smoothed, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_compute_kvn_sample(const dimcv_state* st, const dimcv_params* p, const float* u1, const float* u2,
+                             int32_t do_bernoulli, int32_t fluid_bound, void* stream) {
+  if (do_bernoulli && (!u1 || !u2)) {
+    set_error("dimcv_compute_kvn_sample: uniforms required for the Bernoulli update");
+    return DIMCV_ERR_ARG;
+  }
+  LAUNCH_FLUID(k_kvn_sample, *st, *p, u1, u2, (int)do_bernoulli, fluid_bound);
+  return DIMCV_OK;
+}
+
+int dimcv_record_sample_idx(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  DIMCV_HIP(hipMemsetAsync(st->sample_idx, 0, sizeof(int32_t) * p->n_max, s));
+  return compact_samples(st, particle_bound, ws, s);
+}
+
+int dimcv_compute_kernel_density(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  unsigned nb = nblocks(fluid_bound);
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_kernel_density, dim3(nb), dim3(256), 0, (hipStream_t)stream, *st, *p, fluid_bound,
+                       ws.partials);
+    DIMCV_LAUNCHED("k_kernel_density");
+  }
+  return reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_KDEN_SUM, stream);
+}
+
+int dimcv_update_particle_life_time(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                                    void* stream) {
+  LAUNCH_FLUID(k_life_time, *st, *p, fluid_bound);
+  return DIMCV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// Library plumbing: error state, workspace layout, canonical FP64 reduction of
+// block partials, device-wide exclusive u32 scan.
+#include "common.h"
+
+namespace dimcv {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error("%s failed: %s", what, hipGetErrorString(e));
+  return DIMCV_ERR_HIP;
+}
+
+static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Workspace* ws,
+                         void* base) {
+  (void)n_cells;
+  char* b = (char*)base;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) -> void* {
+    void* q = b ? (void*)(b + off) : nullptr;
+    off += align256(bytes > 0 ? bytes : 1);
+    return q;
+  };
+  int64_t n = n_max > 0 ? n_max : 1;
+  int64_t sort_blocks = (n + DIMCV_SORT_TILE - 1) / DIMCV_SORT_TILE;
+  int64_t nb256 = (n + DIMCV_BLOCK - 1) / DIMCV_BLOCK;
+  int64_t scan_blocks = (n + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
+  Workspace w{};
+  w.keys[0] = (uint32_t*)take(4 * n);
+  w.keys[1] = (uint32_t*)take(4 * n);
+  w.vals[0] = (uint32_t*)take(4 * n);
+  w.vals[1] = (uint32_t*)take(4 * n);
+  w.radix_hist = (uint32_t*)take(4 * 256 * sort_blocks);
+  w.scan_tmp = (uint32_t*)take(4 * (256 * sort_blocks / 1024 + 1024));
+  w.compact_cnt = (uint32_t*)take(4 * (nb256 + 1024));
+  w.partials = (double*)take(8 * (nb256 + 256));
+  w.cdf = (double*)take(8 * n);
+  w.cdf_weights = (double*)take(8 * n);
+  w.block_tot = (double*)take(8 * (scan_blocks + 16));
+  w.block_pre = (double*)take(8 * (scan_blocks + 16));
+  w.near3 = (float*)take(4 * 3 * n);
+  w.scale_s = (float*)take(4 * n);
+  int64_t ms = max_samples > 0 ? max_samples : 1;
+  w.samples = (float4*)take(32 * ms);
+  w.max_samples = ms;
+  w.bytes = off;
+  if (ws) *ws = w;
+  return off;
+}
+
+int bind_workspace(const dimcv_state* st, const dimcv_params* p, Workspace* ws, int64_t max_samples) {
+  if (!st || !p) {
+    set_error("null state or params");
+    return DIMCV_ERR_ARG;
+  }
+  // The sample table is sized by whatever the caller's workspace has room for.
+  int64_t need0 = workspace_layout(p->n_max, p->n_cells, 1, nullptr, nullptr);
+  if (st->workspace == nullptr || st->workspace_bytes < need0) {
+    set_error("workspace too small: %lld < %lld bytes", (long long)st->workspace_bytes,
+              (long long)need0);
+    return DIMCV_ERR_WORKSPACE;
+  }
+  // the sample table is the last region; it starts at need0 - align256(32)
+  int64_t room = (st->workspace_bytes - (need0 - 256)) / 32;
+  workspace_layout(p->n_max, p->n_cells, room, ws, st->workspace);
+  if (max_samples > ws->max_samples) {
+    set_error("workspace holds %lld samples, %lld requested", (long long)ws->max_samples,
+              (long long)max_samples);
+    return DIMCV_ERR_WORKSPACE;
+  }
+  return DIMCV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Canonical FP64 reduction of an array of partials: repeated 256-wide halving
+// trees, one workgroup, level by level (same definition as the per-block
+// trees that produced the partials).
+__global__ void k_reduce_levels(double* a, int64_t n, double* out) {
+  __shared__ double lds[256];
+  int64_t m = n;
+  while (m > 1) {
+    int64_t groups = (m + 255) / 256;
+    for (int64_t g = 0; g < groups; ++g) {
+      int64_t i = g * 256 + threadIdx.x;
+      double v = i < m ? a[i] : 0.0;
+      double s = block_tree_sum256(v, lds);
+      if (threadIdx.x == 0) a[g] = s;
+      __syncthreads();
+    }
+    m = groups;
+  }
+  if (threadIdx.x == 0) *out = (n > 0) ? a[0] : 0.0;
+}
+
+int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream) {
+  hipLaunchKernelGGL(k_reduce_levels, dim3(1), dim3(256), 0, (hipStream_t)stream,
+                     const_cast<double*>(partials), n, out);
+  DIMCV_LAUNCHED("k_reduce_levels");
+  return DIMCV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive u32 scan: 1024 elements per workgroup (4 per thread), block sums
+// scanned by one workgroup, then added back.
+__device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* lds_w, uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) lds_w[w] = incl;
+  __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+    if (i < w) wpre += lds_w[i];
+    tot += lds_w[i];
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return wpre + incl - v;
+}
+
+// in may alias out (in-place scan): every thread reads its four values before
+// writing them, and no two threads touch the same element.
+__global__ void k_scan_u32_local(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t lw[4];
+  int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t a[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = (base + k < n) ? in[base + k] : 0u;
+    s += a[k];
+  }
+  uint32_t tot;
+  uint32_t pre = block_excl_scan_u32(s, lw, &tot);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = pre;
+    pre += a[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_u32_sums(uint32_t* bsum, int64_t nb, uint32_t* total) {
+  __shared__ uint32_t lw[4];
+  uint32_t carry = 0;
+  for (int64_t c = 0; c < nb; c += 256) {
+    int64_t i = c + threadIdx.x;
+    uint32_t v = i < nb ? bsum[i] : 0u;
+    uint32_t tot;
+    uint32_t pre = block_excl_scan_u32(v, lw, &tot);
+    if (i < nb) bsum[i] = carry + pre;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_u32_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ bsum) {
+  int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t add = bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, uint32_t* total,
+                       void* stream) {
+  if (n <= 0) return DIMCV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned nb = nblocks(n, 1024);
+  hipLaunchKernelGGL(k_scan_u32_local, dim3(nb), dim3(256), 0, s, in, out, n, tmp);
+  hipLaunchKernelGGL(k_scan_u32_sums, dim3(1), dim3(256), 0, s, tmp, (int64_t)nb, total);
+  hipLaunchKernelGGL(k_scan_u32_add, dim3(nb), dim3(256), 0, s, out, n, tmp);
+  DIMCV_LAUNCHED("exclusive_scan_u32");
+  return DIMCV_OK;
+}
+
+}  // namespace dimcv
+
+extern "C" {
+
+int dimcv_abi_version(void) { return DIMCV_ABI_VERSION; }
+
+const char* dimcv_last_error(void) { return dimcv::g_error.c_str(); }
+
+int64_t dimcv_workspace_bytes(int64_t n_max, int64_t n_cells, int64_t max_samples) {
+  return dimcv::workspace_layout(n_max, n_cells, max_samples, nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// Velocity recovery from vorticity (vort2vel, dimcv_sph.py:649-719), the
+// inverse-CDF importance sampler, the direct O(N^2) Biot-Savart comparator and
+// the vortex-ring initial condition.
+//
+// Far field.  For target i the reference sums, over samples s with
+// |x_i - x_s| > h,
+//     K(|r|) (r_xy x w_s) S_i / (kden_s + 1e-8),    K(r) = 1 / r^2 for r > h,
+// then scales by 0.1592 V / n_i (n_i = accepted samples).  The three modes:
+//   EXACT: the reference expression, IEEE div/sqrt, per pair (bit-exact);
+//   FAST : c_s = w_s / (kden_s + 1e-8) folded once per sample, S_i factored
+//          out of the sum, K from a hardware reciprocal, FMA accumulation.
+//          The acceptance test |r| > h is r^2 > T with T the largest f32 whose
+//          correctly rounded sqrt is <= h, and r^2 formed without FMA, so the
+//          accepted set (and n_i) equals the reference's exactly.
+//   KAHAN: FAST with Kahan-compensated f32 accumulators.
+// Samples are read with wave-uniform addresses (scalar loads): every lane of a
+// wave walks the same sample at the same time; no LDS staging is needed.
+#include "common.h"
+
+namespace dimcv {
+
+// ---- near field + S_i (one thread per fluid-list entry) --------------------
+__global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p, int32_t bound,
+                                                 float* __restrict__ near3, float* __restrict__ scale_s) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  const int nn = min(st.nbr_count[f], p.nbr_kmax);
+  const uint32_t* __restrict__ nl = st.nbr + f;
+  const int64_t ns = p.n_max;
+  v3 vv = mk(0.0f, 0.0f, 0.0f);
+  float kns = st.kernel_density[i];
+  for (int k = 0; k < nn; ++k) {
+    uint32_t e = nl[(int64_t)k * ns];
+    if (!nb_fluid(e)) continue;
+    int64_t j = nb_slot(e);
+    v3 r2 = vsub(xi, ld3(st.x, j));
+    v3 r = mk(r2.x, r2.y, 0.0f);
+    float rn = vnorm(r);
+    float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
+    vv = vadd(vv, smul(A, vcross(r, ld3(st.d_vort_smoothed, j))));
+    kns += st.kernel_density[j];
+  }
+  vv = vmuls(vv, p.vp_factor);
+  st3(near3, f, vv);
+  scale_s[f] = (float)(st.dscal[DIMCV_D_KDEN_SUM] - (double)kns);
+}
+
+// Final combination (dimcv_sph.py:711-719).
+__device__ __forceinline__ void vv_finish(dimcv_state& st, const dimcv_params& p, int64_t i, v3 near, v3 vs,
+                                          int added) {
+  if (added == 0) vs = mk(0.0f, 0.0f, 0.0f);
+  else vs = vmuls(vs, p.vp_factor / (float)added);
+  v3 vvort = vadd(vs, near);
+  if (st.v_vort) st3(st.v_vort, i, vvort);
+  v3 vi = ld3(st.v, i);
+  if (vnorm(vvort) < vnorm(vi)) st3(st.v, i, vadd(vi, vvort));
+}
+
+// EXACT far field: the reference expression per pair.
+__global__ void __launch_bounds__(256) k_vv_far_exact(dimcv_state st, dimcv_params p, int32_t bound,
+                                                      const float* __restrict__ near3,
+                                                      const float* __restrict__ scale_s) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  const float S = scale_s[f];
+  const int M = st.counters[DIMCV_C_NUM_SAMPLES];
+  v3 vs = mk(0.0f, 0.0f, 0.0f);
+  int added = 0;
+  for (int js = 0; js < M; ++js) {
+    const int64_t pj = st.sample_idx[js];
+    v3 r2 = vsub(xi, ld3(st.x, pj));
+    float rn = vnorm(r2);
+    if (rn > p.support_radius) {
+      v3 r = mk(r2.x, r2.y, 0.0f);
+      float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
+      v3 t = smul(A, vcross(r, ld3(st.d_vort_smoothed, pj)));
+      t = vmuls(t, S);
+      t = vdivs(t, st.kernel_density[pj] + 1e-8f);
+      vs = vadd(vs, t);
+      added += 1;
+    }
+  }
+  vv_finish(st, p, i, ld3(near3, f), vs, added);
+}
+
+// Pack far-field sources: (x, y, z, 0), (c_x, c_y, c_z, 0) with
+// c = w / (kden + 1e-8) (weighted = 1) or c = w (weighted = 0).
+__global__ void k_pack_sources(dimcv_state st, const int32_t* __restrict__ idx, const int32_t* __restrict__ count,
+                               int64_t cap, int weighted, float4* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cap || k >= *count) return;
+  int64_t s = idx[k];
+  v3 w = ld3(st.d_vort_smoothed, s);
+  if (weighted) {
+    float den = st.kernel_density[s] + 1e-8f;
+    w = mk(w.x / den, w.y / den, w.z / den);
+  }
+  out[2 * k] = make_float4(st.x[3 * s], st.x[3 * s + 1], st.x[3 * s + 2], 0.0f);
+  out[2 * k + 1] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+// FAST / KAHAN far field.  Sums sum_s 1/r^2 (r_xy x c_s) over accepted s.
+template <bool KAHAN>
+__device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v3 xi, float T, v3& acc, int& added) {
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
+  int n = 0;
+#pragma unroll 4
+  for (int js = 0; js < M; ++js) {
+    const float4 a = src[2 * js];
+    const float4 c = src[2 * js + 1];
+    float dx = xi.x - a.x, dy = xi.y - a.y, dz = xi.z - a.z;
+    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    bool ok = r2 > T;
+    float inv = ok ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+    n += ok ? 1 : 0;
+    float t1 = dy * inv, t2 = dx * inv;
+    if (KAHAN) {
+      float ex = t1 * c.z;
+      float ey = -(t2 * c.z);
+      float ez = fmaf(t2, c.y, -(t1 * c.x));
+      float y0 = ex - cx_, s0 = ax + y0;
+      cx_ = (s0 - ax) - y0;
+      ax = s0;
+      float y1 = ey - cy_, s1 = ay + y1;
+      cy_ = (s1 - ay) - y1;
+      ay = s1;
+      float y2 = ez - cz_, s2 = az + y2;
+      cz_ = (s2 - az) - y2;
+      az = s2;
+    } else {
+      ax = fmaf(t1, c.z, ax);
+      ay = fmaf(-t2, c.z, ay);
+      az = fmaf(t2, c.y, az);
+      az = fmaf(-t1, c.x, az);
+    }
+  }
+  acc = mk(ax, ay, az);
+  added = n;
+}
+
+template <bool KAHAN>
+__global__ void __launch_bounds__(256) k_vv_far_fast(dimcv_state st, dimcv_params p, int32_t bound,
+                                                     const float* __restrict__ near3,
+                                                     const float* __restrict__ scale_s,
+                                                     const float4* __restrict__ src, int cap) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
+  v3 acc;
+  int added;
+  far_sum<KAHAN>(src, M, xi, p.far_r2_max_rejected, acc, added);
+  const float S = scale_s[f];
+  vv_finish(st, p, i, ld3(near3, f), vmuls(acc, S), added);
+}
+
+// ---- direct O(N^2) Biot-Savart ---------------------------------------------
+template <bool KAHAN>
+__global__ void __launch_bounds__(256) k_direct_f32(dimcv_state st, dimcv_params p, int32_t bound,
+                                                    const float* __restrict__ near3,
+                                                    const float4* __restrict__ src,
+                                                    const int32_t* __restrict__ nsrc) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  v3 acc;
+  int added;
+  far_sum<KAHAN>(src, *nsrc, xi, p.far_r2_max_rejected, acc, added);
+  st3(st.v_vort, i, vadd(ld3(near3, f), vmuls(acc, p.vp_factor)));
+}
+
+// FP64 direct sum (near part with the 2-D norm, far part 1/|r|^2, same
+// acceptance sets as the f32 paths).
+__global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params p, int32_t bound,
+                                                    const float4* __restrict__ src,
+                                                    const int32_t* __restrict__ nsrc, double* __restrict__ out,
+                                                    double vp_factor_d) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  const double X = xi.x, Y = xi.y, Z = xi.z;
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+  const int nn = min(st.nbr_count[f], p.nbr_kmax);
+  for (int k = 0; k < nn; ++k) {
+    uint32_t e = st.nbr[(int64_t)k * p.n_max + f];
+    if (!nb_fluid(e)) continue;
+    int64_t j = nb_slot(e);
+    double rx = X - (double)st.x[3 * j], ry = Y - (double)st.x[3 * j + 1];
+    double rn = sqrt(rx * rx + ry * ry);
+    double ex = exp(-rn / 1e-6);
+    double A = (1.0 - ex) / (rn * (rn + 1e-6 * ex));
+    const float* w = st.d_vort_smoothed + 3 * j;
+    nx += A * (ry * (double)w[2]);
+    ny += A * (-rx * (double)w[2]);
+    nz += A * (rx * (double)w[1] - ry * (double)w[0]);
+  }
+  const int M = *nsrc;
+  const float T = p.far_r2_max_rejected;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int js = 0; js < M; ++js) {
+    const float4 a = src[2 * js];
+    const float4 c = src[2 * js + 1];
+    float dxf = xi.x - a.x, dyf = xi.y - a.y, dzf = xi.z - a.z;
+    float r2f = __fadd_rn(__fadd_rn(__fmul_rn(dxf, dxf), __fmul_rn(dyf, dyf)), __fmul_rn(dzf, dzf));
+    if (!(r2f > T)) continue;
+    double rx = X - (double)a.x, ry = Y - (double)a.y, rz = Z - (double)a.z;
+    double inv = 1.0 / (rx * rx + ry * ry + rz * rz);
+    fx += inv * (ry * (double)c.z);
+    fy += inv * (-rx * (double)c.z);
+    fz += inv * (rx * (double)c.y - ry * (double)c.x);
+  }
+  out[3 * i] = vp_factor_d * (nx + fx);
+  out[3 * i + 1] = vp_factor_d * (ny + fy);
+  out[3 * i + 2] = vp_factor_d * (nz + fz);
+}
+
+// ---- canonical FP64 inclusive scan (block-local part) -----------------------
+// 256 threads x 16 consecutive values: l0 serial per thread, l1 serial per 16
+// threads over thread totals, L2 serial over the 16 group totals;
+// L1[t] = L2[g-1] + l1[t], L[i] = L1[t-1] + l0[i].
+__device__ void block_local_scan_f64(const double* __restrict__ y, double* __restrict__ L, int64_t m,
+                                     double* tt, double* l1s, double* L2s, double* L1s) {
+  const int t = threadIdx.x;
+  const int nthreads = (int)((m + 15) / 16);
+  double l0[16];
+  double acc = 0.0;
+  if (t < nthreads) {
+    int64_t s = (int64_t)t * 16;
+    int64_t e = min(s + 16, m);
+    acc = y[s];
+    l0[0] = acc;
+    for (int64_t k = s + 1; k < e; ++k) {
+      acc = acc + y[k];
+      l0[k - s] = acc;
+    }
+    tt[t] = acc;
+  }
+  __syncthreads();
+  const int ngroups = (nthreads + 15) / 16;
+  if (t < ngroups) {
+    int s = t * 16, e = min(s + 16, nthreads);
+    double a = tt[s];
+    l1s[s] = a;
+    for (int k = s + 1; k < e; ++k) {
+      a = a + tt[k];
+      l1s[k] = a;
+    }
+    L2s[t] = a;  // group total, scanned serially below
+  }
+  __syncthreads();
+  if (t == 0) {
+    double a = L2s[0];
+    for (int g = 1; g < ngroups; ++g) {
+      a = a + L2s[g];
+      L2s[g] = a;
+    }
+  }
+  __syncthreads();
+  if (t < nthreads) {
+    int g = t / 16;
+    L1s[t] = g == 0 ? l1s[t] : L2s[g - 1] + l1s[t];
+  }
+  __syncthreads();
+  if (t < nthreads) {
+    int64_t s = (int64_t)t * 16;
+    int64_t e = min(s + 16, m);
+    for (int64_t k = s; k < e; ++k) L[k] = t == 0 ? l0[k - s] : L1s[t - 1] + l0[k - s];
+  }
+  __syncthreads();
+}
+
+// weights: in-domain fluid kden (fluid-list order), 0 otherwise
+__global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, double* __restrict__ w) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound) return;
+  double v = 0.0;
+  if (f < st.counters[DIMCV_C_FLUID_LIST]) {
+    const int64_t i = st.fluid_list[f];
+    if (in_vortex_domain(p, ld3(st.x, i))) v = (double)st.kernel_density[i];
+  }
+  w[f] = v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_blocks(const double* __restrict__ x, double* __restrict__ L,
+                                                     const int32_t* __restrict__ nptr, double* __restrict__ btot) {
+  __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
+  const int64_t n = *nptr;
+  const int64_t s = (int64_t)blockIdx.x * DIMCV_SCAN_TILE;
+  if (s >= n) return;  // uniform per block
+  const int64_t m = min((int64_t)DIMCV_SCAN_TILE, n - s);
+  block_local_scan_f64(x + s, L + s, m, tt, l1s, L2s, L1s);
+  if (threadIdx.x == 0) btot[blockIdx.x] = L[s + m - 1];
+}
+
+// Scan of the block totals (at most DIMCV_MAX_SCAN_BLOCKS of them: one block-
+// local scan), then out = P[b-1] + L for b > 0.
+__global__ void __launch_bounds__(256) k_scan_totals(const double* __restrict__ btot, double* __restrict__ bpre,
+                                                     const int32_t* __restrict__ nptr) {
+  __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
+  const int64_t n = *nptr;
+  const int64_t nb = (n + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
+  if (nb <= 0) return;
+  block_local_scan_f64(btot, bpre, nb, tt, l1s, L2s, L1s);
+}
+
+__global__ void k_scan_add(double* __restrict__ L, const double* __restrict__ bpre, const int32_t* __restrict__ nptr,
+                           double* __restrict__ total) {
+  const int64_t n = *nptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = i / DIMCV_SCAN_TILE;
+  if (b > 0) L[i] = bpre[b - 1] + L[i];
+  if (i == n - 1) *total = L[i];
+}
+
+__global__ void k_cdf_draw(dimcv_state st, const double* __restrict__ cdf, const double* __restrict__ u, int32_t m,
+                           int64_t zero_cap) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t F = st.counters[DIMCV_C_FLUID_LIST];
+  const double total = F > 0 ? cdf[F - 1] : 0.0;
+  const bool ok = F > 0 && total > 0.0;
+  if (k == 0) st.counters[DIMCV_C_NUM_SAMPLES] = ok ? m : 0;
+  if (k >= zero_cap) return;
+  if (k >= m || !ok) {
+    st.sample_idx[k] = 0;
+    return;
+  }
+  const double t = u[k] * total;
+  int64_t lo = 0, hi = F;  // first cdf > t
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > t) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= F) {  // first cdf >= total
+    int64_t a = 0, b = F;
+    while (a < b) {
+      int64_t mid = (a + b) >> 1;
+      if (cdf[mid] >= total) b = mid;
+      else a = mid + 1;
+    }
+    lo = a;
+  }
+  st.sample_idx[k] = st.fluid_list[lo];
+}
+
+// ---- vortex rings ------------------------------------------------------------
+__global__ void k_ring_velocity(int64_t npts, const double* __restrict__ pts, double inv_s2, int64_t n,
+                                const float* __restrict__ x, float* __restrict__ vel) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double X0 = x[3 * i], X1 = x[3 * i + 1], X2 = x[3 * i + 2];
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  for (int64_t k = 0; k < npts; ++k) {
+    const double* q = pts + 7 * k;
+    double rx = X0 - q[0], ry = X1 - q[1], rz = X2 - q[2];
+    double r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 <= 0.0) continue;
+    double core = 1.0 - exp(-r2 * inv_s2);
+    double f = q[6] * core / (r2 * sqrt(r2));
+    u0 += f * (q[4] * rz - q[5] * ry);
+    u1 += f * (q[5] * rx - q[3] * rz);
+    u2 += f * (q[3] * ry - q[4] * rx);
+  }
+  vel[3 * i] = (float)u0;
+  vel[3 * i + 1] = (float)u1;
+  vel[3 * i + 2] = (float)u2;
+}
+
+struct PredDomainFluid {
+  const int32_t* material;
+  const float* x;
+  const int32_t* n;
+  dimcv_params p;
+  __device__ bool operator()(int64_t i) const {
+    return i < *n && material[i] == 1 && in_vortex_domain(p, ld3(x, i));
+  }
+};
+
+template <class Pred>
+__global__ void __launch_bounds__(256) k_compact_count2(Pred pred, int64_t n, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t wsum[4];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool f = i < n && pred(i);
+  uint64_t b = __ballot(f);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+template <class Pred>
+__global__ void __launch_bounds__(256) k_compact_write2(Pred pred, int64_t n, const uint32_t* __restrict__ off,
+                                                        int32_t* __restrict__ out) {
+  __shared__ uint32_t wsum[4];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool f = i < n && pred(i);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t b = __ballot(f);
+  if (lane == 0) wsum[w] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t pre = off[blockIdx.x];
+  for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (f) out[pre + (uint32_t)__popcll(b & lt)] = (int32_t)i;
+}
+
+}  // namespace dimcv
+
+using namespace dimcv;
+
+extern "C" {
+
+int dimcv_cdf_sample(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
+                     int32_t fluid_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  if (m < 0 || m > p->n_max) {
+    set_error("dimcv_cdf_sample: %d samples outside [0, n_max]", m);
+    return DIMCV_ERR_RANGE;
+  }
+  if ((int64_t)fluid_bound > (int64_t)DIMCV_SCAN_TILE * DIMCV_MAX_SCAN_BLOCKS) {
+    set_error("dimcv_cdf_sample: %d fluid particles exceed the scan limit", fluid_bound);
+    return DIMCV_ERR_RANGE;
+  }
+  const int32_t* nF = st->counters + DIMCV_C_FLUID_LIST;
+  if (fluid_bound > 0) {
+    hipLaunchKernelGGL(k_cdf_weights, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
+                       ws.cdf_weights);
+    unsigned nb = nblocks(fluid_bound, DIMCV_SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, ws.cdf_weights, ws.cdf, nF, ws.block_tot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, s, ws.block_tot, ws.block_pre, nF);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(fluid_bound)), dim3(256), 0, s, ws.cdf, ws.block_pre, nF,
+                       st->dscal + DIMCV_D_CDF_TOTAL);
+    DIMCV_LAUNCHED("canonical scan");
+  }
+  int64_t cap = p->n_max;
+  hipLaunchKernelGGL(k_cdf_draw, dim3(nblocks(cap)), dim3(256), 0, s, *st, ws.cdf, u, m, cap);
+  DIMCV_LAUNCHED("k_cdf_draw");
+  return DIMCV_OK;
+}
+
+int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  if (fluid_bound <= 0) return DIMCV_OK;
+  hipLaunchKernelGGL(k_vv_near, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
+                     ws.scale_s);
+  DIMCV_LAUNCHED("k_vv_near");
+  return DIMCV_OK;
+}
+
+int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, int32_t max_samples,
+                       void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : max_samples);
+  if (rc) return rc;
+  if (fluid_bound <= 0) return DIMCV_OK;
+  if (p->far_mode != DIMCV_FAR_EXACT && max_samples < 1) {
+    set_error("dimcv_vort2vel: max_samples must bound num_samples (use n_max)");
+    return DIMCV_ERR_ARG;
+  }
+  unsigned nb = nblocks(fluid_bound);
+  if (p->far_mode == DIMCV_FAR_EXACT) {
+    hipLaunchKernelGGL(k_vv_far_exact, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s);
+  } else {
+    hipLaunchKernelGGL(k_pack_sources, dim3(nblocks(max_samples)), dim3(256), 0, s, *st, st->sample_idx,
+                       st->counters + DIMCV_C_NUM_SAMPLES, (int64_t)max_samples, 1, ws.samples);
+    if (p->far_mode == DIMCV_FAR_KAHAN)
+      hipLaunchKernelGGL(k_vv_far_fast<true>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
+                         ws.scale_s, ws.samples, (int)max_samples);
+    else
+      hipLaunchKernelGGL(k_vv_far_fast<false>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
+                         ws.scale_s, ws.samples, (int)max_samples);
+  }
+  DIMCV_LAUNCHED("vort2vel far field");
+  return DIMCV_OK;
+}
+
+int dimcv_vort2vel(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, int32_t max_samples,
+                   void* stream) {
+  int rc = dimcv_vort2vel_near(st, p, fluid_bound, stream);
+  if (rc) return rc;
+  return dimcv_vort2vel_far(st, p, fluid_bound, max_samples, stream);
+}
+
+int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precision, double* out_f64,
+                    int32_t fluid_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, p->n_max);
+  if (rc) return rc;
+  if (precision < 0 || precision > 2 || (precision == 2 && !out_f64) || (precision < 2 && !st->v_vort)) {
+    set_error("dimcv_direct_bs: bad precision/output");
+    return DIMCV_ERR_ARG;
+  }
+  if (fluid_bound <= 0) return DIMCV_OK;
+  // sources: in-domain fluid slots, ascending
+  int64_t n = p->n_max;
+  unsigned nb256 = nblocks(n);
+  int32_t* nsrc = st->counters + DIMCV_C_SAMPLE_SLOTS;
+  int32_t* list = (int32_t*)ws.keys[0];
+  PredDomainFluid pr{st->material, st->x, st->counters + DIMCV_C_PARTICLE_NUM, *p};
+  hipLaunchKernelGGL(k_compact_count2<PredDomainFluid>, dim3(nb256), dim3(256), 0, s, pr, n, ws.compact_cnt);
+  rc = exclusive_scan_u32(ws.compact_cnt, ws.compact_cnt, nb256, ws.scan_tmp, (uint32_t*)nsrc, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compact_write2<PredDomainFluid>, dim3(nb256), dim3(256), 0, s, pr, n, ws.compact_cnt, list);
+  hipLaunchKernelGGL(k_pack_sources, dim3(nb256), dim3(256), 0, s, *st, list, nsrc, n, 0, ws.samples);
+  DIMCV_LAUNCHED("direct sources");
+  unsigned nb = nblocks(fluid_bound);
+  if (precision == 2) {
+    hipLaunchKernelGGL(k_direct_f64, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.samples, nsrc, out_f64,
+                       p->vp_factor_f64);
+  } else {
+    hipLaunchKernelGGL(k_vv_near, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s);
+    if (precision == 1)
+      hipLaunchKernelGGL(k_direct_f32<true>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.samples,
+                         nsrc);
+    else
+      hipLaunchKernelGGL(k_direct_f32<false>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
+                         ws.samples, nsrc);
+  }
+  DIMCV_LAUNCHED("direct Biot-Savart");
+  return DIMCV_OK;
+}
+
+int dimcv_ring_velocity(int64_t npts, const double* pts, double inv_sigma2, int64_t n, const float* x, float* vel,
+                        void* stream) {
+  if (n <= 0) return DIMCV_OK;
+  hipLaunchKernelGGL(k_ring_velocity, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, npts, pts, inv_sigma2,
+                     n, x, vel);
+  DIMCV_LAUNCHED("k_ring_velocity");
+  return DIMCV_OK;
+}
+
+}  // extern "C"

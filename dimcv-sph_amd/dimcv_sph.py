@@ -1,0 +1,349 @@
+"""DIMCV-SPH solver (drop-in for dimcv_sph.py of the reference).
+
+``DIMCVSPHSolver`` keeps the reference's constructor, fields and method names
+(dimcv_sph.py:9-819).  Each former ``@ti.kernel`` method launches the matching
+libdimcv entry point; the host-side logic (Jacobi loops with their
+convergence tests, emission, the ``dimcv()`` pass order) is the reference's.
+
+Build-defined behaviour, beyond the reference (each optional, see
+config_builder.py for the keys):
+  * ``ti.random`` is replaced by host-fed uniforms (``u1``/``u2`` per slot)
+    drawn from a seeded numpy generator, or passed explicitly to ``step``;
+  * with ``mcSamples`` = M the Bernoulli birth/death sample set is replaced by
+    M inverse-CDF draws from the normalised AKDE density (FP64 CDF);
+  * residual sums and ``kden_sum`` are FP64 (fixed reduction order);
+  * ``farField`` selects the far-field evaluation of vort2vel: "exact"
+    (reference expression, bit-exact), "fast" or "kahan".
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import _native as N
+from _field import CounterField, DeviceScalarField, Field
+from sph_base import SPHBase
+
+
+class DIMCVSPHSolver(SPHBase):
+
+    def __init__(self, particle_system):
+        super().__init__(particle_system)
+        ps = self.ps
+        self.surface_tension = 0
+        self.dt[None] = ps.cfg.get_cfg("timeStepSize")
+
+        self.eps = 1e-6
+        self.inv_eps = 1.0 / self.eps
+        self.volume_scale = ps.cfg.get_cfg("volumeScale")
+        if self.volume_scale is not None:
+            self.vp_volume = self.volume_scale * ps.m_V0
+
+        self.m_max_iterations_v = 100
+        self.m_max_iterations = 100
+        self.m_eps = 1e-5
+        self.max_error_V = 0.1
+        self.max_error = 0.05
+        self.inv_dt = 1.0 / self.dt[None]
+        self.inv_dt2 = 1.0 / (self.dt[None] * self.dt[None])
+
+        n = int(ps.particle_max_num)
+        dev = ps.device
+        self.d_vort = Field(n, 3, torch.float32, dev)
+        self.d_vort_smoothed = Field(n, 3, torch.float32, dev)
+        self.v_star = Field(n, 3, torch.float32, dev)
+        self.grad_v = Field(n, 9, torch.float32, dev, view_shape=(3, 3))
+        self.x_temp = Field(n, 3, torch.float32, dev)
+        self.num_samples = CounterField(ps._counters, N.C_NUM_SAMPLES)
+        self.sample_idx = Field(n, 0, torch.int32, dev)
+        self.kinematic_vort_num = Field(n, 0, torch.float32, dev)
+        self.kernel_density = Field(n, 0, torch.float32, dev)
+        self.kden_sum = DeviceScalarField(ps._dscal, N.D_KDEN_SUM)
+        self.v_vort = Field(n, 3, torch.float32, dev)   # recovered velocity (analysis)
+        self.vortex_enforcing_domain_start = ps.cfg.get_cfg("vortexEnforcingDomainStart")
+        self.vortex_enforcing_domain_end = ps.cfg.get_cfg("vortexEnforcingDomainEnd")
+        self.a = ps.cfg.get_cfg("GenProbPara")
+        self.b = ps.cfg.get_cfg("DelProbPara")
+        num_samples = ps.cfg.get_cfg("numSamples")
+        if num_samples:
+            self.num_samples[None] = num_samples
+        self.divergence_transit_area = ps.cfg.get_cfg("divergenceTransitArea")
+
+        self.cnt = 0
+        self.emit_interval = 1
+        for emitter in ps.fluid_emitters:
+            init_v = np.array(emitter["velocity"])
+            self.emit_interval = np.ceil(
+                ps.particle_diameter / (np.linalg.norm(init_v) * self.dt[None])).astype(int)
+            break
+
+        # build extensions
+        self.mc_samples = ps.cfg.get_cfg("mcSamples")
+        far = ps.cfg.get_cfg("farField") or ("fast" if self.mc_samples else "exact")
+        self.far_mode = {"exact": N.FAR_EXACT, "fast": N.FAR_FAST, "kahan": N.FAR_KAHAN}[far]
+        seed = ps.cfg.get_cfg("rngSeed")
+        self.rng = np.random.default_rng(0 if seed is None else seed)
+        self._uniforms = (None, None, None)
+        self.last_iterations_v = 0
+        self.last_iterations_p = 0
+
+        p = ps.params
+        f32 = np.float32
+        p.dt = f32(self.dt[None])
+        if self.vortex_enforcing_domain_start is not None:
+            p.has_vortex_domain = 1
+            for d in range(3):
+                p.vortex_start[d] = f32(self.vortex_enforcing_domain_start[d])
+                p.vortex_end[d] = f32(self.vortex_enforcing_domain_end[d])
+        if self.divergence_transit_area is not None:
+            p.has_transit = 1
+            p.transit_a = f32(self.divergence_transit_area[0])
+            p.transit_ba = f32(self.divergence_transit_area[1] - self.divergence_transit_area[0])
+        if self.volume_scale is not None:
+            p.vp_factor = f32(0.1592 * self.vp_volume)
+            p.vp_factor_f64 = 0.1592 * self.vp_volume
+        p.neg_gen_a = f32(-(self.a if self.a is not None else 0))
+        p.neg_del_b = f32(-(self.b if self.b is not None else 0))
+        p.far_mode = self.far_mode
+        ps.solver = self
+        ps.invalidate_state()
+
+    # ------------------------------------------------------------ helpers --
+    def _fluid(self, name, *args):
+        self.ps.ensure_neighbor_lists()
+        self.ps.call(name, *args, self.ps._nfl, self.ps.stream())
+
+    def _slots(self, name, *args):
+        self.ps.call(name, *args, self.ps._pn, self.ps.stream())
+
+    def _fluid_count(self) -> int:
+        return int(self.ps._host_counters[N.C_FLUID_NUM])
+
+    def is_in_vortex_domain(self, pos) -> bool:
+        if self.vortex_enforcing_domain_start is None:
+            return True
+        for i in range(self.ps.dim):
+            if pos[i] < np.float32(self.vortex_enforcing_domain_start[i]) or \
+                    pos[i] > np.float32(self.vortex_enforcing_domain_end[i]):
+                return False
+        return True
+
+    # --------------------------------------------------- DFSPH (dimcv_sph.py) --
+    def compute_DFSPH_factor(self):
+        self._fluid("dimcv_compute_DFSPH_factor")
+
+    def compute_densities(self):
+        self._fluid("dimcv_compute_densities")
+
+    def compute_non_pressure_forces(self):
+        self.ps.ensure_neighbor_lists()
+        self.ps.call("dimcv_compute_non_pressure_forces", self.ps._pn, self.ps._nfl, self.ps.stream())
+
+    def advect(self):
+        self._slots("dimcv_advect")
+        self.ps._nbr_valid = False
+
+    def predict_velocity(self):
+        self._slots("dimcv_predict_velocity")
+
+    def compute_density_change(self):
+        self._fluid("dimcv_compute_density_change")
+
+    def multiply_time_step(self, field, time_step: float):
+        self.ps.call("dimcv_multiply_time_step", field.ptr(), float(time_step), self.ps._nfl,
+                     self.ps.stream())
+
+    def compute_density_error(self, offset: float) -> float:
+        out = np.zeros(1, np.float64)
+        self.ps.call("dimcv_compute_density_error", float(offset), self.ps._nfl, out.ctypes.data,
+                     self.ps.stream())
+        return float(out[0])
+
+    def divergence_solver_iteration_kernel(self):
+        self._fluid("dimcv_divergence_solver_iteration")
+
+    def divergence_solver_iteration(self):
+        self.divergence_solver_iteration_kernel()
+        self.compute_density_change()
+        density_err = self.compute_density_error(0.0)
+        return density_err / self._fluid_count()
+
+    def divergence_solve(self):
+        self.compute_density_change()
+        self.multiply_time_step(self.ps.dfsph_factor, self.inv_dt)
+        m_iterations_v = 0
+        avg_density_err = 0.0
+        while m_iterations_v < 1 or m_iterations_v < self.m_max_iterations_v:
+            avg_density_err = self.divergence_solver_iteration()
+            eta = self.inv_dt * self.max_error_V * 0.01 * self.density_0
+            m_iterations_v += 1
+            if avg_density_err <= eta:
+                break
+        self.last_iterations_v = m_iterations_v
+        self.multiply_time_step(self.ps.dfsph_factor, self.dt[None])
+
+    def compute_density_adv(self):
+        self._fluid("dimcv_compute_density_adv")
+
+    def pressure_solve_iteration_kernel(self):
+        self._fluid("dimcv_pressure_solve_iteration")
+
+    def pressure_solve_iteration(self):
+        self.pressure_solve_iteration_kernel()
+        self.compute_density_adv()
+        density_err = self.compute_density_error(self.density_0)
+        return density_err / self._fluid_count()
+
+    def pressure_solve(self):
+        inv_dt2 = self.inv_dt2
+        self.compute_density_adv()
+        self.multiply_time_step(self.ps.dfsph_factor, inv_dt2)
+        m_iterations = 0
+        avg_density_err = 0.0
+        while m_iterations < 1 or m_iterations < self.m_max_iterations:
+            avg_density_err = self.pressure_solve_iteration()
+            eta = self.max_error * 0.01 * self.density_0
+            m_iterations += 1
+            if avg_density_err <= eta:
+                break
+        self.last_iterations_p = m_iterations
+
+    # ------------------------------------------------------- emission --
+    def emit_particle(self):
+        ps = self.ps
+        for emitter in ps.fluid_emitters:
+            obj_id = emitter["objectId"]
+            square_center = np.array(emitter["squareCenter"])
+            square_size = np.array(emitter["squareSize"])
+            init_v = np.array(emitter["velocity"])
+            lower_corner = square_center - 0.5 * square_size
+            cube_size = square_size + ps.particle_radius * np.abs(init_v) / np.linalg.norm(init_v)
+            ps.object_collection[obj_id]["particleNum"] += ps.compute_cube_particle_num(
+                lower_corner, lower_corner + cube_size)
+            ps.add_cube(object_id=obj_id, lower_corner=lower_corner, cube_size=cube_size,
+                        velocity=init_v, density=emitter["density"], is_dynamic=1,
+                        color=emitter["color"], material=1)
+
+    def dump_num_particles_each_emitters_ti2np(self):
+        if not self.ps.obj_id_emitters:
+            return
+        counts = self.ps.num_particles_each_emitter.to_numpy()
+        for obj_id in self.ps.obj_id_emitters:
+            if 0 <= obj_id < counts.shape[0]:
+                self.ps.object_collection[obj_id]["particleNum"] = int(counts[obj_id])
+
+    def dump_num_particles_each_emitters_np2ti(self):
+        if not self.ps.obj_id_emitters:
+            return
+        counts = self.ps.num_particles_each_emitter.to_numpy()
+        for obj_id in self.ps.obj_id_emitters:
+            if 0 <= obj_id < counts.shape[0]:
+                counts[obj_id] = self.ps.object_collection[obj_id]["particleNum"]
+        self.ps.num_particles_each_emitter.from_numpy(counts)
+
+    # ------------------------------------------------------ DIMCV passes --
+    def compute_v_star(self):
+        self._fluid("dimcv_compute_v_star")
+
+    def compute_grad_v(self):
+        self._fluid("dimcv_compute_grad_v")
+
+    def compute_vorticity(self):
+        self._fluid("dimcv_compute_vorticity")
+
+    def compute_d_vorticity(self):
+        self._fluid("dimcv_compute_d_vorticity")
+
+    def compute_all_d_vort_smoothed(self):
+        self._fluid("dimcv_compute_all_d_vort_smoothed")
+
+    def _upload(self, a, dtype):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.ps.device)
+
+    def compute_kinematic_vorticity_num_sample(self, u1=None, u2=None):
+        """KVN + Bernoulli sample update; u1/u2 replace the two ti.random() draws."""
+        if self.mc_samples:
+            self._fluid("dimcv_compute_kvn_sample", None, None, 0)
+            return
+        pn = self.ps._pn
+        if u1 is None:
+            u1 = self.rng.random(pn, dtype=np.float32)
+        if u2 is None:
+            u2 = self.rng.random(pn, dtype=np.float32)
+        assert len(u1) >= pn and len(u2) >= pn
+        t1 = self._upload(u1, torch.float32)
+        t2 = self._upload(u2, torch.float32)
+        self._fluid("dimcv_compute_kvn_sample", t1.data_ptr(), t2.data_ptr(), 1)
+
+    def record_vort_part_sample_idx(self):
+        self._slots("dimcv_record_sample_idx")
+
+    def compute_kernel_density(self):
+        self._fluid("dimcv_compute_kernel_density")
+
+    def importance_sample(self, u=None, m=None):
+        """Build-defined: M inverse-CDF draws from kden / kden_sum (FP64 CDF)."""
+        m = int(self.mc_samples if m is None else m)
+        if u is None:
+            u = self.rng.random(m)
+        assert len(u) >= m
+        tu = self._upload(u, torch.float64)
+        self.ps.call("dimcv_cdf_sample", tu.data_ptr(), m, self.ps._nfl, self.ps.stream())
+
+    def vort2vel(self):
+        self.ps.ensure_neighbor_lists()
+        self.ps.call("dimcv_vort2vel_near", self.ps._nfl, self.ps.stream())
+        self.vort2vel_far()
+
+    def vort2vel_far(self):
+        """Far-field half of vort2vel (dimcv_sph.py:693-719): the O(N M) sum."""
+        self.ps.call("dimcv_vort2vel_far", self.ps._nfl, int(self.ps.particle_max_num), self.ps.stream())
+
+    def update_particle_life_time(self):
+        self._fluid("dimcv_update_particle_life_time")
+
+    def compute_kernel_sum(self):
+        raise NotImplementedError("compute_kernel_sum is unused by the reference step")
+
+    def compute_vorticity_vis(self):
+        self._fluid("dimcv_compute_vorticity_vis")
+
+    def copy_x_temp(self):
+        self._slots("dimcv_copy_x_temp")
+
+    def init(self):
+        # compute_v_star + compute_vorticity + compute_grad_v as one neighbour pass
+        self._fluid("dimcv_dimcv_init")
+
+    def update_sample(self, u1=None, u2=None):
+        self.compute_kinematic_vorticity_num_sample(u1, u2)
+        if not self.mc_samples:
+            self.record_vort_part_sample_idx()
+
+    def dimcv(self):
+        u1, u2, u_cdf = self._uniforms
+        self.init()
+        self.compute_d_vorticity()
+        self.compute_all_d_vort_smoothed()
+        self.update_sample(u1, u2)
+        self.compute_kernel_density()
+        if self.mc_samples:
+            self.importance_sample(u_cdf)
+        self.vort2vel()
+        self.update_particle_life_time()
+
+    # --------------------------------------------------- comparators --
+    def direct_biot_savart(self, precision: str = "f32"):
+        """Build-defined direct O(N^2) sum of the quantity vort2vel estimates.
+
+        precision "f32" / "kahan" write ``v_vort`` (returned as numpy);
+        "f64" returns an FP64 (n_max, 3) array.  ``v`` is not modified.
+        """
+        code = {"f32": 0, "kahan": 1, "f64": 2}[precision]
+        self.ps.ensure_neighbor_lists()
+        out = None
+        if code == 2:
+            out = torch.zeros((int(self.ps.particle_max_num), 3), dtype=torch.float64, device=self.ps.device)
+        self.ps.call("dimcv_direct_bs", code, out.data_ptr() if out is not None else None,
+                     self.ps._nfl, self.ps.stream())
+        return out.cpu().numpy() if out is not None else self.v_vort.to_numpy()

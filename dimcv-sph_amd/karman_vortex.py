@@ -1,0 +1,129 @@
+"""Kármán vortex scene solver (drop-in for karman_vortex.py of the reference).
+
+``KarmanVortexSolver`` keeps the cylinder carving (karman_vortex.py:10-36),
+the step/substep order (:216-242) and the exporters (:38-214).  ``step``
+accepts optional host uniforms so that two implementations can be driven with
+identical random inputs (``u1``/``u2``: per-slot f32 arrays for the Bernoulli
+sample update; ``u_cdf``: f64 array of M uniforms for inverse-CDF sampling).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from dimcv_sph import DIMCVSPHSolver
+
+
+class KarmanVortexSolver(DIMCVSPHSolver):
+
+    def __init__(self, particle_system):
+        super().__init__(particle_system)
+        self.circle_pos = np.array([0.65, 0.5, 0.5], dtype=np.float32)
+        self.circle_vis = np.array([self.circle_pos[0] * 0.25, self.circle_pos[1], self.circle_pos[2]],
+                                   dtype=np.float32)
+        self.circle_radius = 0.1
+        self.init_cylinder()
+
+    def init_cylinder(self):
+        self.ps.call("dimcv_init_cylinder", 2, np.float32(self.circle_pos[0]), np.float32(self.circle_pos[1]),
+                     np.float32(self.circle_radius), self.ps.stream())
+        self.ps._nbr_valid = False
+
+    def export_png(self, cnt, image_path):
+        import matplotlib
+        matplotlib.use("Agg")
+        import matplotlib.pyplot as plt
+        from matplotlib.colors import Normalize
+        N = self.ps.particle_num[None]
+        material = self.ps.material.to_numpy()[:N]
+        obj_id = self.ps.object_id.to_numpy()[:N]
+        fluid_mask = material == self.ps.material_fluid
+        solid_mask = obj_id == 2
+        x = self.x_temp.to_numpy()[:N]
+        vort = self.ps.vorticity_vis.to_numpy()[:N][:, 2]
+        fluid_x, fluid_vort, solid_x = x[fluid_mask], vort[fluid_mask], x[solid_mask]
+        fig = plt.figure(figsize=(10, 4), dpi=200)
+        ax = fig.add_subplot(111, projection="3d")
+        ax.view_init(elev=30, azim=-60)
+        ax.scatter(fluid_x[:, 0], fluid_x[:, 1], fluid_x[:, 2], c=fluid_vort, cmap="coolwarm", s=0.2,
+                   norm=Normalize(vmin=-40, vmax=40), edgecolors="none", alpha=0.6)
+        ax.scatter(solid_x[:, 0], solid_x[:, 1], solid_x[:, 2], color="#00C853", s=0.5, edgecolors="none")
+        ax.set_xlim(0, 4)
+        ax.set_ylim(0, 1)
+        ax.set_zlim(0, 1)
+        ax.set_box_aspect((4, 1, 1))
+        ax.set_axis_off()
+        plt.subplots_adjust(left=0, right=1, top=1, bottom=0)
+        plt.savefig(image_path / f"vorticity_{cnt:04}.png", bbox_inches="tight", pad_inches=0,
+                    transparent=True, dpi=400)
+        plt.cla()
+        plt.close("all")
+
+    def export_ply(self, cnt, ply_path):
+        """ASCII PLY with the reference schema (karman_vortex.py:122-214)."""
+        N = self.ps.particle_num[None]
+        if N == 0:
+            return
+        x = self.x_temp.to_numpy()[:N]
+        vorticity = self.ps.vorticity.to_numpy()[:N]
+        color = self.ps.color.to_numpy()[:N]
+        obj_id = self.ps.object_id.to_numpy()[:N]
+        is_active = self.ps.is_active.to_numpy()[:N].astype(bool)
+        x, vorticity, color, obj_id = x[is_active], vorticity[is_active], color[is_active], obj_id[is_active]
+        if x.shape[0] == 0:
+            return
+        nv = x.shape[0]
+        ptype = np.full(nv, 4, dtype=np.uint8)
+
+        def mask(ids):
+            ids = list(ids)
+            if not ids:
+                return np.zeros_like(obj_id, dtype=bool)
+            return np.isin(obj_id, np.array(ids, dtype=obj_id.dtype))
+
+        ptype[mask(self.ps.obj_id_fluid_blocks)] = 0
+        ptype[mask(self.ps.obj_id_emitters)] = 1
+        ptype[mask(self.ps.obj_id_rigid_blocks)] = 2
+        ptype[mask(self.ps.object_id_rigid_body)] = 3
+        ply_path.mkdir(parents=True, exist_ok=True)
+        with open(ply_path / f"frame_{cnt:04}.ply", "w") as f:
+            f.write("ply\nformat ascii 1.0\n")
+            f.write(f"element vertex {nv}\n")
+            for nm in ("x", "y", "z", "vort_x", "vort_y", "vort_z"):
+                f.write(f"property float {nm}\n")
+            for nm in ("red", "green", "blue"):
+                f.write(f"property uchar {nm}\n")
+            f.write("property int object_id\nproperty uchar particle_type\nend_header\n")
+            for i in range(nv):
+                px, py, pz = x[i]
+                vx, vy, vz = vorticity[i]
+                r, g, b = color[i]
+                f.write(f"{px:.7f} {py:.7f} {pz:.7f} {vx:.7f} {vy:.7f} {vz:.7f} "
+                        f"{int(r)} {int(g)} {int(b)} {int(obj_id[i])} {int(ptype[i])}\n")
+
+    def step(self, u1=None, u2=None, u_cdf=None):
+        self.dump_num_particles_each_emitters_ti2np()
+        if self.cnt % self.emit_interval == 0:
+            self.emit_particle()
+            self.dump_num_particles_each_emitters_np2ti()
+        self.cnt += 1
+        self.ps.initialize_particle_system()
+        self.compute_moving_boundary_volume()
+        self._uniforms = (u1, u2, u_cdf)
+        try:
+            self.substep()
+        finally:
+            self._uniforms = (None, None, None)
+        if self.ps.cfg.get_cfg("bounceBackBoundary"):
+            self.enforce_boundary_3D(self.ps.material_fluid)
+
+    def substep(self):
+        self.compute_densities()
+        self.compute_DFSPH_factor()
+        self.divergence_solve()
+        self.compute_non_pressure_forces()
+        self.dimcv()
+        self.predict_velocity()
+        self.pressure_solve()
+        self.copy_x_temp()
+        self.compute_vorticity_vis()
+        self.advect()

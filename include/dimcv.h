@@ -1,0 +1,317 @@
+/* dimcv.h -- C ABI of libdimcv.so, the gfx950 implementation of the DIMCV-SPH
+ * particle step.
+ *
+ * The reference has no FFI: its hot path is a set of Taichi @ti.kernel methods
+ * on Python classes.  Each entry point below replaces one of those kernels (or
+ * a fixed sequence of them); the comment names the reference lines.  The
+ * Python drop-in classes (particle_system.py, sph_base.py, dimcv_sph.py,
+ * karman_vortex.py under dimcv-sph_amd/) call these entries through ctypes.
+ *
+ * Conventions
+ *  - All pointers in dimcv_state are device pointers owned by the caller
+ *    (PyTorch tensors).  The library never allocates device memory; scratch
+ *    space is the caller-provided workspace of dimcv_workspace_bytes() bytes.
+ *  - `stream` is a hipStream_t passed as void*; every launch is asynchronous on
+ *    it.  Entries that return a host value (documented) synchronise the stream.
+ *  - Return value: 0 on success, a negative dimcv_status on failure; the
+ *    message is available from dimcv_last_error().  No C++ exception crosses
+ *    the ABI.
+ *  - Per-particle vectors are packed float[3] (x, y, z), grad_v is a row-major
+ *    float[9], colour is int32[3]; bool fields are uint8.
+ *  - Device-resident scalars: counters[] (int32, dimcv_counter) and dscal[]
+ *    (double, dimcv_dscalar).
+ */
+#ifndef DIMCV_H
+#define DIMCV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIMCV_ABI_VERSION 1
+
+typedef enum dimcv_status {
+  DIMCV_OK = 0,
+  DIMCV_ERR_ARG = -1,        /* invalid argument / shape */
+  DIMCV_ERR_HIP = -2,        /* HIP runtime error */
+  DIMCV_ERR_WORKSPACE = -3,  /* workspace too small */
+  DIMCV_ERR_KEY = -4,        /* active particle outside the cell grid */
+  DIMCV_ERR_NBR = -5,        /* neighbour list width exceeded */
+  DIMCV_ERR_RANGE = -6       /* size beyond a documented maximum */
+} dimcv_status;
+
+/* Indices into state->counters (int32). */
+typedef enum dimcv_counter {
+  DIMCV_C_PARTICLE_NUM = 0,   /* ps.particle_num            particle_system.py:48 */
+  DIMCV_C_FLUID_NUM = 1,      /* ps.fluid_particle_num      :113 */
+  DIMCV_C_SOLID_NUM = 2,      /* ps.solid_particle_num      :115 */
+  DIMCV_C_NUM_SAMPLES = 3,    /* solver.num_samples         dimcv_sph.py:46 */
+  DIMCV_C_FLUID_LIST = 4,     /* entries in state->fluid_list (= fluid slots) */
+  DIMCV_C_NBR_MAX = 5,        /* largest neighbour count seen by the last build */
+  DIMCV_C_KEY_ERROR = 6,      /* active particles with an out-of-grid cell key */
+  DIMCV_C_PREV_NUM = 7,       /* particle count before the last activity update */
+  DIMCV_C_SAMPLE_SLOTS = 8,   /* is_sample count (compaction scratch) */
+  DIMCV_C_COUNT = 16
+} dimcv_counter;
+
+/* Indices into state->dscal (double). */
+typedef enum dimcv_dscalar {
+  DIMCV_D_KDEN_SUM = 0,   /* solver.kden_sum (FP64)              dimcv_sph.py:52 */
+  DIMCV_D_RESIDUAL = 1,   /* last compute_density_error sum      :407-419 */
+  DIMCV_D_CDF_TOTAL = 2,  /* total weight of the importance CDF */
+  DIMCV_D_COUNT = 8
+} dimcv_dscalar;
+
+/* Velocity-recovery far-field evaluation (vort2vel, dimcv_sph.py:693-714). */
+typedef enum dimcv_far_mode {
+  DIMCV_FAR_EXACT = 0,  /* reference expression order, IEEE div/sqrt: bit-exact */
+  DIMCV_FAR_FAST = 1,   /* per-sample weights folded, reciprocal-based, FP32 */
+  DIMCV_FAR_KAHAN = 2   /* FAST with Kahan-compensated FP32 accumulation */
+} dimcv_far_mode;
+
+/* Scalar configuration.  Every float is the f32 value the reference kernels
+ * see: Python f64 arithmetic first, then one rounding to f32 (the host
+ * wrapper computes these exactly as particle_system.py:41-58,
+ * sph_base.py:9-48 and dimcv_sph.py:13-64 do). */
+typedef struct dimcv_params {
+  int32_t grid_num[3];          /* ceil(domain_size / support)  particle_system.py:54 */
+  int32_t n_cells;              /* prod(grid_num)               :55 */
+  int64_t n_max;                /* particle_max_num             :117-123 */
+  int32_t nbr_kmax;             /* neighbour-list width (build-defined) */
+  int32_t periodic;             /* 0: reference clipping; 1: minimum-image wrap */
+  float grid_size;              /* f32(support_radius), pos_to_index :382-384 */
+  float support_radius;         /* f32(4 r) */
+  float inv_support_radius;     /* f32(1 / (4 r)) */
+  float kernel_k;               /* 2.5465f * f32(inv^3)         sph_base.py:33-35 */
+  float kernel_dk;              /* (6 * 2.5465f) * f32(inv^3)   sph_base.py:56-57 */
+  float density0;               /* f32(density0) */
+  float dt;                     /* solver.dt[None] */
+  float gravity[3];             /* f32(gravitation) */
+  float visc_coef;              /* f32(2 (dim+2) nu)            dimcv_sph.py:170,177 */
+  float visc_h2;                /* f32(0.01 support^2)          :178-179 */
+  float m_eps;                  /* f32(1e-5)                    :24 */
+  float domain_start[3];        /* f32, update_activity         particle_system.py:420 */
+  float domain_end[3];
+  float box_size[3];            /* periodic wrap lengths (periodic = 1 only) */
+  int32_t has_vortex_domain;    /* dimcv_sph.py:76-84 */
+  float vortex_start[3];
+  float vortex_end[3];
+  int32_t has_transit;          /* dimcv_sph.py:401-405 */
+  float transit_a;              /* f32(area[0]) */
+  float transit_ba;             /* f32(area[1] - area[0]) */
+  float vp_factor;              /* f32(0.1592 * volumeScale * m_V0)  :691,714 */
+  float eps;                    /* f32(1e-6)                    :15 */
+  float inv_eps;                /* f32(1 / 1e-6)                :16 */
+  float neg_gen_a;              /* f32(-GenProbPara)            :298 */
+  float neg_del_b;              /* f32(-DelProbPara)            :303 */
+  float far_r2_max_rejected;    /* largest f32 t with sqrt_rn(t) <= support */
+  float bounce_padding;         /* f32(padding) for enforce_boundary_3D (sph_base.py:149-179) */
+  float bounce_hi[3];           /* f32(domain_size - padding), same kernel */
+  int32_t far_mode;             /* dimcv_far_mode */
+  float m_V0;                   /* f32(0.8 (2 r)^3)             particle_system.py:46 */
+  double vp_factor_f64;         /* 0.1592 * volumeScale * m_V0 in FP64 (direct FP64 sum) */
+} dimcv_params;
+
+/* Device buffers.  Field names follow the reference (particle_system.py:143-220,
+ * dimcv_sph.py:30-53).  Fields marked [sorted] are permuted by the grid build:
+ * the build writes the permuted copy into the matching *_buffer pointer and the
+ * caller swaps the pair afterwards (dimcv_grid_build). */
+typedef struct dimcv_state {
+  /* ParticleSystem */
+  int32_t* object_id;        /* [sorted] */
+  float* x;                  /* [sorted] */
+  float* x_0;                /* [sorted] */
+  float* v;                  /* [sorted] */
+  float* acceleration;
+  float* m_V;                /* [sorted] */
+  float* m;                  /* [sorted] */
+  float* density;            /* [sorted] */
+  float* pressure;           /* [sorted] */
+  int32_t* material;         /* [sorted] */
+  int32_t* color;            /* [sorted] */
+  uint8_t* is_dynamic;       /* [sorted] */
+  float* vorticity;
+  float* vorticity_vis;
+  float* dfsph_factor;       /* [sorted] */
+  float* density_adv;        /* [sorted] */
+  uint8_t* is_active;        /* [sorted] */
+  uint8_t* is_sample;        /* [sorted] */
+  float* life_time;          /* [sorted] */
+  int32_t* grid_ids;         /* [sorted] cell key per slot */
+  /* permutation targets of the [sorted] fields */
+  int32_t* object_id_buffer;
+  float* x_buffer;
+  float* x_0_buffer;
+  float* v_buffer;
+  float* m_V_buffer;
+  float* m_buffer;
+  float* density_buffer;
+  float* pressure_buffer;
+  int32_t* material_buffer;
+  int32_t* color_buffer;
+  uint8_t* is_dynamic_buffer;
+  float* dfsph_factor_buffer;
+  float* density_adv_buffer;
+  uint8_t* is_active_buffer;
+  uint8_t* is_sample_buffer;
+  float* life_time_buffer;
+  int32_t* grid_ids_buffer;
+  /* cell table: inclusive prefix sum of per-cell counts (the reference's
+   * grid_particles_num after PrefixSumExecutor.run), and the per-cell
+   * remainder the counting sort leaves (all zero after a build) */
+  int32_t* grid_particles_num;
+  int32_t* grid_particles_num_temp;
+  /* DIMCVSPHSolver */
+  float* d_vort;
+  float* d_vort_smoothed;
+  float* v_star;
+  float* grad_v;
+  float* x_temp;
+  int32_t* sample_idx;
+  float* kinematic_vort_num;
+  float* kernel_density;
+  int32_t* num_particles_each_emitter;
+  int32_t n_emitters;
+  int32_t pad0;
+  /* build-owned */
+  int32_t* fluid_list;       /* ascending fluid slots, counters[DIMCV_C_FLUID_LIST] entries */
+  int32_t* nbr_count;        /* per fluid-list entry: neighbours found */
+  uint32_t* nbr;             /* [nbr_kmax][n_max]: slot | (is_fluid << 31), ascending slot order */
+  float* v_vort;             /* optional [n_max][3]: recovered velocity per processed particle */
+  int32_t* counters;         /* [DIMCV_C_COUNT] */
+  double* dscal;             /* [DIMCV_D_COUNT] */
+  void* workspace;
+  int64_t workspace_bytes;
+} dimcv_state;
+
+/* ---- library ---------------------------------------------------------------- */
+int dimcv_abi_version(void);
+const char* dimcv_last_error(void);
+/* Workspace bytes needed for n_max particles, n_cells cells and up to
+ * max_samples importance samples. */
+int64_t dimcv_workspace_bytes(int64_t n_max, int64_t n_cells, int64_t max_samples);
+
+/* ---- particle insertion and scene set-up ------------------------------------ */
+/* ParticleSystem.add_particles / _add_particles / add_particle
+ * (particle_system.py:316-380).  Inputs are device arrays of n entries; slots
+ * [particle_num, particle_num + n) are written, entries beyond n_max are
+ * dropped (reference semantics, :361).  The dropped count is returned through
+ * *dropped (host) after a stream synchronisation. */
+int dimcv_add_particles(const dimcv_state* st, const dimcv_params* p, int32_t object_id, int32_t n,
+                        const float* pos, const float* vel, const float* density,
+                        const float* pressure, const int32_t* material, const int32_t* is_dynamic,
+                        const int32_t* color, int32_t* dropped, void* stream);
+/* KarmanVortexSolver.init_cylinder (karman_vortex.py:27-36). */
+int dimcv_init_cylinder(const dimcv_state* st, const dimcv_params* p, int32_t object_id, float cx,
+                        float cy, float radius, void* stream);
+
+/* ---- uniform grid ------------------------------------------------------------ */
+/* ParticleSystem.initialize_particle_system (particle_system.py:555-560):
+ * update_activity (:413-427), particle_partition (:429-488), update_grid_id
+ * (:490-498), PrefixSumExecutor.run (:559) and counting_sort (:500-553),
+ * restated as: activity -> cell keys (inactive slots keyed past the last cell)
+ * -> stable LSD radix sort of (key, slot) -> cell table -> one gather of the
+ * [sorted] fields into the *_buffer pointers -> ascending fluid list.
+ * Slot order equals the serial reference order (ascending compaction, stable
+ * counting sort).  The caller swaps every [sorted] field with its buffer
+ * afterwards.  Synchronises the stream and copies counters[] to host_counters
+ * (DIMCV_C_COUNT int32).  Returns DIMCV_ERR_KEY when an active particle lies
+ * outside the grid (undefined behaviour in the reference). */
+int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_particle_num,
+                     int32_t* host_counters, void* stream);
+/* Fixed-radius neighbour list of every fluid slot, the set and order that
+ * ParticleSystem.for_all_neighbors (particle_system.py:562-586) visits: 27
+ * cells in ndrange order, cell range [gpn[max(0,c-1)], gpn[c]) (cell 0 is never
+ * visited), strict |x_i - x_j| < support test, j != i.  Returns DIMCV_ERR_NBR
+ * (after a stream synchronisation) when a list exceeds nbr_kmax. */
+int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                         void* stream);
+
+/* ---- boundary volumes (sph_base.py:91-112) ---------------------------------- */
+int dimcv_compute_static_boundary_volume(const dimcv_state* st, const dimcv_params* p,
+                                         int32_t particle_bound, void* stream);
+int dimcv_compute_moving_boundary_volume(const dimcv_state* st, const dimcv_params* p,
+                                         int32_t particle_bound, void* stream);
+
+/* ---- SPH / DFSPH passes (dimcv_sph.py).  `fluid_bound` / `particle_bound` are
+ * host-side upper bounds of the fluid-list length / particle count used to
+ * size the launch; the kernels read the exact counts from counters[]. ------- */
+int dimcv_compute_densities(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);        /* :126-148 */
+int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);     /* :86-124 */
+int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);   /* :337-374 */
+/* field *= scale on fluid slots (:395-399). */
+int dimcv_multiply_time_step(const dimcv_state* st, const dimcv_params* p, float* field, float scale,
+                             int32_t fluid_bound, void* stream);
+/* Canonical FP64 sum of w(x)(density_adv - offset) over fluid slots
+ * (:407-419) into dscal[DIMCV_D_RESIDUAL]; if host_sum is non-NULL the stream
+ * is synchronised and the sum copied there. */
+int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, float offset,
+                                int32_t fluid_bound, double* host_sum, void* stream);
+int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :421-468 */
+int dimcv_compute_non_pressure_forces(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound, void* stream); /* :150-208 */
+int dimcv_predict_velocity(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream); /* :220-226 */
+int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream);           /* :210-218 */
+int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :470-493 */
+int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :495-549 */
+int dimcv_copy_x_temp(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream);      /* :797-801 */
+int dimcv_compute_vorticity_vis(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :778-795 */
+/* sph_base.py:149-179 (fluid only, as called from karman_vortex.py:226-230). */
+int dimcv_enforce_boundary_3d(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream);
+
+/* ---- DIMCV passes ------------------------------------------------------------ */
+int dimcv_compute_v_star(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);      /* :264-271 */
+int dimcv_compute_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);   /* :228-248,284-293 */
+int dimcv_compute_grad_v(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);      /* :250-262,273-282 */
+/* init() (:803-806) as one neighbour pass: v*, vorticity and grad_v. */
+int dimcv_dimcv_init(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);
+int dimcv_compute_d_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :551-583 */
+int dimcv_compute_all_d_vort_smoothed(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :660-677 */
+/* compute_kinematic_vorticity_num_sample (:295-324).  u1/u2: device f32 arrays
+ * indexed by slot replacing the two ti.random() draws; with do_bernoulli = 0
+ * only the KVN is written (fixed-M importance sampling). */
+int dimcv_compute_kvn_sample(const dimcv_state* st, const dimcv_params* p, const float* u1,
+                             const float* u2, int32_t do_bernoulli, int32_t fluid_bound, void* stream);
+/* record_vort_part_sample_idx (:326-335): ascending compaction. */
+int dimcv_record_sample_idx(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream);
+/* compute_kernel_density (:721-751) + canonical FP64 kden_sum. */
+int dimcv_compute_kernel_density(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);
+/* Build-defined inverse-CDF importance sampler: weights kden over in-domain
+ * fluid slots, canonical FP64 inclusive scan, u (device f64[m]) mapped to
+ * indices by upper_bound.  Writes sample_idx[0..m) and num_samples. */
+int dimcv_cdf_sample(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
+                     int32_t fluid_bound, void* stream);
+/* vort2vel (:649-719): near field over the neighbour list, far field over the
+ * sample set (p->far_mode), velocity gate |v_vort| < |v|.  max_samples is the
+ * capacity of the packed sample table (pass n_max; num_samples never exceeds
+ * it) and must fit the workspace. */
+int dimcv_vort2vel(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                   int32_t max_samples, void* stream);
+/* The two halves of dimcv_vort2vel, in this order: near field + S_i
+ * (dimcv_sph.py:686-691, into the workspace), then far field, combination and
+ * velocity gate (:693-719). */
+int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);
+int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       int32_t max_samples, void* stream);
+int dimcv_update_particle_life_time(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :753-760 */
+
+/* ---- comparators and initial conditions (build-defined) --------------------- */
+/* Direct O(N^2) Biot-Savart over every in-domain fluid particle: the sum the
+ * far-field Monte Carlo term estimates.  precision: 0 = FP32, 1 = FP32 Kahan,
+ * 2 = FP64.  Writes v_vort (or out_f64 [n_max][3] for precision 2); does not
+ * touch v. */
+int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precision,
+                    double* out_f64, int32_t fluid_bound, void* stream);
+/* Vortex-ring initial velocity at n f32 positions: pts is a device table of
+ * npts ring points, 7 doubles each (position, line element dl, G/(4 pi));
+ * u(x) = sum g dl x (x - p) / |x - p|^3 (1 - exp(-|x - p|^2 inv_sigma2)),
+ * FP64 accumulation, written as f32. */
+int dimcv_ring_velocity(int64_t npts, const double* pts, double inv_sigma2, int64_t n,
+                        const float* x, float* vel, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIMCV_H */
