@@ -10,7 +10,9 @@ Keys this build adds (all optional; absent keys keep reference behaviour):
   ``farField``   "exact" | "fast" | "kahan" far-field evaluation;
   ``viscosity``  kinematic viscosity (the reference hard-codes 1e-3);
   ``rngSeed``    seed of the host uniform generator (replaces ti.random);
-  ``neighborListWidth``  neighbour-list capacity per particle.
+  ``neighborListWidth``  neighbour-list capacity per particle;
+  ``periodic``   periodic box (minimum image, positions wrapped; the box must
+                 start at the origin and span at least 3 cells per axis).
 """
 from __future__ import annotations
 

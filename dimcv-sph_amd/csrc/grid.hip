@@ -74,10 +74,14 @@ __global__ void k_init_cylinder(dimcv_state st, int32_t bound, int32_t obj, floa
 // ---------------------------------------------------------------------------
 // pos_to_index / flatten (particle_system.py:382-402): f32 division by the
 // f32 grid size, truncation toward zero, row-major flattening.
+// In a periodic box (positions wrapped into [0, L)) the index is clamped to
+// the last cell, which f32 rounding of x / h can otherwise overshoot.
 __device__ __forceinline__ void cell_of(const dimcv_params& p, v3 x, int c[3]) {
   c[0] = (int)(x.x / p.grid_size);
   c[1] = (int)(x.y / p.grid_size);
   c[2] = (int)(x.z / p.grid_size);
+  if (p.periodic)
+    for (int d = 0; d < 3; ++d) c[d] = min(max(c[d], 0), p.grid_num[d] - 1);
 }
 
 // update_activity (:413-427) fused with update_grid_id's key computation
@@ -97,6 +101,15 @@ __global__ void k_activity_keys(dimcv_state st, dimcv_params p, int64_t n_in,
   uint8_t active = st.is_active[i];
   if (mat == 0) {
     active = 1;
+  } else if (mat == 1 && p.periodic) {
+    // periodic box: wrap instead of deactivating (build-defined variant)
+    float q[3] = {xi.x, xi.y, xi.z};
+    for (int j = 0; j < 3; ++j) {
+      if (q[j] < p.domain_start[j]) q[j] += p.box_size[j];
+      else if (q[j] >= p.domain_end[j]) q[j] -= p.box_size[j];
+    }
+    xi = mk(q[0], q[1], q[2]);
+    st3(st.x, i, xi);
   } else if (mat == 1) {
     float q[3] = {xi.x, xi.y, xi.z};
     for (int j = 0; j < 3; ++j) {
@@ -317,23 +330,26 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
                                            const int32_t* __restrict__ gpn, int64_t i, v3 xi, F&& f) {
   int cc[3];
   cell_of(p, xi, cc);
-  const int ny = p.grid_num[1], nz = p.grid_num[2];
+  const int nx = p.grid_num[0], ny = p.grid_num[1], nz = p.grid_num[2];
   for (int o0 = -1; o0 <= 1; ++o0) {
     int g0 = cc[0] + o0;
-    if (g0 < 0 || g0 >= p.grid_num[0]) continue;
+    if (p.periodic) g0 = (g0 + nx) % nx;
+    else if (g0 < 0 || g0 >= nx) continue;
     for (int o1 = -1; o1 <= 1; ++o1) {
       int g1 = cc[1] + o1;
-      if (g1 < 0 || g1 >= ny) continue;
+      if (p.periodic) g1 = (g1 + ny) % ny;
+      else if (g1 < 0 || g1 >= ny) continue;
       for (int o2 = -1; o2 <= 1; ++o2) {
         int g2 = cc[2] + o2;
-        if (g2 < 0 || g2 >= nz) continue;
+        if (p.periodic) g2 = (g2 + nz) % nz;
+        else if (g2 < 0 || g2 >= nz) continue;
         int fl = g0 * ny * nz + g1 * nz + g2;
         int beg = gpn[fl > 0 ? fl - 1 : 0];
         int end = gpn[fl];
         for (int j = beg; j < end; ++j) {
           if (j == i) continue;
           v3 xj = ld3(x, j);
-          if (vnorm(vsub(xi, xj)) < p.support_radius) f(j);
+          if (vnorm(rel(p, xi, xj)) < p.support_radius) f(j);
         }
       }
     }
@@ -364,7 +380,7 @@ __global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound,
   v3 xi = ld3(st.x, i);
   float delta = cubic_W(p, 0.0f);
   scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) {
-    if (st.material[j] == 0) delta += cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))));
+    if (st.material[j] == 0) delta += cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j))));
   });
   st.m_V[i] = 1.0f / delta * 3.0f;
 }
@@ -421,6 +437,10 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
               (long long)p->n_max);
     return DIMCV_ERR_ARG;
   }
+  if (p->periodic && (p->grid_num[0] < 3 || p->grid_num[1] < 3 || p->grid_num[2] < 3)) {
+    set_error("periodic boxes need at least 3 cells per axis");
+    return DIMCV_ERR_ARG;
+  }
   const int64_t n_in = prev_particle_num;
   DIMCV_HIP(hipMemsetAsync(st->counters + DIMCV_C_KEY_ERROR, 0, sizeof(int32_t), s));
   if (n_in > 0)
@@ -471,10 +491,6 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
 int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                          void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (p->periodic) {
-    set_error("periodic neighbour search is not available");
-    return DIMCV_ERR_ARG;
-  }
   if (fluid_bound > 0)
     hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound);
   DIMCV_LAUNCHED("k_nbr_build");

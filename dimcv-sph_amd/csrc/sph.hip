@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params 
   v3 xi = ld3(st.x, i);
   float dens = st.m_V[i] * cubic_W(p, 0.0f);
   float den = 0.0f;
-  FOR_NBR(k, e, j) { den += st.m_V[j] * cubic_W(p, vnorm(vsub(xi, ld3(st.x, j)))); }
+  FOR_NBR(k, e, j) { den += st.m_V[j] * cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j)))); }
   dens += den;
   dens *= p.density0;
   st.density[i] = dens;
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_para
   v3 xi = ld3(st.x, i);
   float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
   FOR_NBR(k, e, j) {
-    v3 gw = cubic_dW(p, vsub(xi, ld3(st.x, j)));
+    v3 gw = cubic_dW(p, rel(p, xi, ld3(st.x, j)));
     if (nb_fluid(e)) {
       v3 g = smul(st.m[j], gw);
       r3 += vdot(g, g);
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float dadv = 0.0f;
   FOR_NBR(k, e, j) {
-    dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, vsub(xi, ld3(st.x, j))));
+    dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, rel(p, xi, ld3(st.x, j))));
   }
   float d = fmaxf(dadv, 0.0f);
   if (st.nbr_count[f] < 20) d = 0.0f;
@@ -136,12 +136,12 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
       float k_j = b_j * st.dfsph_factor[j] * st.density[j];
       float k_sum = k_i + k_j;
       if (fabsf(k_sum) > p.m_eps) {
-        v3 g = smul(st.m[j], cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 g = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
         dv = vsub(dv, smul(dt * k_sum, g));
       }
     } else {
       if (fabsf(k_i) > p.m_eps) {
-        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
         v3 vc = smul(-dt * 1.0f * k_i, g);
         dv = vadd(dv, vc);
       }
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
   v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
   FOR_NBR(k, e, j) {
     if (nb_fluid(e)) {
-      v3 r = vsub(xi, ld3(st.x, j));
+      v3 r = rel(p, xi, ld3(st.x, j));
       float v_xy = vdot(vsub(vi, ld3(st.v, j)), r);
       float rn = vnorm(r);
       v3 fv = smul(p.visc_coef * st.m_V[j] * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float delta = 0.0f;
   FOR_NBR(k, e, j) {
-    float dd = vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, vsub(xi, ld3(st.x, j))));
+    float dd = vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, rel(p, xi, ld3(st.x, j))));
     if (nb_fluid(e)) delta += st.m[j] * dd;
     else delta += st.m_V[j] * p.density0 * dd;
   }
@@ -230,13 +230,13 @@ __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_par
       float k_j = b_j * st.dfsph_factor[j];
       float k_sum = k_i + k_j;
       if (fabsf(k_sum) > p.m_eps) {
-        v3 g = smul(st.m[j], cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 g = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
         if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, g));
         else vi = vsub(vi, smul(w * dt * k_sum, g));
       }
     } else {
       if (fabsf(k_i) > p.m_eps) {
-        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, vsub(xi, ld3(st.x, j))));
+        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
         v3 vc = smul(-dt * 1.0f * k_i, g);
         if (!p.has_transit) vi = vadd(vi, vc);
         else vi = vadd(vi, smul(w, vc));
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(256) k_vorticity_vis(dimcv_state st, dimcv_par
   float w = st.m_V[i] * cubic_W(p, 0.0f);
   v3 vort = smul(w, ld3(st.vorticity, i));
   FOR_NBR(k, e, j) {
-    float wj = st.m_V[j] * cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))));
+    float wj = st.m_V[j] * cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j))));
     vort = vadd(vort, smul(wj, ld3(st.vorticity, j)));
     w += wj;
   }
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
   FOR_NBR(k, e, j) {
     if (!nb_fluid(e)) continue;
     v3 vj = ld3(st.v, j);
-    v3 kd = cubic_dW(p, vsub(xi, ld3(st.x, j)));
+    v3 kd = cubic_dW(p, rel(p, xi, ld3(st.x, j)));
     v3 vji = vsub(vj, vi);
     float mv = st.m_V[j];
     if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(256) k_d_vorticity(dimcv_state st, dimcv_param
   v3 curl = mk(0.0f, 0.0f, 0.0f);
   FOR_NBR(k, e, j) {
     if (!nb_fluid(e)) continue;
-    curl = vadd(curl, smul(st.m_V[j], vcross(vsub(ld3(st.v_star, j), vsi), cubic_dW(p, vsub(xi, ld3(st.x, j))))));
+    curl = vadd(curl, smul(st.m_V[j], vcross(vsub(ld3(st.v_star, j), vsi), cubic_dW(p, rel(p, xi, ld3(st.x, j))))));
   }
   st3(st.d_vort, i, vsub(ld3(st.vorticity, i), curl));
 }
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(256) k_d_vort_smoothed(dimcv_state st, dimcv_p
   v3 acc = vmuls(smul(st.m_V[i], ld3(st.d_vort, i)), cubic_W(p, 0.0f));
   FOR_NBR(k, e, j) {
     if (!nb_fluid(e)) continue;
-    acc = vadd(acc, vmuls(smul(st.m_V[j], ld3(st.d_vort, j)), cubic_W(p, vnorm(vsub(xi, ld3(st.x, j))))));
+    acc = vadd(acc, vmuls(smul(st.m_V[j], ld3(st.d_vort, j)), cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j))))));
   }
   st3(st.d_vort_smoothed, i, acc);
 }
@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
         uint32_t e = nl[(int64_t)k * ns];
         if (!nb_fluid(e)) continue;
         int64_t j = nb_slot(e);
-        kden += st.kinematic_vort_num[j] * epan(p, vsub(xi, ld3(st.x, j)));
+        kden += st.kinematic_vort_num[j] * epan(p, rel(p, xi, ld3(st.x, j)));
       }
       st.kernel_density[i] = kden;
       contrib = (double)kden;

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
     uint32_t e = nl[(int64_t)k * ns];
     if (!nb_fluid(e)) continue;
     int64_t j = nb_slot(e);
-    v3 r2 = vsub(xi, ld3(st.x, j));
+    v3 r2 = rel(p, xi, ld3(st.x, j));
     v3 r = mk(r2.x, r2.y, 0.0f);
     float rn = vnorm(r);
     float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_vv_far_exact(dimcv_state st, dimcv_para
   int added = 0;
   for (int js = 0; js < M; ++js) {
     const int64_t pj = st.sample_idx[js];
-    v3 r2 = vsub(xi, ld3(st.x, pj));
+    v3 r2 = rel(p, xi, ld3(st.x, pj));
     float rn = vnorm(r2);
     if (rn > p.support_radius) {
       v3 r = mk(r2.x, r2.y, 0.0f);
@@ -106,8 +106,9 @@ __global__ void k_pack_sources(dimcv_state st, const int32_t* __restrict__ idx, 
 }
 
 // FAST / KAHAN far field.  Sums sum_s 1/r^2 (r_xy x c_s) over accepted s.
-template <bool KAHAN>
-__device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v3 xi, float T, v3& acc, int& added) {
+template <bool KAHAN, bool PERIODIC>
+__device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v3 xi, float T, v3 L, v3& acc,
+                                        int& added) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
   int n = 0;
@@ -116,6 +117,12 @@ __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v
     const float4 a = src[2 * js];
     const float4 c = src[2 * js + 1];
     float dx = xi.x - a.x, dy = xi.y - a.y, dz = xi.z - a.z;
+    if (PERIODIC) {
+      const float hx = 0.5f * L.x, hy = 0.5f * L.y, hz = 0.5f * L.z;
+      dx = dx > hx ? dx - L.x : (dx < -hx ? dx + L.x : dx);
+      dy = dy > hy ? dy - L.y : (dy < -hy ? dy + L.y : dy);
+      dz = dz > hz ? dz - L.z : (dz < -hz ? dz + L.z : dz);
+    }
     float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
     bool ok = r2 > T;
     float inv = ok ? __builtin_amdgcn_rcpf(r2) : 0.0f;
@@ -145,7 +152,7 @@ __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v
   added = n;
 }
 
-template <bool KAHAN>
+template <bool KAHAN, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_vv_far_fast(dimcv_state st, dimcv_params p, int32_t bound,
                                                      const float* __restrict__ near3,
                                                      const float* __restrict__ scale_s,
@@ -158,13 +165,14 @@ __global__ void __launch_bounds__(256) k_vv_far_fast(dimcv_state st, dimcv_param
   const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
   v3 acc;
   int added;
-  far_sum<KAHAN>(src, M, xi, p.far_r2_max_rejected, acc, added);
+  far_sum<KAHAN, PERIODIC>(src, M, xi, p.far_r2_max_rejected, mk(p.box_size[0], p.box_size[1], p.box_size[2]),
+                           acc, added);
   const float S = scale_s[f];
   vv_finish(st, p, i, ld3(near3, f), vmuls(acc, S), added);
 }
 
 // ---- direct O(N^2) Biot-Savart ---------------------------------------------
-template <bool KAHAN>
+template <bool KAHAN, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_direct_f32(dimcv_state st, dimcv_params p, int32_t bound,
                                                     const float* __restrict__ near3,
                                                     const float4* __restrict__ src,
@@ -176,8 +184,17 @@ __global__ void __launch_bounds__(256) k_direct_f32(dimcv_state st, dimcv_params
   if (!in_vortex_domain(p, xi)) return;
   v3 acc;
   int added;
-  far_sum<KAHAN>(src, *nsrc, xi, p.far_r2_max_rejected, acc, added);
+  far_sum<KAHAN, PERIODIC>(src, *nsrc, xi, p.far_r2_max_rejected, mk(p.box_size[0], p.box_size[1], p.box_size[2]),
+                           acc, added);
   st3(st.v_vort, i, vadd(ld3(near3, f), vmuls(acc, p.vp_factor)));
+}
+
+// minimum image of an FP64 separation (periodic box)
+__device__ __forceinline__ void wrap_d(const dimcv_params& p, double& rx, double& ry, double& rz) {
+  double L0 = p.box_size[0], L1 = p.box_size[1], L2 = p.box_size[2];
+  if (rx > 0.5 * L0) rx -= L0; else if (rx < -0.5 * L0) rx += L0;
+  if (ry > 0.5 * L1) ry -= L1; else if (ry < -0.5 * L1) ry += L1;
+  if (rz > 0.5 * L2) rz -= L2; else if (rz < -0.5 * L2) rz += L2;
 }
 
 // FP64 direct sum (near part with the 2-D norm, far part 1/|r|^2, same
@@ -199,6 +216,10 @@ __global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params
     if (!nb_fluid(e)) continue;
     int64_t j = nb_slot(e);
     double rx = X - (double)st.x[3 * j], ry = Y - (double)st.x[3 * j + 1];
+    if (p.periodic) {
+      double rz = 0.0;
+      wrap_d(p, rx, ry, rz);
+    }
     double rn = sqrt(rx * rx + ry * ry);
     double ex = exp(-rn / 1e-6);
     double A = (1.0 - ex) / (rn * (rn + 1e-6 * ex));
@@ -213,10 +234,11 @@ __global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params
   for (int js = 0; js < M; ++js) {
     const float4 a = src[2 * js];
     const float4 c = src[2 * js + 1];
-    float dxf = xi.x - a.x, dyf = xi.y - a.y, dzf = xi.z - a.z;
-    float r2f = __fadd_rn(__fadd_rn(__fmul_rn(dxf, dxf), __fmul_rn(dyf, dyf)), __fmul_rn(dzf, dzf));
+    v3 rf = rel(p, xi, mk(a.x, a.y, a.z));
+    float r2f = __fadd_rn(__fadd_rn(__fmul_rn(rf.x, rf.x), __fmul_rn(rf.y, rf.y)), __fmul_rn(rf.z, rf.z));
     if (!(r2f > T)) continue;
     double rx = X - (double)a.x, ry = Y - (double)a.y, rz = Z - (double)a.z;
+    if (p.periodic) wrap_d(p, rx, ry, rz);
     double inv = 1.0 / (rx * rx + ry * ry + rz * rz);
     fx += inv * (ry * (double)c.z);
     fy += inv * (-rx * (double)c.z);
@@ -483,12 +505,11 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
   } else {
     hipLaunchKernelGGL(k_pack_sources, dim3(nblocks(max_samples)), dim3(256), 0, s, *st, st->sample_idx,
                        st->counters + DIMCV_C_NUM_SAMPLES, (int64_t)max_samples, 1, ws.samples);
-    if (p->far_mode == DIMCV_FAR_KAHAN)
-      hipLaunchKernelGGL(k_vv_far_fast<true>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
-                         ws.scale_s, ws.samples, (int)max_samples);
-    else
-      hipLaunchKernelGGL(k_vv_far_fast<false>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
-                         ws.scale_s, ws.samples, (int)max_samples);
+    auto kern = p->far_mode == DIMCV_FAR_KAHAN
+                    ? (p->periodic ? k_vv_far_fast<true, true> : k_vv_far_fast<true, false>)
+                    : (p->periodic ? k_vv_far_fast<false, true> : k_vv_far_fast<false, false>);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s, ws.samples,
+                       (int)max_samples);
   }
   DIMCV_LAUNCHED("vort2vel far field");
   return DIMCV_OK;
@@ -530,12 +551,9 @@ int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precis
                        p->vp_factor_f64);
   } else {
     hipLaunchKernelGGL(k_vv_near, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s);
-    if (precision == 1)
-      hipLaunchKernelGGL(k_direct_f32<true>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.samples,
-                         nsrc);
-    else
-      hipLaunchKernelGGL(k_direct_f32<false>, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3,
-                         ws.samples, nsrc);
+    auto kern = precision == 1 ? (p->periodic ? k_direct_f32<true, true> : k_direct_f32<true, false>)
+                               : (p->periodic ? k_direct_f32<false, true> : k_direct_f32<false, false>);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.samples, nsrc);
   }
   DIMCV_LAUNCHED("direct Biot-Savart");
   return DIMCV_OK;

@@ -191,7 +191,9 @@ class ParticleSystem:
         p.n_cells = int(self.flattened_grid_num)
         p.n_max = int(self.particle_max_num)
         p.nbr_kmax = self.nbr_kmax
-        p.periodic = 0
+        p.periodic = 1 if self.cfg.get_cfg("periodic") else 0
+        if p.periodic and (np.any(self.domain_start != 0) or np.any(self.grid_num < 3)):
+            raise ValueError("a periodic box must start at the origin and span at least 3 cells per axis")
         p.grid_size = f32(self.grid_size)
         p.support_radius = f32(self.support_radius)
         p.inv_support_radius = f32(self.inv_support_radius)

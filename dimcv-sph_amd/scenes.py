@@ -24,7 +24,7 @@ import _native as N
 
 def ring_scene_config(nx: int, ny: int, nz: int, r: float = 0.01, dt: float = 2e-4,
                       mc_samples: int | None = 4096, far_field: str = "fast", viscosity: float = 1e-3,
-                      seed: int = 2) -> dict:
+                      seed: int = 2, periodic: bool = False) -> dict:
     d = 2 * r
     cfg = {
         "domainStart": [0.0, 0.0, 0.0],
@@ -43,6 +43,8 @@ def ring_scene_config(nx: int, ny: int, nz: int, r: float = 0.01, dt: float = 2e
         "viscosity": viscosity,
         "rngSeed": seed,
     }
+    if periodic:
+        cfg["periodic"] = True
     if mc_samples:
         cfg["mcSamples"] = int(mc_samples)
     return {"Configuration": cfg}
@@ -101,7 +103,7 @@ def ring_velocity(x: torch.Tensor, table: np.ndarray, sigma: float) -> torch.Ten
 def build_ring_system(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.01, seed: int = 2,
                       mc_samples: int | None = 4096, far_field: str = "fast", dt: float = 2e-4,
                       viscosity: float = 1e-3, sigma: float | None = None, K: int = 256,
-                      device: str = "cuda"):
+                      periodic: bool = False, device: str = "cuda"):
     """ParticleSystem + KarmanVortexSolver for a ring-lattice scene, initialised.
 
     Returns (ps, solver, info) where info holds the positions, ring table and
@@ -117,7 +119,7 @@ def build_ring_system(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.
     sigma = 4 * r if sigma is None else sigma
     table = ring_points(*rings, K=K)
     scene = ring_scene_config(nx, ny, nz, r=r, dt=dt, mc_samples=mc_samples, far_field=far_field,
-                              viscosity=viscosity, seed=seed)
+                              viscosity=viscosity, seed=seed, periodic=periodic)
     cfg = SimConfig(scene=scene)
     ps = ParticleSystem(cfg, device=device)
     xt = torch.as_tensor(pos, device=ps.device)

@@ -154,7 +154,7 @@ struct OrConfig {
   int32_t has_vortex_domain;
   int32_t has_transit;
   int32_t num_emitters;
-  int32_t pad;
+  int32_t periodic;  // build-defined periodic box (minimum image, wrap)
 };
 
 struct Sim {
@@ -169,6 +169,7 @@ struct Sim {
   float f_dstart[3], f_dend[3], f_vstart[3], f_vend[3], f_g[3];
   float f_visc_coef, f_visc_h2, f_transit_a, f_transit_ba, f_vp_factor;
   float f_eps, f_inv_eps, f_neg_a, f_neg_b;
+  float f_box[3];
   float dt;
   // scalar fields
   int32_t particle_num, fluid_particle_num, solid_particle_num, num_samples;
@@ -233,11 +234,27 @@ struct Sim {
     }
     return res;
   }
-  // particle_system.py:382-402
+  // x_i - x_j; in the periodic variant the minimum image
+  V3 rel(V3 a, V3 b) const {
+    V3 r = vsub(a, b);
+    if (cfg.periodic) {
+      float L[3] = {f_box[0], f_box[1], f_box[2]};
+      float* q[3] = {&r.x, &r.y, &r.z};
+      for (int d = 0; d < 3; ++d) {
+        float hl = 0.5f * L[d];
+        if (*q[d] > hl) *q[d] -= L[d];
+        else if (*q[d] < -hl) *q[d] += L[d];
+      }
+    }
+    return r;
+  }
+  // particle_system.py:382-402 (periodic variant: clamped to the last cell)
   void cell_of(V3 p, int c[3]) const {
     c[0] = (int)(p.x / f_grid_size);
     c[1] = (int)(p.y / f_grid_size);
     c[2] = (int)(p.z / f_grid_size);
+    if (cfg.periodic)
+      for (int d = 0; d < 3; ++d) c[d] = std::min(std::max(c[d], 0), grid_num[d] - 1);
   }
   int64_t flat(const int c[3]) const {
     return (int64_t)c[0] * grid_num[1] * grid_num[2] + (int64_t)c[1] * grid_num[2] + c[2];
@@ -255,14 +272,16 @@ struct Sim {
         for (int o2 = -1; o2 <= 1; ++o2) {
           int g[3] = {cc[0] + o0, cc[1] + o1, cc[2] + o2};
           bool ok = true;
-          for (int d = 0; d < 3; ++d)
-            if (ok && (g[d] < 0 || g[d] >= grid_num[d])) ok = false;
+          for (int d = 0; d < 3; ++d) {
+            if (cfg.periodic) g[d] = (g[d] + grid_num[d]) % grid_num[d];
+            else if (ok && (g[d] < 0 || g[d] >= grid_num[d])) ok = false;
+          }
           if (!ok) continue;
           int64_t fl = flat(g);
           int32_t beg = grid_particles_num[std::max<int64_t>(0, fl - 1)];
           int32_t end = grid_particles_num[fl];
           for (int64_t p_j = beg; p_j < end; ++p_j)
-            if (p_i != p_j && vnorm(vsub(xi, X(p_j))) < f_support) task(p_j);
+            if (p_i != p_j && vnorm(rel(xi, X(p_j))) < f_support) task(p_j);
         }
   }
   // dimcv_sph.py:76-84
@@ -335,6 +354,7 @@ Sim* or_new(const OrConfig* c) {
     s->f_vstart[d] = (float)c->vortex_start[d];
     s->f_vend[d] = (float)c->vortex_end[d];
     s->f_g[d] = (float)c->gravitation[d];
+    s->f_box[d] = (float)(c->domain_end[d] - c->domain_start[d]);
   }
   s->f_visc_coef = (float)(2 * (3 + 2) * c->viscosity);
   s->f_visc_h2 = (float)(0.01 * std::pow(s->support, 2.0));
@@ -434,6 +454,12 @@ void or_update_activity(Sim* s) {
   for (int64_t i = 0; i < s->particle_num; ++i) {
     if (s->material[i] == 0) {
       s->is_active[i] = 1;
+    } else if (s->material[i] == 1 && s->cfg.periodic) {
+      for (int j = 0; j < 3; ++j) {
+        float& q = s->x[3 * i + j];
+        if (q < s->f_dstart[j]) q += s->f_box[j];
+        else if (q >= s->f_dend[j]) q -= s->f_box[j];
+      }
     } else if (s->material[i] == 1) {
       for (int j = 0; j < 3; ++j) {
         float xj = s->x[3 * i + j];
@@ -567,7 +593,7 @@ void or_compute_static_boundary_volume(Sim* s) {
     float delta = s->W(0.0f);
     V3 xi = s->X(p_i);
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      if (s->is_solid(p_j)) delta += s->W(vnorm(vsub(xi, s->X(p_j))));
+      if (s->is_solid(p_j)) delta += s->W(vnorm(s->rel(xi, s->X(p_j))));
     });
     s->m_V[p_i] = 1.0f / delta * 3.0f;
   }
@@ -581,7 +607,7 @@ void or_compute_moving_boundary_volume(Sim* s) {
     float delta = s->W(0.0f);
     V3 xi = s->X(p_i);
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      if (s->is_solid(p_j)) delta += s->W(vnorm(vsub(xi, s->X(p_j))));
+      if (s->is_solid(p_j)) delta += s->W(vnorm(s->rel(xi, s->X(p_j))));
     });
     s->m_V[p_i] = 1.0f / delta * 3.0f;
   }
@@ -596,8 +622,8 @@ void or_compute_densities(Sim* s) {
     float dens = s->m_V[p_i] * s->W(0.0f);
     float den = 0.0f;
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      if (s->is_fluid(p_j)) den += s->m_V[p_j] * s->W(vnorm(vsub(xi, s->X(p_j))));
-      else if (s->is_solid(p_j)) den += s->m_V[p_j] * s->W(vnorm(vsub(xi, s->X(p_j))));
+      if (s->is_fluid(p_j)) den += s->m_V[p_j] * s->W(vnorm(s->rel(xi, s->X(p_j))));
+      else if (s->is_solid(p_j)) den += s->m_V[p_j] * s->W(vnorm(s->rel(xi, s->X(p_j))));
     });
     dens += den;
     dens *= s->f_density0;
@@ -614,11 +640,11 @@ void or_compute_DFSPH_factor(Sim* s) {
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j)) {
-        V3 g = smul(s->m[p_j], s->gradW(vsub(xi, s->X(p_j))));
+        V3 g = smul(s->m[p_j], s->gradW(s->rel(xi, s->X(p_j))));
         r3 += vdot(g, g);
         r0 += g.x; r1 += g.y; r2 += g.z;
       } else if (s->is_solid(p_j)) {
-        V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(vsub(xi, s->X(p_j))));
+        V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(s->rel(xi, s->X(p_j))));
         r0 += g.x; r1 += g.y; r2 += g.z;
       }
     });
@@ -643,9 +669,9 @@ void or_compute_density_change(Sim* s) {
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       V3 vj = s->Vv(p_j);
       if (s->is_fluid(p_j))
-        dadv += s->m_V[p_j] * vdot(vsub(vi, vj), s->gradW(vsub(xi, s->X(p_j))));
+        dadv += s->m_V[p_j] * vdot(vsub(vi, vj), s->gradW(s->rel(xi, s->X(p_j))));
       else if (s->is_solid(p_j))
-        dadv += s->m_V[p_j] * vdot(vsub(vi, vj), s->gradW(vsub(xi, s->X(p_j))));
+        dadv += s->m_V[p_j] * vdot(vsub(vi, vj), s->gradW(s->rel(xi, s->X(p_j))));
       nn += 1;
     });
     float d = std::fmax(dadv, 0.0f);
@@ -692,12 +718,12 @@ void or_divergence_solver_iteration_kernel(Sim* s) {
         float k_j = b_j * s->dfsph_factor[p_j] * s->density[p_j];
         float k_sum = k_i + k_j;
         if (std::fabs(k_sum) > s->f_m_eps) {
-          V3 g = smul(s->m[p_j], s->gradW(vsub(xi, s->X(p_j))));
+          V3 g = smul(s->m[p_j], s->gradW(s->rel(xi, s->X(p_j))));
           dv = vsub(dv, smul(dt * k_sum, g));
         }
       } else if (s->is_solid(p_j)) {
         if (std::fabs(k_i) > s->f_m_eps) {
-          V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(vsub(xi, s->X(p_j))));
+          V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(s->rel(xi, s->X(p_j))));
           V3 vc = smul(-dt * 1.0f * k_i, g);
           dv = vadd(dv, vc);
         }
@@ -723,7 +749,7 @@ void or_compute_non_pressure_forces(Sim* s) {
     if (s->is_fluid(p_i)) {
       V3 xi = s->X(p_i), vi = s->Vv(p_i);
       s->for_all_neighbors(p_i, [&](int64_t p_j) {
-        V3 r = vsub(xi, s->X(p_j));
+        V3 r = s->rel(xi, s->X(p_j));
         float v_xy = vdot(vsub(vi, s->Vv(p_j)), r);
         if (s->is_fluid(p_j)) {
           float rn = vnorm(r);
@@ -773,7 +799,7 @@ void or_compute_vorticity(Sim* s) {
     V3 vort{0.0f, 0.0f, 0.0f};
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j))
-        vort = vadd(vort, smul(s->m_V[p_j], vcross(vsub(s->Vv(p_j), vi), s->gradW(vsub(xi, s->X(p_j))))));
+        vort = vadd(vort, smul(s->m_V[p_j], vcross(vsub(s->Vv(p_j), vi), s->gradW(s->rel(xi, s->X(p_j))))));
     });
     Sim::put(s->vorticity, p_i, vort);
   }
@@ -790,7 +816,7 @@ void or_compute_grad_v(Sim* s) {
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j)) {
         V3 vji = vsub(s->Vv(p_j), vi);
-        V3 kd = s->gradW(vsub(xi, s->X(p_j)));
+        V3 kd = s->gradW(s->rel(xi, s->X(p_j)));
         V3 a = smul(s->m_V[p_j], vji);
         float av[3] = {a.x, a.y, a.z}, kv[3] = {kd.x, kd.y, kd.z};
         for (int r = 0; r < 3; ++r)
@@ -812,7 +838,7 @@ void or_compute_d_vorticity(Sim* s) {
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j))
         curl = vadd(curl, smul(s->m_V[p_j], vcross(vsub(Sim::get(s->v_star, p_j), vsi),
-                                                    s->gradW(vsub(xi, s->X(p_j))))));
+                                                    s->gradW(s->rel(xi, s->X(p_j))))));
     });
     Sim::put(s->d_vort, p_i, vsub(Sim::get(s->vorticity, p_i), curl));
   }
@@ -828,7 +854,7 @@ void or_compute_all_d_vort_smoothed(Sim* s) {
     V3 acc = vmuls(smul(s->m_V[p_i], Sim::get(s->d_vort, p_i)), w0);
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j))
-        acc = vadd(acc, vmuls(smul(s->m_V[p_j], Sim::get(s->d_vort, p_j)), s->W(vnorm(vsub(xi, s->X(p_j))))));
+        acc = vadd(acc, vmuls(smul(s->m_V[p_j], Sim::get(s->d_vort, p_j)), s->W(vnorm(s->rel(xi, s->X(p_j))))));
     });
     Sim::put(s->d_vort_smoothed, p_i, acc);
   }
@@ -893,7 +919,7 @@ void or_compute_kernel_density(Sim* s) {
     V3 xi = s->X(p_i);
     float kden = s->kvn[p_i] * e0;
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      if (s->is_fluid(p_j)) kden += s->kvn[p_j] * epan(s, vsub(xi, s->X(p_j)));
+      if (s->is_fluid(p_j)) kden += s->kvn[p_j] * epan(s, s->rel(xi, s->X(p_j)));
     });
     s->kernel_density[p_i] = kden;
   }
@@ -948,7 +974,7 @@ void or_vort2vel(Sim* s, float* v_vort_out) {
     float kns = s->kernel_density[p_i];
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (s->is_fluid(p_j)) {
-        V3 r2 = vsub(xi, s->X(p_j));
+        V3 r2 = s->rel(xi, s->X(p_j));
         V3 r{r2.x, r2.y, 0.0f};
         float rn = vnorm(r);
         float A = (1.0f - expf_c(-rn * inv_eps)) / (rn * (rn + eps * expf_c(-rn * inv_eps)));
@@ -962,7 +988,7 @@ void or_vort2vel(Sim* s, float* v_vort_out) {
     float S = (float)(s->kden_sum - (double)kns);
     for (int32_t j = 0; j < s->num_samples; ++j) {
       int32_t p_j = s->sample_idx[j];
-      V3 r2 = vsub(xi, s->X(p_j));
+      V3 r2 = s->rel(xi, s->X(p_j));
       float rn = vnorm(r2);
       if (rn > s->f_support) {
         V3 r{r2.x, r2.y, 0.0f};
@@ -1006,9 +1032,9 @@ void or_compute_density_adv(Sim* s) {
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       V3 vj = s->Vv(p_j);
       if (s->is_fluid(p_j))
-        delta += s->m[p_j] * vdot(vsub(vi, vj), s->gradW(vsub(xi, s->X(p_j))));
+        delta += s->m[p_j] * vdot(vsub(vi, vj), s->gradW(s->rel(xi, s->X(p_j))));
       else if (s->is_solid(p_j))
-        delta += s->m_V[p_j] * s->f_density0 * vdot(vsub(vi, vj), s->gradW(vsub(xi, s->X(p_j))));
+        delta += s->m_V[p_j] * s->f_density0 * vdot(vsub(vi, vj), s->gradW(s->rel(xi, s->X(p_j))));
     });
     float da = s->density[p_i] + dt * delta;
     s->density_adv[p_i] = std::fmax(da, s->f_density0);
@@ -1031,13 +1057,13 @@ void or_pressure_solve_iteration_kernel(Sim* s) {
         float k_j = b_j * s->dfsph_factor[p_j];
         float k_sum = k_i + k_j;
         if (std::fabs(k_sum) > s->f_m_eps) {
-          V3 g = smul(s->m[p_j], s->gradW(vsub(xi, s->X(p_j))));
+          V3 g = smul(s->m[p_j], s->gradW(s->rel(xi, s->X(p_j))));
           if (!s->cfg.has_transit) vi = vsub(vi, smul(dt * k_sum, g));
           else vi = vsub(vi, smul(s->transit_weight(xi.x) * dt * k_sum, g));
         }
       } else if (s->is_solid(p_j)) {
         if (std::fabs(k_i) > s->f_m_eps) {
-          V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(vsub(xi, s->X(p_j))));
+          V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(s->rel(xi, s->X(p_j))));
           V3 vc = smul(-dt * 1.0f * k_i, g);
           if (!s->cfg.has_transit) vi = vadd(vi, vc);
           else vi = vadd(vi, smul(s->transit_weight(xi.x), vc));
@@ -1065,7 +1091,7 @@ void or_compute_vorticity_vis(Sim* s) {
     float w = s->m_V[p_i] * w0;
     V3 vort = smul(w, Sim::get(s->vorticity, p_i));
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      float wj = s->m_V[p_j] * s->W(vnorm(vsub(xi, s->X(p_j))));
+      float wj = s->m_V[p_j] * s->W(vnorm(s->rel(xi, s->X(p_j))));
       vort = vadd(vort, smul(wj, Sim::get(s->vorticity, p_j)));
       w += wj;
     });
@@ -1075,6 +1101,16 @@ void or_compute_vorticity_vis(Sim* s) {
 
 // ---------------------------------------------------------------------------
 // Build-defined comparators (no reference function; see the module header).
+
+// minimum image of an FP64 separation (periodic variant)
+static void wrap_d(const Sim* s, double& rx, double& ry, double& rz) {
+  double* q[3] = {&rx, &ry, &rz};
+  for (int d = 0; d < 3; ++d) {
+    double L = (double)s->f_box[d];
+    if (*q[d] > 0.5 * L) *q[d] -= L;
+    else if (*q[d] < -0.5 * L) *q[d] += L;
+  }
+}
 
 // FP64 direct Biot-Savart: exactly the quantity vort2vel estimates, with the
 // sample set = every in-domain fluid particle and weight 1.  Near part over
@@ -1096,6 +1132,7 @@ void or_direct_bs_f64(Sim* s, double* out) {
     s->for_all_neighbors(p_i, [&](int64_t p_j) {
       if (!s->is_fluid(p_j)) return;
       double rx = (double)xi.x - (double)s->x[3 * p_j], ry = (double)xi.y - (double)s->x[3 * p_j + 1];
+      if (s->cfg.periodic) { double rz = 0; wrap_d(s, rx, ry, rz); }
       double rn = std::sqrt(rx * rx + ry * ry);
       double e = std::exp(-rn / eps);
       double A = (1.0 - e) / (rn * (rn + eps * e));
@@ -1106,10 +1143,11 @@ void or_direct_bs_f64(Sim* s, double* out) {
     });
     double fx = 0, fy = 0, fz = 0;
     for (int32_t p_j : src) {
-      V3 r2 = vsub(xi, s->X(p_j));
+      V3 r2 = s->rel(xi, s->X(p_j));
       if (!(vnorm(r2) > s->f_support)) continue;
       double rx = (double)xi.x - (double)s->x[3 * p_j], ry = (double)xi.y - (double)s->x[3 * p_j + 1];
       double rz = (double)xi.z - (double)s->x[3 * p_j + 2];
+      if (s->cfg.periodic) wrap_d(s, rx, ry, rz);
       double r2d = rx * rx + ry * ry + rz * rz;
       double A = 1.0 / r2d;
       const float* w = &s->d_vort_smoothed[3 * p_j];

@@ -49,7 +49,7 @@ class OrConfig(ctypes.Structure):
         ("has_vortex_domain", ctypes.c_int32),
         ("has_transit", ctypes.c_int32),
         ("num_emitters", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("periodic", ctypes.c_int32),
     ]
 
 
@@ -209,6 +209,7 @@ class RefSim:
         c.viscosity = float(get("viscosity") or 1e-3)
         c.particle_max_num = int(self.particle_max_num)
         c.num_emitters = len(self.fluid_emitters)
+        c.periodic = 1 if get("periodic") else 0
         self._c = c
         self.sim = L.or_new(ctypes.byref(c))
         info = np.zeros(5, np.int64)

@@ -131,3 +131,21 @@ def test_ring_velocity_vs_oracle():
     v = scenes.ring_velocity(torch.as_tensor(pos, device="cuda"), table, 0.04).cpu().numpy()
     r = ring_velocity(pos, *rings, sigma=0.04, K=64)
     assert H.rel_l2(v, r) < 1e-6
+
+
+@pytest.mark.parametrize("far", ["exact", "fast"])
+def test_periodic_ring_box(far):
+    ps, solver, ref, info = H.make_ring_pair(12, 12, 12, n_rings=2, seed=13, mc_samples=128, far_field=far,
+                                             periodic=True)
+    rng = np.random.default_rng(14)
+    for k in range(3):
+        u = rng.random(128)
+        solver.step(u_cdf=u)
+        ref.step(u_cdf=u)
+        if far == "exact":
+            bad = H.mismatches(ps, solver, ref)
+            assert not bad, f"step {k}: {bad}"
+        else:
+            pn = ref.particle_num
+            assert ps.particle_num[None] == pn == 12 ** 3   # nothing leaves a periodic box
+            assert H.rel_l2(ps.v.to_numpy()[:pn], ref.field("v")[:pn]) < 1e-5

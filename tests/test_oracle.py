@@ -355,3 +355,41 @@ def test_mini_karman_runs_and_conserves_counts():
     mat = s.field("material")[:n]
     assert (mat == 1).sum() == s.fluid_particle_num
     assert not np.isnan(s.field("v")[:n]).any()
+
+
+def test_periodic_equals_open_box_in_the_interior():
+    """Minimum-image wrap changes nothing for particles more than 2h from
+    every face: their neighbour sets and separations are identical."""
+    res = {}
+    for periodic in (False, True):
+        s = _sim(n_max=20 ** 3, domainEnd=[0.4, 0.4, 0.4], periodic=periodic)
+        rng = np.random.default_rng(9)
+        g = (np.arange(20) + 0.5) * 0.02
+        pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        pos = (pos + rng.uniform(-0.001, 0.001, pos.shape)).astype(np.float32)
+        m = pos.shape[0]
+        vel = rng.normal(size=pos.shape).astype(np.float32)
+        s.add_particles(0, m, pos, vel, np.full(m, 1000.0, np.float32), np.zeros(m, np.float32),
+                        np.ones(m, np.int32), np.ones(m, np.int32), np.zeros((m, 3), np.int32))
+        s.initialize()
+        L.or_compute_densities(s.sim)
+        L.or_compute_density_change(s.sim)
+        res[periodic] = (s.field("x")[:m].copy(), s.field("density")[:m].copy(),
+                         s.field("density_adv")[:m].copy())
+    x0, d0, a0 = res[False]
+    x1, d1, a1 = res[True]
+    np.testing.assert_array_equal(x0, x1)   # same slot order (no particle wrapped)
+    inner = np.all((x0 > 0.09) & (x0 < 0.31), axis=1)
+    assert inner.sum() > 500
+    np.testing.assert_array_equal(d0[inner], d1[inner])
+    np.testing.assert_array_equal(a0[inner], a1[inner])
+    # near the faces the periodic box sees neighbours across the boundary
+    outer = np.any(x0 < 0.02, axis=1)
+    assert np.all(d1[outer] > d0[outer])
+    # the periodic box is homogeneous: near-face particles have interior
+    # densities, except next to cell 0 (never visited, also across the wrap)
+    c = (x0 / np.float32(0.04)).astype(np.int64)
+    next_to_cell0 = np.all(np.isin(c, [0, 1, 9]), axis=1)
+    face = outer & ~next_to_cell0
+    assert abs(d1[face].mean() - d1[inner].mean()) < 0.01 * d1[inner].mean()
+    assert np.all(d1[next_to_cell0 & outer] <= d1[face].max())
