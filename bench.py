@@ -46,32 +46,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-class EventTimer:
-    """Accumulates HIP-event time of one call site on the current stream."""
-
-    def __init__(self):
-        import torch
-        self.torch = torch
-        self.pairs = []
-
-    def wrap(self, fn):
-        def timed(*a, **k):
-            s = self.torch.cuda.Event(enable_timing=True)
-            e = self.torch.cuda.Event(enable_timing=True)
-            s.record()
-            r = fn(*a, **k)
-            e.record()
-            self.pairs.append((s, e))
-            return r
-        return timed
-
-    def mean_ms(self):
-        self.torch.cuda.synchronize()
-        if not self.pairs:
-            return float("nan")
-        return sum(s.elapsed_time(e) for s, e in self.pairs) / len(self.pairs)
-
-
 def run_gpu(args):
     import numpy as np
     import torch
@@ -88,20 +62,22 @@ def run_gpu(args):
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} steps done (k_v={solver.last_iterations_v}, k_p={solver.last_iterations_p})")
 
-    far_timer = EventTimer()
-    solver.vort2vel_far = far_timer.wrap(solver.vort2vel_far)
-    kv, kp, nf = [], [], []
+    # HIP events around the far-field kernel, recorded by dimcv_substep on the
+    # launch stream inside every timed step
+    solver.time_far_field = True
+    kv, kp, far = [], [], []
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
         solver.step()
         kv.append(solver.last_iterations_v)
         kp.append(solver.last_iterations_p)
+        far.append(solver.last_far_ms)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     n_fluid = ps.fluid_particle_num[None]
     sec = t2 - t1
-    far_ms = far_timer.mean_ms()
+    far_ms = float(np.mean(far))
     m = solver.num_samples[None]
     pairs = n_fluid * m
     achieved = FLOPS_PER_PAIR_MC * pairs / (far_ms * 1e-3) / 1e12
@@ -116,48 +92,80 @@ def run_gpu(args):
     return res, info
 
 
-def run_direct_comparison():
-    """MC-VVPE (FP32 Kahan) vs direct O(N^2) Biot-Savart (FP32) at N = 131,072."""
+def _recovery_errors(ps, solver, mode, m, u, v0, ref64, fluid):
+    """Run importance sampling (M = m) + vort2vel in far-field `mode` on the
+    current state; return (ms, rel L2 error of the recovered field, rel L2
+    error of the updated velocity) against the FP64 direct recovery."""
+    import numpy as np
+    import torch
+    import _native as N
+    ps.params.far_mode = {"fast": N.FAR_FAST, "kahan": N.FAR_KAHAN}[mode]
+    ps.v.t.copy_(v0)
+    solver.importance_sample(u[:m], m)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    solver.vort2vel()
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1])
+    vv = solver.v_vort.to_numpy()[fluid].astype(np.float64)
+    v_new = ps.v.to_numpy()[fluid].astype(np.float64)
+    v_old = v0.cpu().numpy()[fluid].astype(np.float64)
+    r = ref64[fluid]
+    gate = np.linalg.norm(r, axis=1) < np.linalg.norm(v_old, axis=1)
+    v_ref = v_old + gate[:, None] * r
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return ms, rel(vv, r), rel(v_new, v_ref)
+
+
+def run_recovery_comparison(nx, ny, nz, ms_list, direct_reps=2):
+    """MC-VVPE vs direct O(N^2) Biot-Savart on one state of an N = nx*ny*nz
+    ring lattice: FP64 direct as the oracle, FP32 naive and Kahan direct
+    timed, MC (FP32 naive and Kahan) for each M."""
     import numpy as np
     import torch
     import scenes
 
-    ps, solver, info = scenes.build_ring_system(64, 64, 32, n_rings=8, seed=2, mc_samples=4096,
+    ps, solver, info = scenes.build_ring_system(nx, ny, nz, n_rings=8, seed=2, mc_samples=4096,
                                                 far_field="kahan")
-    solver.step()  # one step so that every field of the recovery is populated
+    solver.step()  # populate every field the recovery reads
+    ps.ensure_neighbor_lists()
     torch.cuda.synchronize()
-    # FP64 direct reference and FP32 direct timing on the same state
-    ref = solver.direct_biot_savart("f64")
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    solver.direct_biot_savart("f32")  # warm
-    ev[0].record()
-    reps = 3
-    for _ in range(reps):
-        d32 = solver.direct_biot_savart("f32")
-    ev[1].record()
-    torch.cuda.synchronize()
-    direct_ms = ev[0].elapsed_time(ev[1]) / reps
-    # MC estimate on the same state: near + far recovery into v_vort
-    v_save = ps.v.t.clone()
-    ev2 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    solver.importance_sample()
-    ev2[0].record()
-    solver.vort2vel()
-    ev2[1].record()
-    torch.cuda.synchronize()
-    mc_ms = ev2[0].elapsed_time(ev2[1])
-    mc = solver.v_vort.to_numpy()
-    ps.v.t.copy_(v_save)
     pn = ps.particle_num[None]
-    rel = lambda a, b: float(np.linalg.norm(a[:pn] - b[:pn]) / np.linalg.norm(b[:pn]))
-    out = {
-        "n": pn, "m": solver.num_samples[None], "direct_f32_ms": round(direct_ms, 4),
-        "mc_kahan_vort2vel_ms": round(mc_ms, 4), "speedup": round(direct_ms / mc_ms, 2),
-        "rel_l2_err_mc_vs_f64": rel(mc.astype(np.float64), ref),
-        "rel_l2_err_direct_f32_vs_f64": rel(d32.astype(np.float64), ref),
-        "direct_tflops": round(FLOPS_PER_PAIR_DIRECT * pn * (pn - 1) / (direct_ms * 1e-3) / 1e12, 2),
-    }
-    log(f"direct vs MC at N={pn}: {out}")
+    fluid = np.zeros(ps.particle_max_num, bool)
+    fluid[ps.fluid_list.to_numpy()[: ps._nfl]] = True
+    v0 = ps.v.t.clone()
+    t0 = time.perf_counter()
+    ref64 = solver.direct_biot_savart("f64")
+    f64_ms = (time.perf_counter() - t0) * 1e3
+    out = {"n": pn, "direct_f64_ms": round(f64_ms, 2)}
+    for prec in ("f32", "kahan"):
+        solver.direct_biot_savart(prec)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(direct_reps):
+            d = solver.direct_biot_savart(prec)
+        ev[1].record()
+        torch.cuda.synchronize()
+        dms = ev[0].elapsed_time(ev[1]) / direct_reps
+        err = float(np.linalg.norm(d[fluid] - ref64[fluid]) / np.linalg.norm(ref64[fluid]))
+        out[f"direct_{prec}_ms"] = round(dms, 4)
+        out[f"direct_{prec}_rel_l2_err"] = err
+        out[f"direct_{prec}_tflops"] = round(FLOPS_PER_PAIR_DIRECT * pn * (pn - 1) / (dms * 1e-3) / 1e12, 2)
+    u = np.random.default_rng(7).random(max(ms_list))
+    sweep = []
+    for m in ms_list:
+        row = {"m": m}
+        for mode in ("fast", "kahan"):
+            ms, e_field, e_vel = _recovery_errors(ps, solver, mode, m, u, v0, ref64, fluid)
+            row[f"{mode}_ms"] = round(ms, 4)
+            row[f"{mode}_rel_l2_err_recovered"] = round(e_field, 6)
+            row[f"{mode}_rel_l2_err_velocity"] = round(e_vel, 8)
+        row["speedup_vs_direct_f32"] = round(out["direct_f32_ms"] / row["kahan_ms"], 2)
+        sweep.append(row)
+    out["mc"] = sweep
+    ps.v.t.copy_(v0)
+    log(f"recovery comparison at N={pn}: {json.dumps(out)}")
     del solver, ps
     torch.cuda.empty_cache()
     return out
@@ -202,7 +210,10 @@ def main():
         sys.exit(2)
 
     res, info = run_gpu(args)
-    direct = None if args.no_direct else run_direct_comparison()
+    direct = None
+    if not args.no_direct:
+        direct = {"comparator_131072": run_recovery_comparison(64, 64, 32, [4096]),
+                  "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536])}
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(info)
     value = res["n_fluid"] * args.steps / res["sec"]
     traffic = None

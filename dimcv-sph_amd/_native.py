@@ -12,7 +12,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
-SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip"]
+SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -27,6 +27,7 @@ _LIB = None
 # dimcv_counter / dimcv_dscalar / dimcv_far_mode (include/dimcv.h)
 C_PARTICLE_NUM, C_FLUID_NUM, C_SOLID_NUM, C_NUM_SAMPLES = 0, 1, 2, 3
 C_FLUID_LIST, C_NBR_MAX, C_KEY_ERROR, C_PREV_NUM, C_SAMPLE_SLOTS = 4, 5, 6, 7, 8
+C_JACOBI_DONE, C_JACOBI_ITERS = 9, 10
 C_COUNT = 16
 D_KDEN_SUM, D_RESIDUAL, D_CDF_TOTAL, D_COUNT = 0, 1, 2, 8
 FAR_EXACT, FAR_FAST, FAR_KAHAN = 0, 1, 2
@@ -65,6 +66,7 @@ class Params(ctypes.Structure):
         ("neg_gen_a", ctypes.c_float), ("neg_del_b", ctypes.c_float),
         ("far_r2_max_rejected", ctypes.c_float), ("bounce_padding", ctypes.c_float),
         ("bounce_hi", F3), ("far_mode", ctypes.c_int32), ("m_V0", ctypes.c_float),
+        ("nbr_r2_limit", ctypes.c_float),
         ("vp_factor_f64", ctypes.c_double),
     ]
 
@@ -87,6 +89,19 @@ STATE_POINTERS_2 = ["fluid_list", "nbr_count", "nbr", "v_vort", "counters", "dsc
 SORTED_FIELDS = ["object_id", "x", "x_0", "v", "m_V", "m", "density", "pressure", "material", "color",
                  "is_dynamic", "dfsph_factor", "density_adv", "is_active", "is_sample", "life_time",
                  "grid_ids"]
+
+
+class StepOpts(ctypes.Structure):
+    """dimcv_step_opts (include/dimcv.h)."""
+    _fields_ = [
+        ("u1", ctypes.c_void_p), ("u2", ctypes.c_void_p), ("u_cdf", ctypes.c_void_p),
+        ("m", ctypes.c_int32), ("max_iter_v", ctypes.c_int32), ("max_iter_p", ctypes.c_int32),
+        ("chunk", ctypes.c_int32), ("eta_v", ctypes.c_double), ("eta_p", ctypes.c_double),
+        ("inv_dt", ctypes.c_float), ("dt", ctypes.c_float), ("inv_dt2", ctypes.c_float),
+        ("density0", ctypes.c_float), ("fluid_count", ctypes.c_int32), ("bounce_back", ctypes.c_int32),
+        ("out_kv", ctypes.c_int32), ("out_kp", ctypes.c_int32),
+        ("time_far", ctypes.c_int32), ("out_far_ms", ctypes.c_float),
+    ]
 
 
 class State(ctypes.Structure):
@@ -127,6 +142,7 @@ def lib():
         "dimcv_vort2vel_far": (I32, [P, P, I32, I32, P]),
         "dimcv_direct_bs": (I32, [P, P, I32, P, I32, P]),
         "dimcv_ring_velocity": (I32, [I64, P, F64, I64, P, P, P]),
+        "dimcv_substep": (I32, [P, P, P, I32, I32, P]),
     }
     for name in ["dimcv_compute_densities", "dimcv_compute_DFSPH_factor",
                  "dimcv_compute_density_change", "dimcv_divergence_solver_iteration",

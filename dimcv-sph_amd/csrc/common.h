@@ -235,7 +235,8 @@ __device__ inline double block_tree_sum256(double v, double* lds) {
 }
 
 // host helpers implemented in util.hip
-int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream);
+int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream,
+                        const int32_t* done = nullptr);
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, uint32_t* total,
                        void* stream);
 

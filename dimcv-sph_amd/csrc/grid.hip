@@ -356,6 +356,38 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
   }
 }
 
+// Open grid: the three cells (c0+o0, c1+o1, c2-1..c2+1) are consecutive in the
+// flattened order, so the 27 cell ranges collapse into 9 contiguous slot runs
+// [gpn[max(0, first-1)], gpn[last]) visited in the same order; the run form
+// keeps cell 0 empty exactly as the per-cell rule does.  The distance test
+// sqrt_rn(r.r) < h is evaluated as r.r < nbr_r2_limit, which is equivalent
+// for every f32 r.r (sqrt is monotone and correctly rounded).
+template <class F>
+__device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __restrict__ x,
+                                          const int32_t* __restrict__ gpn, int64_t i, v3 xi, F&& f) {
+  int cc[3];
+  cell_of(p, xi, cc);
+  const int nx = p.grid_num[0], ny = p.grid_num[1], nz = p.grid_num[2];
+  const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, nz - 1);
+  const float lim = p.nbr_r2_limit;
+  for (int o0 = -1; o0 <= 1; ++o0) {
+    int g0 = cc[0] + o0;
+    if (g0 < 0 || g0 >= nx) continue;
+    for (int o1 = -1; o1 <= 1; ++o1) {
+      int g1 = cc[1] + o1;
+      if (g1 < 0 || g1 >= ny) continue;
+      int fl = (g0 * ny + g1) * nz;
+      int beg = gpn[max(fl + zlo - 1, 0)];
+      int end = gpn[fl + zhi];
+      for (int j = beg; j < end; ++j) {
+        v3 r = vsub(xi, ld3(x, j));
+        float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+        if (r2 < lim && j != i) f(j);
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
@@ -364,10 +396,12 @@ __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params 
   int cnt = 0;
   const int kmax = p.nbr_kmax;
   const int64_t stride = p.n_max;
-  scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) {
+  auto emit = [&](int j) {
     if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
     ++cnt;
-  });
+  };
+  if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, emit);
+  else scan_runs(p, st.x, st.grid_particles_num, i, xi, emit);
   st.nbr_count[f] = cnt;
   atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
 }

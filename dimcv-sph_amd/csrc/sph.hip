@@ -18,7 +18,7 @@ namespace dimcv {
   const int64_t ns = p.n_max
 
 #define FOR_NBR(k, e, j)                       \
-  for (int k = 0; k < nn; ++k)                 \
+  _Pragma("unroll 4") for (int k = 0; k < nn; ++k) \
     if (const uint32_t e = nl[(int64_t)k * ns]; true) \
       if (const int64_t j = nb_slot(e); true)
 
@@ -56,6 +56,43 @@ __global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params 
   st.density[i] = dens;
 }
 
+// compute_densities (dimcv_sph.py:126-148) and compute_DFSPH_factor (:86-124)
+// in one neighbour loop: both read only geometry and masses, and each sum is
+// accumulated exactly as in its own kernel.
+__global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  float dens = st.m_V[i] * cubic_W(p, 0.0f);
+  float den = 0.0f;
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+  FOR_NBR(k, e, j) {
+    v3 r = rel(p, xi, ld3(st.x, j));
+    den += st.m_V[j] * cubic_W(p, vnorm(r));
+    v3 gw = cubic_dW(p, r);
+    if (nb_fluid(e)) {
+      v3 g = smul(st.m[j], gw);
+      r3 += vdot(g, g);
+      r0 += g.x;
+      r1 += g.y;
+      r2 += g.z;
+    } else {
+      v3 g = smul(st.m_V[j] * p.density0, gw);
+      r0 += g.x;
+      r1 += g.y;
+      r2 += g.z;
+    }
+  }
+  dens += den;
+  dens *= p.density0;
+  st.density[i] = dens;
+  float sum = r3;
+  v3 gp = mk(r0, r1, r2);
+  sum += vdot(gp, gp);
+  float factor = 0.0f;
+  if (sum > 1e-6f) factor = 1.0f / sum;
+  st.dfsph_factor[i] = factor;
+}
+
 // dimcv_sph.py:86-124
 __global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_params p, int32_t bound) {
   FLUID_ENTRY();
@@ -85,7 +122,9 @@ __global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_para
 }
 
 // dimcv_sph.py:337-374
-__global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound) {
+__global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done) {
+  if (done && *done) return;
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float dadv = 0.0f;
@@ -107,8 +146,9 @@ __global__ void k_multiply(dimcv_state st, float* field, float scale, int32_t bo
 
 // dimcv_sph.py:407-419: per-block canonical FP64 tree over fluid-list order.
 __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p, float offset, int32_t bound,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, const int32_t* __restrict__ done) {
   __shared__ double lds[256];
+  if (done && *done) return;  // uniform per block
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
@@ -123,7 +163,9 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
 }
 
 // dimcv_sph.py:421-468
-__global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound) {
+__global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done) {
+  if (done && *done) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -202,7 +244,9 @@ __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
 }
 
 // dimcv_sph.py:470-493
-__global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound) {
+__global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done) {
+  if (done && *done) return;
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float delta = 0.0f;
@@ -216,7 +260,9 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
 }
 
 // dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour)
-__global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound) {
+__global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done) {
+  if (done && *done) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -308,11 +354,16 @@ __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
 
 // dimcv_sph.py:228-248,284-293 and 250-262,273-282; MODE 1 = vorticity,
 // 2 = grad_v, 3 = both (+ v*) in one neighbour pass.
+// MODE 7 also forms d_vort = vorticity - curl v* (:551-583) in the same loop:
+// v*_j = v_j + a_j dt is formed on the fly exactly as compute_v_star does.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound) {
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 vort = mk(0.0f, 0.0f, 0.0f);
+  v3 curl = mk(0.0f, 0.0f, 0.0f);
+  v3 vsi = mk(0.0f, 0.0f, 0.0f);
+  if (MODE & 4) vsi = vadd(vi, vmuls(ld3(st.acceleration, i), p.dt));
   float G[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   FOR_NBR(k, e, j) {
     if (!nb_fluid(e)) continue;
@@ -327,11 +378,16 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
       G[3] = G[3] + a.y * kd.x; G[4] = G[4] + a.y * kd.y; G[5] = G[5] + a.y * kd.z;
       G[6] = G[6] + a.z * kd.x; G[7] = G[7] + a.z * kd.y; G[8] = G[8] + a.z * kd.z;
     }
+    if (MODE & 4) {
+      v3 vsj = vadd(vj, vmuls(ld3(st.acceleration, j), p.dt));
+      curl = vadd(curl, smul(mv, vcross(vsub(vsj, vsi), kd)));
+    }
   }
   if (MODE & 1) st3(st.vorticity, i, vort);
   if (MODE & 2)
     for (int q = 0; q < 9; ++q) st.grad_v[9 * i + q] = G[q];
-  if (MODE == 3) st3(st.v_star, i, vadd(vi, vmuls(ld3(st.acceleration, i), p.dt)));
+  if (MODE >= 3) st3(st.v_star, i, vadd(vi, vmuls(ld3(st.acceleration, i), p.dt)));
+  if (MODE & 4) st3(st.d_vort, i, vsub(vort, curl));
 }
 
 // dimcv_sph.py:551-583
@@ -357,6 +413,31 @@ __global__ void __launch_bounds__(256) k_d_vort_smoothed(dimcv_state st, dimcv_p
     acc = vadd(acc, vmuls(smul(st.m_V[j], ld3(st.d_vort, j)), cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j))))));
   }
   st3(st.d_vort_smoothed, i, acc);
+}
+
+// compute_all_d_vort_smoothed (dimcv_sph.py:660-677) and compute_vorticity_vis
+// (:778-795) in one neighbour loop.  Both read only geometry, m_V, d_vort and
+// vorticity, none of which changes between the vorticity passes and the end
+// of the substep, so the visualisation field is the one the reference
+// computes after the pressure solve.
+__global__ void __launch_bounds__(256) k_smooth_vis(dimcv_state st, dimcv_params p, int32_t bound) {
+  FLUID_ENTRY();
+  v3 xi = ld3(st.x, i);
+  const bool dom = in_vortex_domain(p, xi);
+  const float w0 = cubic_W(p, 0.0f);
+  v3 acc = vmuls(smul(st.m_V[i], ld3(st.d_vort, i)), w0);
+  float w = st.m_V[i] * w0;
+  v3 vort = smul(w, ld3(st.vorticity, i));
+  FOR_NBR(k, e, j) {
+    float Wr = cubic_W(p, vnorm(rel(p, xi, ld3(st.x, j))));
+    float mv = st.m_V[j];
+    if (dom && nb_fluid(e)) acc = vadd(acc, vmuls(smul(mv, ld3(st.d_vort, j)), Wr));
+    float wj = mv * Wr;
+    vort = vadd(vort, smul(wj, ld3(st.vorticity, j)));
+    w += wj;
+  }
+  if (dom) st3(st.d_vort_smoothed, i, acc);
+  st3(st.vorticity_vis, i, vdivs(vort, w + 1e-6f));
 }
 
 // dimcv_sph.py:295-324 with host-fed uniforms u1/u2 per slot.
@@ -444,6 +525,52 @@ __global__ void k_life_time(dimcv_state st, dimcv_params p, int32_t bound) {
 
 int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t s);
 
+// Jacobi pieces gated by a device flag (iterations after convergence are
+// skipped on the device), used by the native substep.
+int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       const int32_t* done, hipStream_t s) {
+  if (fluid_bound <= 0) return DIMCV_OK;
+  dim3 g(nblocks(fluid_bound)), b(256);
+  switch (which) {
+    case 0: hipLaunchKernelGGL(k_div_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
+    case 1: hipLaunchKernelGGL(k_density_change, g, b, 0, s, *st, *p, fluid_bound, done); break;
+    case 2: hipLaunchKernelGGL(k_pressure_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
+    default: hipLaunchKernelGGL(k_density_adv, g, b, 0, s, *st, *p, fluid_bound, done); break;
+  }
+  DIMCV_LAUNCHED("jacobi pass");
+  return DIMCV_OK;
+}
+
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s) {
+  if (which == 1) {  // v*, vorticity, grad_v, d_vort: zero the non-fluid slots first
+    int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
+    if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
+    if (rc) return rc;
+  } else if (which == 2) {
+    int rc = zero_nonfluid(st, p, s, st->vorticity_vis, 3);
+    if (rc) return rc;
+  }
+  if (fluid_bound <= 0) return DIMCV_OK;
+  dim3 g(nblocks(fluid_bound)), b(256);
+  switch (which) {
+    case 0: hipLaunchKernelGGL(k_density_factor, g, b, 0, s, *st, *p, fluid_bound); break;
+    case 1: hipLaunchKernelGGL(k_vort_gradv<7>, g, b, 0, s, *st, *p, fluid_bound); break;
+    default: hipLaunchKernelGGL(k_smooth_vis, g, b, 0, s, *st, *p, fluid_bound); break;
+  }
+  DIMCV_LAUNCHED("fused pass");
+  return DIMCV_OK;
+}
+
+int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, int32_t fluid_bound,
+                    double* partials, const int32_t* done, hipStream_t s) {
+  unsigned nb = nblocks(fluid_bound);
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, fluid_bound, partials, done);
+    DIMCV_LAUNCHED("k_residual");
+  }
+  return reduce_partials_f64(partials, nb, st->dscal + DIMCV_D_RESIDUAL, s, done);
+}
+
 }  // namespace dimcv
 
 using namespace dimcv;
@@ -477,7 +604,7 @@ int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int
 }
 
 int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
@@ -496,10 +623,10 @@ int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, fl
   if (rc) return rc;
   unsigned nb = nblocks(fluid_bound);
   if (nb > 0) {
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, fluid_bound, ws.partials);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, fluid_bound, ws.partials, nullptr);
     DIMCV_LAUNCHED("k_residual");
   }
-  rc = reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_RESIDUAL, stream);
+  rc = reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_RESIDUAL, stream, nullptr);
   if (rc) return rc;
   if (host_sum) {
     DIMCV_HIP(hipMemcpyAsync(host_sum, st->dscal + DIMCV_D_RESIDUAL, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -509,7 +636,7 @@ int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, fl
 }
 
 int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_div_iter, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_div_iter, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
@@ -531,12 +658,12 @@ int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_
 }
 
 int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
 int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_pressure_iter, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_pressure_iter, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
@@ -628,7 +755,7 @@ int dimcv_compute_kernel_density(const dimcv_state* st, const dimcv_params* p, i
                        ws.partials);
     DIMCV_LAUNCHED("k_kernel_density");
   }
-  return reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_KDEN_SUM, stream);
+  return reduce_partials_f64(ws.partials, nb, st->dscal + DIMCV_D_KDEN_SUM, stream, nullptr);
 }
 
 int dimcv_update_particle_life_time(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,

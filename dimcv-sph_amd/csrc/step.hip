@@ -1,0 +1,148 @@
+// Native substep: the whole of KarmanVortexSolver.substep
+// (karman_vortex.py:232-242) issued from C++, with the DFSPH Jacobi loops
+// (dimcv_sph.py:376-393, 597-614) converging on the device.
+//
+// Loop semantics are the reference's: at least one and at most max
+// iterations; after iteration k the average residual (FP64 sum / fluid
+// count) is compared with eta.  Here the comparison runs in a one-thread
+// kernel that raises a device flag; every kernel of later iterations reads
+// the flag first and returns, so the host may issue `chunk` iterations
+// between synchronisations without changing any result.
+#include "common.h"
+
+namespace dimcv {
+
+int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       const int32_t* done, hipStream_t s);
+int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, int32_t fluid_bound,
+                    double* partials, const int32_t* done, hipStream_t s);
+void set_far_timing(hipEvent_t a, hipEvent_t b);
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
+
+// Events for the optional far-field timing (created on first use, kept for
+// the life of the process).
+static hipEvent_t g_step_ev[2] = {nullptr, nullptr};
+
+__global__ void k_jacobi_reset(int32_t* counters) {
+  counters[DIMCV_C_JACOBI_DONE] = 0;
+  counters[DIMCV_C_JACOBI_ITERS] = 0;
+}
+
+__global__ void k_jacobi_check(int32_t* counters, const double* sum, int32_t fluid_count, double eta,
+                               int32_t max_iter) {
+  if (counters[DIMCV_C_JACOBI_DONE]) return;
+  int32_t it = ++counters[DIMCV_C_JACOBI_ITERS];
+  double avg = *sum / (double)fluid_count;
+  if (avg <= eta || it >= max_iter) counters[DIMCV_C_JACOBI_DONE] = 1;
+}
+
+// which = 0: divergence (iteration, density change, residual offset 0)
+// which = 1: pressure (iteration, density_adv, residual offset rho0)
+static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                  int32_t fluid_bound, Workspace& ws, hipStream_t s, int32_t* iters_out) {
+  const int32_t* done = st->counters + DIMCV_C_JACOBI_DONE;
+  const int32_t max_iter = which == 0 ? o->max_iter_v : o->max_iter_p;
+  const double eta = which == 0 ? o->eta_v : o->eta_p;
+  const float offset = which == 0 ? 0.0f : o->density0;
+  const int chunk = o->chunk > 0 ? o->chunk : 1;
+  hipLaunchKernelGGL(k_jacobi_reset, dim3(1), dim3(1), 0, s, st->counters);
+  int32_t host[2] = {0, 0};
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s);
+      if (!rc) rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s);
+      if (!rc) rc = launch_residual(st, p, offset, fluid_bound, ws.partials, done, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(1), 0, s, st->counters, st->dscal + DIMCV_D_RESIDUAL,
+                         o->fluid_count, eta, max_iter);
+    }
+    DIMCV_LAUNCHED("jacobi chunk");
+    DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, 2 * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s));
+    DIMCV_HIP(hipStreamSynchronize(s));
+    if (host[0]) break;
+  }
+  *iters_out = host[1];
+  return DIMCV_OK;
+}
+
+}  // namespace dimcv
+
+using namespace dimcv;
+
+#define CHECK(call)        \
+  do {                     \
+    int rc_ = (call);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
+extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                             int32_t particle_bound, int32_t fluid_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!st || !p || !o) {
+    set_error("dimcv_substep: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->fluid_count <= 0) {
+    set_error("dimcv_substep: no fluid particles (the reference divides by fluid_particle_num)");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m == 0 && (!o->u1 || !o->u2)) {
+    set_error("dimcv_substep: Bernoulli uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m > 0 && !o->u_cdf) {
+    set_error("dimcv_substep: inverse-CDF uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  Workspace ws;
+  CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
+
+  // compute_densities + compute_DFSPH_factor in one pass
+  CHECK(launch_fused(0, st, p, fluid_bound, s));
+  // divergence_solve (dimcv_sph.py:376-393)
+  CHECK(dimcv_compute_density_change(st, p, fluid_bound, stream));
+  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->inv_dt, fluid_bound, stream));
+  CHECK(jacobi(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
+  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
+  CHECK(dimcv_compute_non_pressure_forces(st, p, particle_bound, fluid_bound, stream));
+  // dimcv() (dimcv_sph.py:812-819)
+  // init() + compute_d_vorticity in one pass, then the smoothing pass, which
+  // also produces vorticity_vis (compute_vorticity_vis below in the reference)
+  CHECK(launch_fused(1, st, p, fluid_bound, s));
+  CHECK(launch_fused(2, st, p, fluid_bound, s));
+  if (o->m > 0) {
+    CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
+    CHECK(dimcv_compute_kernel_density(st, p, fluid_bound, stream));
+    CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
+  } else {
+    CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
+    CHECK(dimcv_record_sample_idx(st, p, particle_bound, stream));
+    CHECK(dimcv_compute_kernel_density(st, p, fluid_bound, stream));
+  }
+  if (o->time_far) {
+    if (!g_step_ev[0]) {
+      DIMCV_HIP(hipEventCreate(&g_step_ev[0]));
+      DIMCV_HIP(hipEventCreate(&g_step_ev[1]));
+    }
+    set_far_timing(g_step_ev[0], g_step_ev[1]);
+  }
+  int rc_vv = dimcv_vort2vel(st, p, fluid_bound, (int32_t)p->n_max, stream);
+  set_far_timing(nullptr, nullptr);
+  CHECK(rc_vv);
+  CHECK(dimcv_update_particle_life_time(st, p, fluid_bound, stream));
+  CHECK(dimcv_predict_velocity(st, p, particle_bound, stream));
+  // pressure_solve (dimcv_sph.py:597-614)
+  CHECK(dimcv_compute_density_adv(st, p, fluid_bound, stream));
+  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->inv_dt2, fluid_bound, stream));
+  CHECK(jacobi(1, st, p, o, fluid_bound, ws, s, &o->out_kp));
+  CHECK(dimcv_copy_x_temp(st, p, particle_bound, stream));
+  CHECK(dimcv_advect(st, p, particle_bound, stream));
+  if (o->bounce_back) CHECK(dimcv_enforce_boundary_3d(st, p, particle_bound, stream));
+  if (o->time_far && p->far_mode != DIMCV_FAR_EXACT) {
+    // the pressure solve synchronised the stream after the far field
+    DIMCV_HIP(hipEventSynchronize(g_step_ev[1]));
+    DIMCV_HIP(hipEventElapsedTime(&o->out_far_ms, g_step_ev[0], g_step_ev[1]));
+  }
+  return DIMCV_OK;
+}

@@ -44,7 +44,7 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.radix_hist = (uint32_t*)take(4 * 256 * sort_blocks);
   w.scan_tmp = (uint32_t*)take(4 * (256 * sort_blocks / 1024 + 1024));
   w.compact_cnt = (uint32_t*)take(4 * (nb256 + 1024));
-  w.partials = (double*)take(8 * (nb256 + 256));
+  w.partials = (double*)take(8 * (nb256 + nb256 / 256 + 512));
   w.cdf = (double*)take(8 * n);
   w.cdf_weights = (double*)take(8 * n);
   w.block_tot = (double*)take(8 * (scan_blocks + 16));
@@ -86,8 +86,9 @@ int bind_workspace(const dimcv_state* st, const dimcv_params* p, Workspace* ws, 
 // Canonical FP64 reduction of an array of partials: repeated 256-wide halving
 // trees, one workgroup, level by level (same definition as the per-block
 // trees that produced the partials).
-__global__ void k_reduce_levels(double* a, int64_t n, double* out) {
+__global__ void k_reduce_levels(double* a, int64_t n, double* out, const int32_t* done) {
   __shared__ double lds[256];
+  if (done && *done) return;
   int64_t m = n;
   while (m > 1) {
     int64_t groups = (m + 255) / 256;
@@ -103,9 +104,28 @@ __global__ void k_reduce_levels(double* a, int64_t n, double* out) {
   if (threadIdx.x == 0) *out = (n > 0) ? a[0] : 0.0;
 }
 
-int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream) {
-  hipLaunchKernelGGL(k_reduce_levels, dim3(1), dim3(256), 0, (hipStream_t)stream,
-                     const_cast<double*>(partials), n, out);
+// One level of the canonical tree for all groups at once: group g of 256
+// consecutive values -> b[g].
+__global__ void k_reduce_groups(const double* __restrict__ a, int64_t n, double* __restrict__ b,
+                                const int32_t* done) {
+  __shared__ double lds[256];
+  if (done && *done) return;
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double s = block_tree_sum256(i < n ? a[i] : 0.0, lds);
+  if (threadIdx.x == 0) b[blockIdx.x] = s;
+}
+
+int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream, const int32_t* done) {
+  hipStream_t s = (hipStream_t)stream;
+  double* a = const_cast<double*>(partials);
+  if (n > 256) {
+    // level 2 for all groups in parallel, written after the partials
+    int64_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_reduce_groups, dim3((unsigned)g), dim3(256), 0, s, a, n, a + n, done);
+    a = a + n;
+    n = g;
+  }
+  hipLaunchKernelGGL(k_reduce_levels, dim3(1), dim3(256), 0, s, a, n, out, done);
   DIMCV_LAUNCHED("k_reduce_levels");
   return DIMCV_OK;
 }

@@ -105,51 +105,94 @@ __global__ void k_pack_sources(dimcv_state st, const int32_t* __restrict__ idx, 
   out[2 * k + 1] = make_float4(w.x, w.y, w.z, 0.0f);
 }
 
-// FAST / KAHAN far field.  Sums sum_s 1/r^2 (r_xy x c_s) over accepted s.
+// FAST / KAHAN far field: sum_s 1/r^2 (r_xy x c_s) over accepted samples s.
+//
+// Common path (no lane of the wave near the cut-off): r^2 with FMAs, one
+// reciprocal shared by two samples (1/a = b / (ab)), four FMAs into the
+// accumulators, no masking.  A wave takes the exact path for a sample pair
+// when any lane has r^2 <= T_hi = T (1 + 2^-20), a margin far wider than the
+// few-ulp difference between the FMA and the reference r^2: there every lane
+// forms r^2 in the reference order, accepts iff r^2 > T (<=> sqrt_rn(r^2) >
+// h), and counts rejections, so the accepted set and n_i are exact.
+template <bool KAHAN>
+__device__ __forceinline__ void acc_add(float& s, float& comp, float term) {
+  if (KAHAN) {
+    float y = term - comp, t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  } else {
+    s += term;
+  }
+}
+
+template <bool PERIODIC>
+__device__ __forceinline__ void far_delta(v3 xi, float4 a, v3 L, float& dx, float& dy, float& dz) {
+  dx = xi.x - a.x;
+  dy = xi.y - a.y;
+  dz = xi.z - a.z;
+  if (PERIODIC) {
+    const float hx = 0.5f * L.x, hy = 0.5f * L.y, hz = 0.5f * L.z;
+    dx = dx > hx ? dx - L.x : (dx < -hx ? dx + L.x : dx);
+    dy = dy > hy ? dy - L.y : (dy < -hy ? dy + L.y : dy);
+    dz = dz > hz ? dz - L.z : (dz < -hz ? dz + L.z : dz);
+  }
+}
+
+template <bool KAHAN>
+__device__ __forceinline__ void far_accumulate(float dx, float dy, float inv, float4 c, float* A, float* C) {
+  const float t1 = dy * inv, t2 = dx * inv;
+  if (KAHAN) {
+    acc_add<true>(A[0], C[0], t1 * c.z);
+    acc_add<true>(A[1], C[1], -(t2 * c.z));
+    acc_add<true>(A[2], C[2], fmaf(t2, c.y, -(t1 * c.x)));
+  } else {
+    A[0] = fmaf(t1, c.z, A[0]);
+    A[1] = fmaf(-t2, c.z, A[1]);
+    A[2] = fmaf(t2, c.y, A[2]);
+    A[2] = fmaf(-t1, c.x, A[2]);
+  }
+}
+
+template <bool KAHAN, bool PERIODIC>
+__device__ __forceinline__ void far_exact_sample(v3 xi, float4 a, float4 c, float T, v3 L, float* A, float* C,
+                                                 int& rej) {
+  float dx, dy, dz;
+  far_delta<PERIODIC>(xi, a, L, dx, dy, dz);
+  float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+  bool ok = r2 > T;
+  float inv = ok ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+  rej += ok ? 0 : 1;
+  far_accumulate<KAHAN>(dx, dy, inv, c, A, C);
+}
+
 template <bool KAHAN, bool PERIODIC>
 __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v3 xi, float T, v3 L, v3& acc,
                                         int& added) {
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  float cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
-  int n = 0;
-#pragma unroll 4
-  for (int js = 0; js < M; ++js) {
-    const float4 a = src[2 * js];
-    const float4 c = src[2 * js + 1];
-    float dx = xi.x - a.x, dy = xi.y - a.y, dz = xi.z - a.z;
-    if (PERIODIC) {
-      const float hx = 0.5f * L.x, hy = 0.5f * L.y, hz = 0.5f * L.z;
-      dx = dx > hx ? dx - L.x : (dx < -hx ? dx + L.x : dx);
-      dy = dy > hy ? dy - L.y : (dy < -hy ? dy + L.y : dy);
-      dz = dz > hz ? dz - L.z : (dz < -hz ? dz + L.z : dz);
-    }
-    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-    bool ok = r2 > T;
-    float inv = ok ? __builtin_amdgcn_rcpf(r2) : 0.0f;
-    n += ok ? 1 : 0;
-    float t1 = dy * inv, t2 = dx * inv;
-    if (KAHAN) {
-      float ex = t1 * c.z;
-      float ey = -(t2 * c.z);
-      float ez = fmaf(t2, c.y, -(t1 * c.x));
-      float y0 = ex - cx_, s0 = ax + y0;
-      cx_ = (s0 - ax) - y0;
-      ax = s0;
-      float y1 = ey - cy_, s1 = ay + y1;
-      cy_ = (s1 - ay) - y1;
-      ay = s1;
-      float y2 = ez - cz_, s2 = az + y2;
-      cz_ = (s2 - az) - y2;
-      az = s2;
+  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+  int rej = 0;
+  const float T_hi = T * (1.0f + 9.5367431640625e-07f);
+  int js = 0;
+#pragma unroll 2
+  for (; js + 1 < M; js += 2) {
+    const float4 a0 = src[2 * js], c0 = src[2 * js + 1];
+    const float4 a1 = src[2 * js + 2], c1 = src[2 * js + 3];
+    float dx0, dy0, dz0, dx1, dy1, dz1;
+    far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
+    far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
+    const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+    const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+    if (__any(fminf(r20, r21) <= T_hi)) {
+      far_exact_sample<KAHAN, PERIODIC>(xi, a0, c0, T, L, A, C, rej);
+      far_exact_sample<KAHAN, PERIODIC>(xi, a1, c1, T, L, A, C, rej);
     } else {
-      ax = fmaf(t1, c.z, ax);
-      ay = fmaf(-t2, c.z, ay);
-      az = fmaf(t2, c.y, az);
-      az = fmaf(-t1, c.x, az);
+      const float q = __builtin_amdgcn_rcpf(r20 * r21);
+      far_accumulate<KAHAN>(dx0, dy0, r21 * q, c0, A, C);
+      far_accumulate<KAHAN>(dx1, dy1, r20 * q, c1, A, C);
     }
   }
-  acc = mk(ax, ay, az);
-  added = n;
+  if (js < M) far_exact_sample<KAHAN, PERIODIC>(xi, src[2 * js], src[2 * js + 1], T, L, A, C, rej);
+  acc = mk(A[0], A[1], A[2]);
+  added = M - rej;
 }
 
 template <bool KAHAN, bool PERIODIC>
@@ -488,6 +531,20 @@ int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fl
   return DIMCV_OK;
 }
 
+}  // extern "C"
+
+namespace dimcv {
+// Optional HIP events recorded around the far-field kernel launch (used by
+// dimcv_substep's timing option; null when not timing).
+static hipEvent_t g_far_ev[2] = {nullptr, nullptr};
+void set_far_timing(hipEvent_t a, hipEvent_t b) {
+  g_far_ev[0] = a;
+  g_far_ev[1] = b;
+}
+}  // namespace dimcv
+
+extern "C" {
+
 int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, int32_t max_samples,
                        void* stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -508,8 +565,10 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
     auto kern = p->far_mode == DIMCV_FAR_KAHAN
                     ? (p->periodic ? k_vv_far_fast<true, true> : k_vv_far_fast<true, false>)
                     : (p->periodic ? k_vv_far_fast<false, true> : k_vv_far_fast<false, false>);
+    if (g_far_ev[0]) DIMCV_HIP(hipEventRecord(g_far_ev[0], s));
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s, ws.samples,
                        (int)max_samples);
+    if (g_far_ev[1]) DIMCV_HIP(hipEventRecord(g_far_ev[1], s));
   }
   DIMCV_LAUNCHED("vort2vel far field");
   return DIMCV_OK;

@@ -17,6 +17,8 @@ config_builder.py for the keys):
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -86,6 +88,8 @@ class DIMCVSPHSolver(SPHBase):
         self._uniforms = (None, None, None)
         self.last_iterations_v = 0
         self.last_iterations_p = 0
+        self.time_far_field = False   # record HIP events around the far-field kernel
+        self.last_far_ms = None
 
         p = ps.params
         f32 = np.float32
@@ -331,6 +335,53 @@ class DIMCVSPHSolver(SPHBase):
             self.importance_sample(u_cdf)
         self.vort2vel()
         self.update_particle_life_time()
+
+    # ------------------------------------------------- native substep --
+    def native_substep(self, u1=None, u2=None, u_cdf=None):
+        """substep() issued by libdimcv in one call (dimcv_substep): same kernels
+        in the same order; the Jacobi loops converge on the device and the host
+        checks every ``jacobiChunk`` iterations."""
+        ps = self.ps
+        o = N.StepOpts()
+        keep = []
+        if self.mc_samples:
+            m = int(self.mc_samples)
+            if u_cdf is None:
+                u_cdf = self.rng.random(m)
+            t = self._upload(np.asarray(u_cdf)[:m], torch.float64)
+            keep.append(t)
+            o.u_cdf, o.m = t.data_ptr(), m
+        else:
+            pn = ps._pn
+            if u1 is None:
+                u1 = self.rng.random(pn, dtype=np.float32)
+            if u2 is None:
+                u2 = self.rng.random(pn, dtype=np.float32)
+            assert len(u1) >= pn and len(u2) >= pn
+            t1, t2 = self._upload(u1, torch.float32), self._upload(u2, torch.float32)
+            keep += [t1, t2]
+            o.u1, o.u2, o.m = t1.data_ptr(), t2.data_ptr(), 0
+        o.max_iter_v = self.m_max_iterations_v
+        o.max_iter_p = self.m_max_iterations
+        o.chunk = int(ps.cfg.get_cfg("jacobiChunk") or 2)
+        o.eta_v = self.inv_dt * self.max_error_V * 0.01 * self.density_0
+        o.eta_p = self.max_error * 0.01 * self.density_0
+        o.inv_dt = np.float32(self.inv_dt)
+        o.dt = np.float32(self.dt[None])
+        o.inv_dt2 = np.float32(self.inv_dt2)
+        o.density0 = np.float32(self.density_0)
+        o.fluid_count = self._fluid_count()
+        if o.fluid_count <= 0:
+            raise ZeroDivisionError("no fluid particles (fluid_particle_num is 0)")
+        o.bounce_back = 0
+        o.time_far = 1 if self.time_far_field else 0
+        ps.ensure_neighbor_lists()
+        ps.call("dimcv_substep", ctypes.byref(o), ps._pn, ps._nfl, ps.stream())
+        self.last_iterations_v = int(o.out_kv)
+        self.last_iterations_p = int(o.out_kp)
+        self.last_far_ms = float(o.out_far_ms) if self.time_far_field else None
+        ps._nbr_valid = False
+        del keep
 
     # --------------------------------------------------- comparators --
     def direct_biot_savart(self, precision: str = "f32"):

@@ -2,6 +2,8 @@
 
 ``KarmanVortexSolver`` keeps the cylinder carving (karman_vortex.py:10-36),
 the step/substep order (:216-242) and the exporters (:38-214).  ``step``
+issues the substep through one native call (``dimcv_substep``) unless
+``use_native_substep`` is False or a subclass overrides ``substep``.  ``step``
 accepts optional host uniforms so that two implementations can be driven with
 identical random inputs (``u1``/``u2``: per-slot f32 arrays for the Bernoulli
 sample update; ``u_cdf``: f64 array of M uniforms for inverse-CDF sampling).
@@ -21,6 +23,9 @@ class KarmanVortexSolver(DIMCVSPHSolver):
         self.circle_vis = np.array([self.circle_pos[0] * 0.25, self.circle_pos[1], self.circle_pos[2]],
                                    dtype=np.float32)
         self.circle_radius = 0.1
+        # step() issues the substep through libdimcv in one call unless a
+        # subclass overrides substep() (then the Python pass sequence runs)
+        self.use_native_substep = True
         self.init_cylinder()
 
     def init_cylinder(self):
@@ -108,11 +113,14 @@ class KarmanVortexSolver(DIMCVSPHSolver):
         self.cnt += 1
         self.ps.initialize_particle_system()
         self.compute_moving_boundary_volume()
-        self._uniforms = (u1, u2, u_cdf)
-        try:
-            self.substep()
-        finally:
-            self._uniforms = (None, None, None)
+        if self.use_native_substep and type(self).substep is KarmanVortexSolver.substep:
+            self.native_substep(u1, u2, u_cdf)
+        else:
+            self._uniforms = (u1, u2, u_cdf)
+            try:
+                self.substep()
+            finally:
+                self._uniforms = (None, None, None)
         if self.ps.cfg.get_cfg("bounceBackBoundary"):
             self.enforce_boundary_3D(self.ps.material_fluid)
 

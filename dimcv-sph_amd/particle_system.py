@@ -206,6 +206,7 @@ class ParticleSystem:
         p.bounce_padding = f32(self.padding)
         p.m_V0 = f32(self.m_V0)
         p.far_r2_max_rejected = float(far_threshold(f32(self.support_radius)))
+        p.nbr_r2_limit = float(near_threshold(f32(self.support_radius)))
 
     def state(self) -> N.State:
         """The dimcv_state of the current buffers (cached; rebuilt after a swap)."""
@@ -350,4 +351,16 @@ def far_threshold(h: np.float32) -> np.float32:
         t = np.nextafter(t, inf, dtype=np.float32)
     while np.sqrt(t) > h:
         t = np.nextafter(t, np.float32(0), dtype=np.float32)
+    return t
+
+
+def near_threshold(h: np.float32) -> np.float32:
+    """Smallest f32 t with sqrt_rn(t) >= h, so that sqrt(r2) < h  <=>  r2 < t."""
+    h = np.float32(h)
+    t = np.float32(h * h)
+    zero = np.float32(0)
+    while np.sqrt(np.nextafter(t, zero, dtype=np.float32)) >= h:
+        t = np.nextafter(t, zero, dtype=np.float32)
+    while np.sqrt(t) < h:
+        t = np.nextafter(t, np.float32(np.inf), dtype=np.float32)
     return t

@@ -54,6 +54,8 @@ typedef enum dimcv_counter {
   DIMCV_C_KEY_ERROR = 6,      /* active particles with an out-of-grid cell key */
   DIMCV_C_PREV_NUM = 7,       /* particle count before the last activity update */
   DIMCV_C_SAMPLE_SLOTS = 8,   /* is_sample count (compaction scratch) */
+  DIMCV_C_JACOBI_DONE = 9,    /* native substep: current solve has converged */
+  DIMCV_C_JACOBI_ITERS = 10,  /* native substep: iterations of the current solve */
   DIMCV_C_COUNT = 16
 } dimcv_counter;
 
@@ -112,6 +114,7 @@ typedef struct dimcv_params {
   float bounce_hi[3];           /* f32(domain_size - padding), same kernel */
   int32_t far_mode;             /* dimcv_far_mode */
   float m_V0;                   /* f32(0.8 (2 r)^3)             particle_system.py:46 */
+  float nbr_r2_limit;           /* smallest f32 t with sqrt_rn(t) >= support: |r| < h <=> r.r < t */
   double vp_factor_f64;         /* 0.1592 * volumeScale * m_V0 in FP64 (direct FP64 sum) */
 } dimcv_params;
 
@@ -296,6 +299,41 @@ int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fl
 int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        int32_t max_samples, void* stream);
 int dimcv_update_particle_life_time(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :753-760 */
+
+/* ---- native substep ---------------------------------------------------------- */
+/* Options of dimcv_substep.  The host-side scalars are the values the
+ * reference's Python computes (f64 arithmetic, f32 kernel arguments). */
+typedef struct dimcv_step_opts {
+  const float* u1;          /* Bernoulli uniforms per slot (device) or NULL in fixed-M mode */
+  const float* u2;
+  const double* u_cdf;      /* fixed-M mode: m uniforms (device) */
+  int32_t m;                /* fixed-M sample count; 0 selects the Bernoulli sample set */
+  int32_t max_iter_v;       /* m_max_iterations_v (dimcv_sph.py:22) */
+  int32_t max_iter_p;       /* m_max_iterations (:23) */
+  int32_t chunk;            /* Jacobi iterations issued per host convergence check (>= 1) */
+  double eta_v;             /* inv_dt * 0.1 * 0.01 * rho0 (:389) */
+  double eta_p;             /* 0.05 * 0.01 * rho0 (:611) */
+  float inv_dt;             /* f32(1 / dt) (:27, :381) */
+  float dt;                 /* f32(dt) (:393) */
+  float inv_dt2;            /* f32(1 / dt^2) (:28, :602) */
+  float density0;           /* offset of the pressure residual (:588) */
+  int32_t fluid_count;      /* fluid_particle_num (residual divisor, :589,595) */
+  int32_t bounce_back;      /* enforce_boundary_3D after the substep (karman_vortex.py:226-230) */
+  int32_t out_kv;           /* output: divergence iterations */
+  int32_t out_kp;           /* output: pressure iterations */
+  int32_t time_far;         /* 1: time the far-field kernel with HIP events on `stream` */
+  float out_far_ms;         /* output: far-field kernel duration (time_far = 1) */
+} dimcv_step_opts;
+
+/* KarmanVortexSolver.substep (karman_vortex.py:232-242) with the two Jacobi
+ * solves (dimcv_sph.py:376-393, 597-614) run inside the library: each
+ * iteration's kernels are skipped on the device once the solve has
+ * converged, and the host checks convergence every `chunk` iterations
+ * (chunk = 1 is the reference's cadence).  The kernels and their order are
+ * those of the per-pass entries, so results are identical to calling them
+ * one by one.  Requires the neighbour lists of the current positions. */
+int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
+                  int32_t particle_bound, int32_t fluid_bound, void* stream);
 
 /* ---- comparators and initial conditions (build-defined) --------------------- */
 /* Direct O(N^2) Biot-Savart over every in-domain fluid particle: the sum the
