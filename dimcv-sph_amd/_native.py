@@ -104,12 +104,25 @@ class StepOpts(ctypes.Structure):
     ]
 
 
+class Scene(ctypes.Structure):
+    """dimcv_scene (include/dimcv.h)."""
+    _fields_ = [("grid_num", I3), ("cell_base", ctypes.c_int32), ("n_cells", ctypes.c_int32),
+                ("m", ctypes.c_int32), ("sample_base", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("domain_start", F3), ("domain_end", F3), ("visc_coef", ctypes.c_float),
+                ("pad1", ctypes.c_float)]
+
+
 class State(ctypes.Structure):
     """dimcv_state (include/dimcv.h)."""
     _fields_ = ([(n, ctypes.c_void_p) for n in STATE_POINTERS]
                 + [("n_emitters", ctypes.c_int32), ("pad0", ctypes.c_int32)]
                 + [(n, ctypes.c_void_p) for n in STATE_POINTERS_2]
-                + [("workspace_bytes", ctypes.c_int64)])
+                + [("workspace_bytes", ctypes.c_int64)]
+                + [("scene_id", ctypes.c_void_p), ("scene_id_buffer", ctypes.c_void_p),
+                   ("scenes", ctypes.c_void_p), ("n_scenes", ctypes.c_int32), ("pad1", ctypes.c_int32),
+                   ("scene_fluid_start", ctypes.c_void_p), ("scene_counters", ctypes.c_void_p),
+                   ("scene_dscal", ctypes.c_void_p), ("seg_blocks", ctypes.c_void_p),
+                   ("n_seg_blocks", ctypes.c_int32), ("pad2", ctypes.c_int32)])
 
 
 def lib():

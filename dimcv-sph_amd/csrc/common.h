@@ -61,6 +61,8 @@ struct Workspace {
   double* block_pre;      // scan of block totals
   float* near3;           // vort2vel near field per fluid entry (3 floats)
   float* scale_s;         // vort2vel S_i per fluid entry
+  void* far_partials;     // split far-field partial sums (32 B each)
+  int64_t far_partial_cap;
   float4* samples;        // packed far-field sources: (x, y, z, -) (cx, cy, cz, -)
   int64_t max_samples;
   int64_t bytes;

@@ -165,34 +165,115 @@ __device__ __forceinline__ void far_exact_sample(v3 xi, float4 a, float4 c, floa
   far_accumulate<KAHAN>(dx, dy, inv, c, A, C);
 }
 
+// Sum over sources [j0, j1).  The next pair's sample data are loaded before
+// the current pair is evaluated (software pipelining of the scalar loads).
+template <bool KAHAN, bool PERIODIC>
+__device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, int j0, int j1, v3 xi, float T, v3 L,
+                                              float* A, float* C, int& rej) {
+  const float T_hi = T * (1.0f + 9.5367431640625e-07f);
+  int js = j0;
+  if (js + 1 < j1) {
+    float4 a0 = src[2 * js], c0 = src[2 * js + 1];
+    float4 a1 = src[2 * js + 2], c1 = src[2 * js + 3];
+    for (; js + 1 < j1; js += 2) {
+      const int nx = js + 3 < j1 ? js + 2 : js;  // in-range prefetch index
+      const float4 na0 = src[2 * nx], nc0 = src[2 * nx + 1];
+      const float4 na1 = src[2 * nx + 2], nc1 = src[2 * nx + 3];
+      float dx0, dy0, dz0, dx1, dy1, dz1;
+      far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
+      far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
+      const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+      const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+      if (__builtin_expect(__any(fminf(r20, r21) <= T_hi), 0)) {
+        far_exact_sample<KAHAN, PERIODIC>(xi, a0, c0, T, L, A, C, rej);
+        far_exact_sample<KAHAN, PERIODIC>(xi, a1, c1, T, L, A, C, rej);
+      } else {
+        const float q = __builtin_amdgcn_rcpf(r20 * r21);
+        far_accumulate<KAHAN>(dx0, dy0, r21 * q, c0, A, C);
+        far_accumulate<KAHAN>(dx1, dy1, r20 * q, c1, A, C);
+      }
+      a0 = na0; c0 = nc0; a1 = na1; c1 = nc1;
+    }
+  }
+  if (js < j1) far_exact_sample<KAHAN, PERIODIC>(xi, src[2 * js], src[2 * js + 1], T, L, A, C, rej);
+}
+
 template <bool KAHAN, bool PERIODIC>
 __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v3 xi, float T, v3 L, v3& acc,
                                         int& added) {
   float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
   int rej = 0;
-  const float T_hi = T * (1.0f + 9.5367431640625e-07f);
-  int js = 0;
-#pragma unroll 2
-  for (; js + 1 < M; js += 2) {
-    const float4 a0 = src[2 * js], c0 = src[2 * js + 1];
-    const float4 a1 = src[2 * js + 2], c1 = src[2 * js + 3];
-    float dx0, dy0, dz0, dx1, dy1, dz1;
-    far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
-    far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
-    const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
-    const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-    if (__any(fminf(r20, r21) <= T_hi)) {
-      far_exact_sample<KAHAN, PERIODIC>(xi, a0, c0, T, L, A, C, rej);
-      far_exact_sample<KAHAN, PERIODIC>(xi, a1, c1, T, L, A, C, rej);
-    } else {
-      const float q = __builtin_amdgcn_rcpf(r20 * r21);
-      far_accumulate<KAHAN>(dx0, dy0, r21 * q, c0, A, C);
-      far_accumulate<KAHAN>(dx1, dy1, r20 * q, c1, A, C);
-    }
-  }
-  if (js < M) far_exact_sample<KAHAN, PERIODIC>(xi, src[2 * js], src[2 * js + 1], T, L, A, C, rej);
+  far_sum_range<KAHAN, PERIODIC>(src, 0, M, xi, T, L, A, C, rej);
   acc = mk(A[0], A[1], A[2]);
   added = M - rej;
+}
+
+// Split form for small target counts: workgroup (b, s) sums sources
+// [s*chunk, (s+1)*chunk) for targets of block b and stores the partial
+// (A, compensation, rejections); k_far_combine adds the partials in split
+// order.
+struct FarPartial {
+  float a[3];
+  float c[3];
+  int rej;
+  int pad;
+};
+
+template <bool KAHAN, bool PERIODIC>
+__global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params p, int32_t bound,
+                                                   const float4* __restrict__ src, const int32_t* __restrict__ msrc,
+                                                   int cap, int splits, FarPartial* __restrict__ part) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  const int M = min(*msrc, cap);
+  const int chunk = (M + splits - 1) / splits;
+  const int sidx = blockIdx.y;
+  const int j0 = min(sidx * chunk, M), j1 = min(j0 + chunk, M);
+  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+  int rej = 0;
+  far_sum_range<KAHAN, PERIODIC>(src, j0, j1, xi, p.far_r2_max_rejected,
+                                 mk(p.box_size[0], p.box_size[1], p.box_size[2]), A, C, rej);
+  FarPartial& o = part[(int64_t)sidx * bound + f];
+  for (int d = 0; d < 3; ++d) {
+    o.a[d] = A[d];
+    o.c[d] = C[d];
+  }
+  o.rej = rej;
+}
+
+// Combine the partials: naive sums in split order, or Kahan sums carrying
+// each partial's compensation.  mode 0: MC far field (scale by S_i, gate into
+// v); mode 1: direct sum into v_vort.
+template <bool KAHAN>
+__global__ void k_far_combine(dimcv_state st, dimcv_params p, int32_t bound, const int32_t* __restrict__ msrc, int cap,
+                              int splits, const FarPartial* __restrict__ part, const float* __restrict__ near3,
+                              const float* __restrict__ scale_s, int mode) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+  int rej = 0;
+  for (int s2 = 0; s2 < splits; ++s2) {
+    const FarPartial& q = part[(int64_t)s2 * bound + f];
+    for (int d = 0; d < 3; ++d) {
+      if (KAHAN) {
+        acc_add<true>(A[d], C[d], q.a[d]);
+        acc_add<true>(A[d], C[d], -q.c[d]);
+      } else {
+        A[d] += q.a[d];
+      }
+    }
+    rej += q.rej;
+  }
+  const int M = min(*msrc, cap);
+  v3 acc = mk(A[0], A[1], A[2]);
+  if (mode == 0) vv_finish(st, p, i, ld3(near3, f), vmuls(acc, scale_s[f]), M - rej);
+  else st3(st.v_vort, i, vadd(ld3(near3, f), vmuls(acc, p.vp_factor)));
 }
 
 template <bool KAHAN, bool PERIODIC>
@@ -482,6 +563,32 @@ __global__ void __launch_bounds__(256) k_compact_write2(Pred pred, int64_t n, co
   if (f) out[pre + (uint32_t)__popcll(b & lt)] = (int32_t)i;
 }
 
+// Source splits so that a small target set still fills the GPU: about
+// 16 waves per SIMD (256 CUs x 4 SIMDs), bounded by the partials workspace.
+static int far_splits(int64_t targets, const Workspace& ws) {
+  const int64_t want_waves = 256LL * 4 * 16;
+  int64_t waves = (targets + 63) / 64;
+  int64_t s = waves > 0 ? (want_waves + waves - 1) / waves : 1;
+  if (s > 16) s = 16;
+  while (s > 1 && s * targets > ws.far_partial_cap) --s;
+  return s < 1 ? 1 : (int)s;
+}
+
+static int launch_far_split(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, const float4* src,
+                            const int32_t* msrc, int cap, int splits, bool kahan, Workspace& ws, int mode,
+                            hipStream_t s) {
+  FarPartial* part = (FarPartial*)ws.far_partials;
+  dim3 g(nblocks(fluid_bound), splits);
+  auto kern = kahan ? (p->periodic ? k_far_split<true, true> : k_far_split<true, false>)
+                    : (p->periodic ? k_far_split<false, true> : k_far_split<false, false>);
+  hipLaunchKernelGGL(kern, g, dim3(256), 0, s, *st, *p, fluid_bound, src, msrc, cap, splits, part);
+  auto comb = kahan ? k_far_combine<true> : k_far_combine<false>;
+  hipLaunchKernelGGL(comb, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, msrc, cap, splits,
+                     (const FarPartial*)part, ws.near3, ws.scale_s, mode);
+  DIMCV_LAUNCHED("far field (split)");
+  return DIMCV_OK;
+}
+
 }  // namespace dimcv
 
 using namespace dimcv;
@@ -562,12 +669,19 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
   } else {
     hipLaunchKernelGGL(k_pack_sources, dim3(nblocks(max_samples)), dim3(256), 0, s, *st, st->sample_idx,
                        st->counters + DIMCV_C_NUM_SAMPLES, (int64_t)max_samples, 1, ws.samples);
-    auto kern = p->far_mode == DIMCV_FAR_KAHAN
-                    ? (p->periodic ? k_vv_far_fast<true, true> : k_vv_far_fast<true, false>)
-                    : (p->periodic ? k_vv_far_fast<false, true> : k_vv_far_fast<false, false>);
+    const int splits = far_splits(fluid_bound, ws);
+    const bool kahan = p->far_mode == DIMCV_FAR_KAHAN;
     if (g_far_ev[0]) DIMCV_HIP(hipEventRecord(g_far_ev[0], s));
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s, ws.samples,
-                       (int)max_samples);
+    if (splits == 1) {
+      auto kern = kahan ? (p->periodic ? k_vv_far_fast<true, true> : k_vv_far_fast<true, false>)
+                        : (p->periodic ? k_vv_far_fast<false, true> : k_vv_far_fast<false, false>);
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s, ws.samples,
+                         (int)max_samples);
+    } else {
+      rc = launch_far_split(st, p, fluid_bound, ws.samples, st->counters + DIMCV_C_NUM_SAMPLES, (int)max_samples,
+                            splits, kahan, ws, 0, s);
+      if (rc) return rc;
+    }
     if (g_far_ev[1]) DIMCV_HIP(hipEventRecord(g_far_ev[1], s));
   }
   DIMCV_LAUNCHED("vort2vel far field");
@@ -610,9 +724,15 @@ int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precis
                        p->vp_factor_f64);
   } else {
     hipLaunchKernelGGL(k_vv_near, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.scale_s);
-    auto kern = precision == 1 ? (p->periodic ? k_direct_f32<true, true> : k_direct_f32<true, false>)
-                               : (p->periodic ? k_direct_f32<false, true> : k_direct_f32<false, false>);
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.samples, nsrc);
+    const int splits = far_splits(fluid_bound, ws);
+    if (splits == 1) {
+      auto kern = precision == 1 ? (p->periodic ? k_direct_f32<true, true> : k_direct_f32<true, false>)
+                                 : (p->periodic ? k_direct_f32<false, true> : k_direct_f32<false, false>);
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, *st, *p, fluid_bound, ws.near3, ws.samples, nsrc);
+    } else {
+      rc = launch_far_split(st, p, fluid_bound, ws.samples, nsrc, (int)p->n_max, splits, precision == 1, ws, 1, s);
+      if (rc) return rc;
+    }
   }
   DIMCV_LAUNCHED("direct Biot-Savart");
   return DIMCV_OK;

@@ -118,6 +118,22 @@ typedef struct dimcv_params {
   double vp_factor_f64;         /* 0.1592 * volumeScale * m_V0 in FP64 (direct FP64 sum) */
 } dimcv_params;
 
+/* One scene of a batch (build-defined batching).  All scenes of a batch
+ * share the particle radius, rest density, time step and far-field mode;
+ * each has its own box, cell grid, viscosity and sample count. */
+typedef struct dimcv_scene {
+  int32_t grid_num[3];
+  int32_t cell_base;        /* first global cell of the scene */
+  int32_t n_cells;
+  int32_t m;                /* importance samples per step */
+  int32_t sample_base;      /* offset of the scene's samples in sample_idx / u */
+  int32_t pad0;
+  float domain_start[3];
+  float domain_end[3];
+  float visc_coef;          /* f32(2 (dim+2) nu_scene) */
+  float pad1;
+} dimcv_scene;
+
 /* Device buffers.  Field names follow the reference (particle_system.py:143-220,
  * dimcv_sph.py:30-53).  Fields marked [sorted] are permuted by the grid build:
  * the build writes the permuted copy into the matching *_buffer pointer and the
@@ -188,6 +204,18 @@ typedef struct dimcv_state {
   double* dscal;             /* [DIMCV_D_COUNT] */
   void* workspace;
   int64_t workspace_bytes;
+  /* batching (all NULL / 0 for a single scene) */
+  int32_t* scene_id;           /* [sorted] scene of each slot */
+  int32_t* scene_id_buffer;
+  const dimcv_scene* scenes;   /* [n_scenes] (device) */
+  int32_t n_scenes;
+  int32_t pad1;
+  int32_t* scene_fluid_start;  /* [n_scenes + 1] fluid-list segment of each scene */
+  int32_t* scene_counters;     /* [n_scenes][4]: done, iterations, num_samples, fluid count */
+  double* scene_dscal;         /* [n_scenes][4]: kden_sum, residual, CDF total, - */
+  const int32_t* seg_blocks;   /* [n_seg_blocks][2]: scene, first fluid entry (256-entry blocks) */
+  int32_t n_seg_blocks;
+  int32_t pad2;
 } dimcv_state;
 
 /* ---- library ---------------------------------------------------------------- */

@@ -41,8 +41,9 @@ def test_library_version_and_workspace_query():
     big = L.dimcv_workspace_bytes(1_048_576, 131_072, 1_048_576)
     assert 0 < small < big
     assert big % 256 == 0
-    # keys/values 16 B, CDF 16 B, near field 16 B, sample table 32 B per particle
-    assert 64 * 1_048_576 < big < 128 * 1_048_576
+    # keys/values 16 B, CDF 16 B, near field 16 B, sample table 32 B per
+    # particle, plus at most 2^21 split partials of 32 B
+    assert 80 * 1_048_576 <= big <= 80 * 1_048_576 + 32 * 2 ** 21 + (1 << 20)
 
 
 def _probe_layout():
@@ -54,12 +55,15 @@ def _probe_layout():
 int main(void) {
   printf("sizeof.params %zu\n", sizeof(dimcv_params));
   printf("sizeof.state %zu\n", sizeof(dimcv_state));
+  printf("sizeof.scene %zu\n", sizeof(dimcv_scene));
   O(dimcv_params, n_max) O(dimcv_params, grid_size) O(dimcv_params, gravity)
   O(dimcv_params, has_vortex_domain) O(dimcv_params, vp_factor) O(dimcv_params, bounce_hi)
   O(dimcv_params, far_mode) O(dimcv_params, m_V0) O(dimcv_params, vp_factor_f64)
   O(dimcv_state, grid_ids) O(dimcv_state, grid_ids_buffer) O(dimcv_state, num_particles_each_emitter)
   O(dimcv_state, n_emitters) O(dimcv_state, fluid_list) O(dimcv_state, workspace)
-  O(dimcv_state, workspace_bytes)
+  O(dimcv_state, workspace_bytes) O(dimcv_state, scene_id) O(dimcv_state, n_scenes)
+  O(dimcv_state, scene_fluid_start) O(dimcv_state, seg_blocks) O(dimcv_state, n_seg_blocks)
+  O(dimcv_scene, cell_base) O(dimcv_scene, domain_start) O(dimcv_scene, visc_coef)
   return 0;
 }
 '''
@@ -77,9 +81,10 @@ def test_ctypes_structs_match_c_layout():
     lay = _probe_layout()
     assert ctypes.sizeof(N.Params) == lay["sizeof.params"]
     assert ctypes.sizeof(N.State) == lay["sizeof.state"]
+    assert ctypes.sizeof(N.Scene) == lay["sizeof.scene"]
     for key, off in lay.items():
         if key.startswith("sizeof"):
             continue
         tname, field = key.split(".")
-        cls = N.Params if tname == "dimcv_params" else N.State
+        cls = {"dimcv_params": N.Params, "dimcv_state": N.State, "dimcv_scene": N.Scene}[tname]
         assert getattr(cls, field).offset == off, key
