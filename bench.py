@@ -19,7 +19,11 @@ Also measured in the same run:
   * the CPU oracle (a C++ restatement of the reference step, OpenMP) on a
     bounded sample of the same 1M workload.
 
+  * batched scenes: 64 uneven ring scenes stepped by one batched, graph-
+    replayed step against the same scenes stepped one after another.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                       [--no-direct] [--no-batching] [--batch-steps K]
 """
 from __future__ import annotations
 
@@ -171,6 +175,63 @@ def run_recovery_comparison(nx, ny, nz, ms_list, direct_reps=2):
     return out
 
 
+def run_batching(n_scenes=64, steps=10, warmup=1, seed=7):
+    """Batched scenes (64 ring scenes, N log-uniform in [8192, 131072], nu
+    log-uniform in [1e-4, 1e-2], 2-6 rings, M = 2% of N): graph-replayed
+    batched steps against the same scenes stepped one after another (each its
+    own ParticleSystem + native substep), with every scene's final x and v
+    compared bitwise between the two."""
+    import numpy as np
+    import torch
+    from batch import SceneBatch, autotune, random_specs, solo_system
+
+    specs = random_specs(n_scenes, seed)
+    n_total = sum(sp.n for sp in specs)
+    t0 = time.time()
+    batch = SceneBatch(specs)
+    log(f"batch of {n_scenes} scenes ({n_total} particles) ready in {time.time() - t0:.1f}s")
+    for _ in range(warmup):
+        batch.step()
+    tune = autotune(batch)
+    log(f"autotune: {tune}")
+    batch.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step_graph()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {steps} graph steps in {t_batch * 1e3:.1f} ms")
+    t_seq, equal = 0.0, 0
+    for s, sp in enumerate(specs):
+        ps, sv = solo_system(sp)
+        for _ in range(warmup):
+            sv.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            sv.step()
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t1
+        k = ps.particle_num[None]
+        same = all(np.array_equal(batch.scene_field(f, s).view(np.uint8),
+                                  getattr(ps, f).to_numpy()[:k].view(np.uint8)) for f in ("x", "v"))
+        equal += int(same)
+        del ps, sv
+    batch.release_graph()
+    out = {"workload": f"{n_scenes} ring scenes, N log-uniform [8192, 131072], M = 2% N",
+           "n_scenes": n_scenes, "n_particles": n_total, "steps": steps,
+           "batched_ms_per_step": round(t_batch * 1e3 / steps, 4),
+           "sequential_ms_per_step": round(t_seq * 1e3 / steps, 4),
+           "batched_particle_steps_per_s": round(n_total * steps / t_batch, 1),
+           "sequential_particle_steps_per_s": round(n_total * steps / t_seq, 1),
+           "speedup": round(t_seq / t_batch, 3),
+           "scenes_bitwise_equal": f"{equal}/{n_scenes}",
+           "autotune": {k: tune[k] for k in ("far_workgroups", "nbr_block", "trials")}}
+    log(f"batching: {json.dumps(out)}")
+    return out
+
+
 def run_cpu_baseline(info, steps=1):
     """The oracle on a bounded sample of the 1M workload: one step."""
     import copy
@@ -201,6 +262,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -214,6 +277,7 @@ def main():
     if not args.no_direct:
         direct = {"comparator_131072": run_recovery_comparison(64, 64, 32, [4096]),
                   "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536])}
+    batching = None if args.no_batching else run_batching(steps=args.batch_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(info)
     value = res["n_fluid"] * args.steps / res["sec"]
     traffic = None
@@ -246,6 +310,7 @@ def main():
                      "note": "FP32 vector-ALU bound (no MFMA); peak = MI355X FP32 157.3 TFLOP/s"},
         "cpu_baseline": cpu,
         "mc_vs_direct": direct,
+        "batching": batching,
     }
     print(json.dumps(line), flush=True)
 

@@ -12,7 +12,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
-SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip"]
+SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip",
+           "csrc/batch.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -31,6 +32,11 @@ C_JACOBI_DONE, C_JACOBI_ITERS = 9, 10
 C_COUNT = 16
 D_KDEN_SUM, D_RESIDUAL, D_CDF_TOTAL, D_COUNT = 0, 1, 2, 8
 FAR_EXACT, FAR_FAST, FAR_KAHAN = 0, 1, 2
+# dimcv_scene_counter / dimcv_scene_dscalar
+SC_DONE, SC_ITERS, SC_NUM_SAMPLES, SC_FLUID, SC_FLUID_START, SC_SLOT_START, SC_SLOTS = 0, 1, 2, 3, 4, 5, 6
+SC_TILE_START, SC_SPLITS, SC_PART_BASE, SC_ITEM_START, SC_ITERS_V, SC_ITERS_P, SC_FLUID_SRC = 7, 8, 9, 10, 11, 12, 13
+SC_COUNT = 16
+SD_KDEN_SUM, SD_RESIDUAL, SD_CDF_TOTAL, SD_COUNT = 0, 1, 2, 4
 
 
 def build(verbose: bool = False) -> str:
@@ -120,9 +126,9 @@ class State(ctypes.Structure):
                 + [("workspace_bytes", ctypes.c_int64)]
                 + [("scene_id", ctypes.c_void_p), ("scene_id_buffer", ctypes.c_void_p),
                    ("scenes", ctypes.c_void_p), ("n_scenes", ctypes.c_int32), ("pad1", ctypes.c_int32),
-                   ("scene_fluid_start", ctypes.c_void_p), ("scene_counters", ctypes.c_void_p),
-                   ("scene_dscal", ctypes.c_void_p), ("seg_blocks", ctypes.c_void_p),
-                   ("n_seg_blocks", ctypes.c_int32), ("pad2", ctypes.c_int32)])
+                   ("scene_counters", ctypes.c_void_p), ("scene_dscal", ctypes.c_void_p),
+                   ("block_scene", ctypes.c_void_p), ("batch_workspace", ctypes.c_void_p),
+                   ("batch_workspace_bytes", ctypes.c_int64)])
 
 
 def lib():
@@ -156,6 +162,15 @@ def lib():
         "dimcv_direct_bs": (I32, [P, P, I32, P, I32, P]),
         "dimcv_ring_velocity": (I32, [I64, P, F64, I64, P, P, P]),
         "dimcv_substep": (I32, [P, P, P, I32, I32, P]),
+        "dimcv_batch_workspace_bytes": (I64, [I32, P, I64]),
+        "dimcv_batch_grid_build": (I32, [P, P, P]),
+        "dimcv_batch_neighbor_build": (I32, [P, P, P]),
+        "dimcv_batch_status": (I32, [P, P, P, P]),
+        "dimcv_batch_substep": (I32, [P, P, P, P, P]),
+        "dimcv_batch_set_launch": (I32, [I32, I32]),
+        "dimcv_batch_graph_create": (I32, [P, P, P, P, P, P]),
+        "dimcv_batch_graph_step": (I32, [P, I32, P, P]),
+        "dimcv_batch_graph_destroy": (None, [P]),
     }
     for name in ["dimcv_compute_densities", "dimcv_compute_DFSPH_factor",
                  "dimcv_compute_density_change", "dimcv_divergence_solver_iteration",

@@ -1,0 +1,628 @@
+// Batched scenes: many independent scenes advanced by one set of launches.
+//
+// Layout.  Slots of scene s are contiguous and in scene order; a scene's cell
+// keys are cell_base + its own flattened cell index, so the stable sort keeps
+// every scene in the order its solo run would have (inactive slots of all
+// scenes follow the last scene, past particle_num, as in a solo run).  The
+// fluid list gives scene s the segment [fluid_start, fluid_start + F) with
+// fluid_start a multiple of 256 (the rest of the last block holds -1).
+// Hence every 256-entry block, every 64-lane wave, every FP64 partial and
+// every canonical tree or scan of a scene covers exactly what it covers in a
+// solo run, and the per-scene results are bitwise those of the solo run.
+//
+// Per scene: Jacobi convergence flags (every Jacobi kernel reads the flag of
+// its block's scene), FP64 reductions of the block partials, the CDF scan
+// and draws, and the far-field work (persistent kernel over a work queue).
+//
+// The step is issued as five segments separated by the host's convergence
+// checks: grid + neighbours + the first divergence iterations; further
+// divergence iterations; the middle of the substep + the first pressure
+// iterations; further pressure iterations; the tail.  None of them
+// synchronises or allocates, so each can be captured into a HIP graph.
+#include "common.h"
+
+namespace dimcv {
+
+int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       const int32_t* done, hipStream_t s);
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
+int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,
+                          double* partials, const int32_t* done, hipStream_t s);
+int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,
+                      int32_t* ucount, hipStream_t s);
+int batch_nbr_launch(const dimcv_state* st, const dimcv_params* p, hipStream_t s);
+void batch_set_nbr_block(int b);
+int batch_cdf_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, BatchWs& bw, const double* u,
+                     int64_t m_total, hipStream_t s);
+int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, BatchWs& bw, hipStream_t s);
+void batch_set_far_workgroups(int w);
+
+static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs* bw, void* base) {
+  char* b = (char*)base;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) -> void* {
+    void* q = b ? (void*)(b + off) : nullptr;
+    off += align256(bytes > 0 ? bytes : 1);
+    return q;
+  };
+  BatchWs w{};
+  const int64_t S = n_scenes > 0 ? n_scenes : 1;
+  const int64_t n = n_max > 0 ? n_max : 1;
+  w.tile_bound = n / DIMCV_SCAN_TILE + S + 1;
+  w.ulist = (int32_t*)take(4 * n);
+  w.order = (int32_t*)take(4 * S);
+  w.item_start = (int32_t*)take(4 * (S + 1));
+  w.misc = (int32_t*)take(4 * BM_COUNT);
+  w.btot = (double*)take(8 * w.tile_bound);
+  w.bpre = (double*)take(8 * w.tile_bound);
+  w.part_cap = part_cap > 0 ? part_cap : 1;
+  w.part = (FarPartial*)take((int64_t)sizeof(FarPartial) * w.part_cap);
+  w.bytes = off;
+  if (bw) *bw = w;
+  return off;
+}
+
+int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw) {
+  if (!st->scenes || st->n_scenes <= 0 || !st->scene_id || !st->scene_id_buffer || !st->scene_counters ||
+      !st->scene_dscal || !st->block_scene || !st->batch_workspace) {
+    set_error("batched call on a state without a scene table");
+    return DIMCV_ERR_ARG;
+  }
+  if (p->periodic || p->has_vortex_domain || p->has_transit || st->n_emitters > 0) {
+    set_error("batched scenes are open boxes without vortex domain, transit area or emitters");
+    return DIMCV_ERR_ARG;
+  }
+  const int64_t need0 = batch_layout(st->n_scenes, p->n_max, 1, nullptr, nullptr);
+  if (st->batch_workspace_bytes < need0) {
+    set_error("batch workspace too small: %lld < %lld bytes", (long long)st->batch_workspace_bytes,
+              (long long)need0);
+    return DIMCV_ERR_WORKSPACE;
+  }
+  // the partials are the last region
+  const int64_t cap = (st->batch_workspace_bytes - (need0 - align256((int64_t)sizeof(FarPartial)))) /
+                      (int64_t)sizeof(FarPartial);
+  batch_layout(st->n_scenes, p->n_max, cap, bw, st->batch_workspace);
+  return DIMCV_OK;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_batch_begin(int32_t* counters, int32_t* misc) {
+  counters[DIMCV_C_KEY_ERROR] = 0;
+  misc[BM_ERROR] = 0;
+}
+
+__device__ __forceinline__ int32_t lower_bound_i32(const int32_t* __restrict__ a, int32_t n, int32_t v) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Per-scene slot ranges (from the cell table), fluid ranges (in the unpadded
+// ascending fluid list), padded fluid segments, CDF tiles, far-field splits,
+// partial bases and the far-field work order.  One workgroup.
+__global__ void __launch_bounds__(256) k_batch_scene_setup(dimcv_state st, dimcv_params p, BatchWs bw) {
+  const int S = st.n_scenes;
+  const int32_t U = bw.misc[BM_UCOUNT];
+  const int32_t* __restrict__ gpn = st.grid_particles_num;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const dimcv_scene& sc = st.scenes[s];
+    int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    const int32_t a = sc.cell_base > 0 ? gpn[sc.cell_base - 1] : 0;
+    const int32_t b = gpn[sc.cell_base + sc.n_cells - 1];
+    const int32_t lo = lower_bound_i32(bw.ulist, U, a);
+    const int32_t hi = lower_bound_i32(bw.ulist, U, b);
+    c[DIMCV_SC_SLOT_START] = a;
+    c[DIMCV_SC_SLOTS] = b - a;
+    c[DIMCV_SC_FLUID_SRC] = lo;
+    c[DIMCV_SC_FLUID] = hi - lo;
+    c[DIMCV_SC_SPLITS] = far_split_count(hi - lo);
+  }
+  __syncthreads();
+  // far-field order: larger per-item source chunks first, then scene index
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    const int64_t cost = (st.scenes[s].m + c[DIMCV_SC_SPLITS] - 1) / c[DIMCV_SC_SPLITS];
+    int rank = 0;
+    for (int t = 0; t < S; ++t) {
+      const int32_t* ct = st.scene_counters + (int64_t)t * DIMCV_SC_COUNT;
+      const int64_t cost_t = (st.scenes[t].m + ct[DIMCV_SC_SPLITS] - 1) / ct[DIMCV_SC_SPLITS];
+      rank += (cost_t > cost || (cost_t == cost && t < s)) ? 1 : 0;
+    }
+    bw.order[rank] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int64_t fs = 0, tiles = 0, part = 0;
+  int err = 0;
+  for (int s = 0; s < S; ++s) {
+    int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    const int64_t F = c[DIMCV_SC_FLUID];
+    c[DIMCV_SC_FLUID_START] = (int32_t)fs;
+    fs += (F + 255) & ~int64_t(255);
+    c[DIMCV_SC_TILE_START] = (int32_t)tiles;
+    tiles += (F + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
+    c[DIMCV_SC_PART_BASE] = 0;
+    if (c[DIMCV_SC_SPLITS] > 1) {
+      const int64_t need = (int64_t)c[DIMCV_SC_SPLITS] * F;
+      if (part + need > bw.part_cap) {
+        c[DIMCV_SC_SPLITS] = 1;
+        err |= 2;
+      } else {
+        c[DIMCV_SC_PART_BASE] = (int32_t)part;
+        part += need;
+      }
+    }
+  }
+  if (fs > p.n_max) err |= 1;
+  int64_t items = 0;
+  for (int r = 0; r < S; ++r) {
+    const int s = bw.order[r];
+    int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    bw.item_start[r] = (int32_t)items;
+    c[DIMCV_SC_ITEM_START] = (int32_t)items;
+    items += ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256 * c[DIMCV_SC_SPLITS];
+  }
+  bw.item_start[S] = (int32_t)items;
+  bw.misc[BM_ITEMS] = (int32_t)items;
+  bw.misc[BM_QUEUE] = 0;
+  bw.misc[BM_TILES] = (int32_t)tiles;
+  bw.misc[BM_ERROR] |= err;
+  st.counters[DIMCV_C_FLUID_LIST] = (int32_t)(fs < p.n_max ? fs : p.n_max);
+}
+
+// Scene of each 256-entry fluid-list block (-1: none).
+__global__ void k_batch_block_scene(dimcv_state st, int64_t nb) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t q = b * 256;
+  const int s = last_le(st.scene_counters + DIMCV_SC_FLUID_START, st.n_scenes, DIMCV_SC_COUNT, q);
+  int r = -1;
+  if (s >= 0) {
+    const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    if (q < (int64_t)c[DIMCV_SC_FLUID_START] + c[DIMCV_SC_FLUID]) r = s;
+  }
+  st.block_scene[b] = r;
+}
+
+// Padded fluid list from the unpadded one.
+__global__ void k_batch_place_fluid(dimcv_state st, const int32_t* __restrict__ ulist, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int s = st.block_scene[q >> 8];
+  int32_t v = -1;
+  if (s >= 0) {
+    const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    const int64_t lf = q - c[DIMCV_SC_FLUID_START];
+    if (lf < c[DIMCV_SC_FLUID]) v = ulist[c[DIMCV_SC_FLUID_SRC] + lf];
+  }
+  st.fluid_list[q] = v;
+}
+
+// Canonical FP64 reduction of each scene's block partials (one workgroup per
+// scene; the levels of k_reduce_levels) into scene_dscal[s][which].
+__global__ void __launch_bounds__(256) k_batch_reduce(dimcv_state st, double* __restrict__ partials, int which,
+                                                      const int32_t* __restrict__ done) {
+  __shared__ double lds[256];
+  const int s = blockIdx.x;
+  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  if (done && done[(int64_t)s * DIMCV_SC_COUNT]) return;  // uniform per block
+  double* a = partials + c[DIMCV_SC_FLUID_START] / 256;
+  const int64_t n = ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256;
+  int64_t m = n;
+  while (m > 1) {
+    const int64_t groups = (m + 255) / 256;
+    for (int64_t g = 0; g < groups; ++g) {
+      const int64_t i = g * 256 + threadIdx.x;
+      const double v = i < m ? a[i] : 0.0;
+      const double r = block_tree_sum256(v, lds);
+      if (threadIdx.x == 0) a[g] = r;
+      __syncthreads();
+    }
+    m = groups;
+  }
+  if (threadIdx.x == 0) st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + which] = n > 0 ? a[0] : 0.0;
+}
+
+__global__ void k_batch_jacobi_reset(dimcv_state st) {
+  for (int s = threadIdx.x; s < st.n_scenes; s += blockDim.x) {
+    st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_DONE] = 0;
+    st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_ITERS] = 0;
+  }
+  if (threadIdx.x == 0) {
+    st.counters[DIMCV_C_JACOBI_DONE] = 0;
+    st.counters[DIMCV_C_JACOBI_ITERS] = 0;
+  }
+}
+
+// Per-scene convergence test of dimcv_sph.py:384-391 / 606-613 (average =
+// FP64 residual / the scene's fluid count); counters[JACOBI_DONE] = every
+// scene converged.  A scene without fluid counts as converged.
+__global__ void __launch_bounds__(256) k_batch_jacobi_check(dimcv_state st, double eta, int32_t max_iter,
+                                                            int which) {
+  __shared__ int all;
+  if (threadIdx.x == 0) all = 1;
+  __syncthreads();
+  for (int s = threadIdx.x; s < st.n_scenes; s += blockDim.x) {
+    int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    if (!c[DIMCV_SC_DONE]) {
+      const int32_t it = ++c[DIMCV_SC_ITERS];
+      const int32_t F = c[DIMCV_SC_FLUID];
+      const double avg = F > 0 ? st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + DIMCV_SD_RESIDUAL] / (double)F : 0.0;
+      if (F <= 0 || avg <= eta || it >= max_iter) c[DIMCV_SC_DONE] = 1;
+      c[which ? DIMCV_SC_ITERS_P : DIMCV_SC_ITERS_V] = it;
+    }
+    if (!c[DIMCV_SC_DONE]) atomicAnd(&all, 0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) st.counters[DIMCV_C_JACOBI_DONE] = all;
+}
+
+// ---------------------------------------------------------------------------
+// step segments
+struct BatchCtx {
+  const dimcv_state* pre;  // state before the grid build's swap
+  const dimcv_state* st;   // state after it
+  const dimcv_params* p;
+  const dimcv_step_opts* o;
+  Workspace ws;
+  BatchWs bw;
+};
+
+#define CHECK(call)      \
+  do {                   \
+    int rc_ = (call);    \
+    if (rc_) return rc_; \
+  } while (0)
+
+static int seg_grid(BatchCtx& c, hipStream_t s) {
+  const dimcv_state* st = c.pre;
+  const int64_t n = c.p->n_max;
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, s, st->counters, c.bw.misc);
+  CHECK(batch_grid_launch(st, c.p, c.ws, c.bw.ulist, c.bw.misc + BM_UCOUNT, s));
+  hipLaunchKernelGGL(k_batch_scene_setup, dim3(1), dim3(256), 0, s, *st, *c.p, c.bw);
+  const int64_t nb = n / 256 + 1;
+  hipLaunchKernelGGL(k_batch_block_scene, dim3(nblocks(nb)), dim3(256), 0, s, *st, nb);
+  hipLaunchKernelGGL(k_batch_place_fluid, dim3(nblocks(n)), dim3(256), 0, s, *st, (const int32_t*)c.bw.ulist, n);
+  DIMCV_LAUNCHED("batched grid build");
+  return DIMCV_OK;
+}
+
+static int seg_div_begin(BatchCtx& c, hipStream_t s) {
+  const dimcv_state* st = c.st;
+  const int32_t nb = (int32_t)c.p->n_max;
+  CHECK(launch_fused(0, st, c.p, nb, s));
+  CHECK(dimcv_compute_density_change(st, c.p, nb, s));
+  CHECK(dimcv_multiply_time_step(st, c.p, st->dfsph_factor, c.o->inv_dt, nb, s));
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
+  DIMCV_LAUNCHED("k_batch_jacobi_reset");
+  return DIMCV_OK;
+}
+
+// `chunk` Jacobi iterations of the divergence (which = 0) or pressure solve.
+static int seg_iters(BatchCtx& c, int which, hipStream_t s) {
+  const dimcv_state* st = c.st;
+  const int32_t nb = (int32_t)c.p->n_max;
+  const int32_t* done = st->scene_counters + DIMCV_SC_DONE;
+  const int chunk = c.o->chunk > 0 ? c.o->chunk : 1;
+  for (int k = 0; k < chunk; ++k) {
+    CHECK(launch_jacobi_pass(which == 0 ? 0 : 2, st, c.p, nb, done, s));
+    CHECK(launch_jacobi_pass(which == 0 ? 1 : 3, st, c.p, nb, done, s));
+    CHECK(launch_block_partials(0, st, c.p, which == 0 ? 0.0f : c.o->density0, nb, c.ws.partials, done, s));
+    hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
+                       (int)DIMCV_SD_RESIDUAL, done);
+    hipLaunchKernelGGL(k_batch_jacobi_check, dim3(1), dim3(256), 0, s, *st, which == 0 ? c.o->eta_v : c.o->eta_p,
+                       which == 0 ? c.o->max_iter_v : c.o->max_iter_p, which);
+    DIMCV_LAUNCHED("batched Jacobi iteration");
+  }
+  return DIMCV_OK;
+}
+
+static int seg_mid(BatchCtx& c, hipStream_t s) {
+  const dimcv_state* st = c.st;
+  const dimcv_params* p = c.p;
+  const int32_t nb = (int32_t)p->n_max;
+  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, c.o->dt, nb, s));
+  CHECK(dimcv_compute_non_pressure_forces(st, p, nb, nb, s));
+  CHECK(launch_fused(1, st, p, nb, s));
+  CHECK(launch_fused(2, st, p, nb, s));
+  CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, nb, s));
+  CHECK(launch_block_partials(1, st, p, 0.0f, nb, c.ws.partials, nullptr, s));
+  hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
+                     (int)DIMCV_SD_KDEN_SUM, (const int32_t*)nullptr);
+  DIMCV_LAUNCHED("batched kden");
+  CHECK(batch_cdf_launch(st, p, c.ws, c.bw, c.o->u_cdf, c.o->m, s));
+  CHECK(batch_vort2vel_launch(st, p, c.ws, c.bw, s));
+  CHECK(dimcv_update_particle_life_time(st, p, nb, s));
+  CHECK(dimcv_predict_velocity(st, p, nb, s));
+  CHECK(dimcv_compute_density_adv(st, p, nb, s));
+  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, c.o->inv_dt2, nb, s));
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
+  DIMCV_LAUNCHED("k_batch_jacobi_reset");
+  return DIMCV_OK;
+}
+
+static int seg_post(BatchCtx& c, hipStream_t s) {
+  const int32_t nb = (int32_t)c.p->n_max;
+  CHECK(dimcv_copy_x_temp(c.st, c.p, nb, s));
+  CHECK(dimcv_advect(c.st, c.p, nb, s));
+  return DIMCV_OK;
+}
+
+static int make_ctx(BatchCtx& c, const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                    const dimcv_step_opts* o) {
+  c.pre = pre;
+  c.st = st;
+  c.p = p;
+  c.o = o;
+  CHECK(bind_workspace(st, p, &c.ws, o ? (int64_t)o->m : 0));
+  CHECK(bind_batch(st, p, &c.bw));
+  if (o) {
+    if (o->m <= 0 || !o->u_cdf) {
+      set_error("batched substep: fixed-M uniforms required (m > 0, u_cdf)");
+      return DIMCV_ERR_ARG;
+    }
+  }
+  return DIMCV_OK;
+}
+
+static int read_done(const dimcv_state* st, int32_t* host, hipStream_t s) {
+  DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  return DIMCV_OK;
+}
+
+static int read_iters(const dimcv_state* st, int32_t* out_iters, int32_t* tmp, hipStream_t s) {
+  const int64_t S = st->n_scenes;
+  DIMCV_HIP(hipMemcpyAsync(tmp, st->scene_counters, sizeof(int32_t) * DIMCV_SC_COUNT * S, hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  for (int64_t k = 0; k < S; ++k) {
+    out_iters[2 * k] = tmp[k * DIMCV_SC_COUNT + DIMCV_SC_ITERS_V];
+    out_iters[2 * k + 1] = tmp[k * DIMCV_SC_COUNT + DIMCV_SC_ITERS_P];
+  }
+  return DIMCV_OK;
+}
+
+static int check_counters(const int32_t* hc, const int32_t* misc_err, const dimcv_params* p) {
+  if (hc[DIMCV_C_KEY_ERROR] != 0) {
+    set_error("%d active particles lie outside their scene's cell grid", hc[DIMCV_C_KEY_ERROR]);
+    return DIMCV_ERR_KEY;
+  }
+  if (hc[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
+    set_error("neighbour list width %d exceeds nbr_kmax %d", hc[DIMCV_C_NBR_MAX], p->nbr_kmax);
+    return DIMCV_ERR_NBR;
+  }
+  if (misc_err && *misc_err) {
+    set_error("batch capacity exceeded (flags %d: 1 fluid list, 2 far-field partials)", *misc_err);
+    return DIMCV_ERR_RANGE;
+  }
+  return DIMCV_OK;
+}
+
+}  // namespace dimcv
+
+using namespace dimcv;
+
+// ---------------------------------------------------------------------------
+// step graphs
+struct dimcv_batch_graph {
+  hipGraph_t graph[2][5];
+  hipGraphExec_t exec[2][5];
+  dimcv_state st[2];
+  dimcv_params p;
+  dimcv_step_opts o;
+  int32_t* pinned;  // counters, misc error, scene counters
+  int32_t n_scenes;
+};
+
+extern "C" {
+
+int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_particles, int64_t n_max) {
+  if (n_scenes <= 0 || !scene_particles) return -1;
+  int64_t cap = 0;
+  for (int32_t s = 0; s < n_scenes; ++s) {
+    const int64_t n = scene_particles[s] > 0 ? scene_particles[s] : 0;
+    cap += 16 * n < (int64_t)(1 << 21) ? 16 * n : (int64_t)(1 << 21);
+  }
+  return batch_layout(n_scenes, n_max, cap, nullptr, nullptr);
+}
+
+int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block) {
+  if (far_workgroups < 0 || far_workgroups > 65535 ||
+      !(nbr_block == 0 || nbr_block == 64 || nbr_block == 128 || nbr_block == 256 || nbr_block == 512 ||
+        nbr_block == 1024)) {
+    set_error("dimcv_batch_set_launch: workgroups %d or block %d out of range", far_workgroups, nbr_block);
+    return DIMCV_ERR_ARG;
+  }
+  batch_set_far_workgroups(far_workgroups);
+  batch_set_nbr_block(nbr_block > 0 ? nbr_block : 256);
+  return DIMCV_OK;
+}
+
+int dimcv_batch_grid_build(const dimcv_state* st, const dimcv_params* p, void* stream) {
+  if (!st || !p) {
+    set_error("dimcv_batch_grid_build: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  BatchCtx c;
+  CHECK(make_ctx(c, st, st, p, nullptr));
+  return seg_grid(c, (hipStream_t)stream);
+}
+
+int dimcv_batch_neighbor_build(const dimcv_state* st, const dimcv_params* p, void* stream) {
+  if (!st || !p) {
+    set_error("dimcv_batch_neighbor_build: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  BatchCtx c;
+  CHECK(make_ctx(c, st, st, p, nullptr));
+  return batch_nbr_launch(st, p, (hipStream_t)stream);
+}
+
+int dimcv_batch_status(const dimcv_state* st, const dimcv_params* p, int32_t* host_counters, void* stream) {
+  if (!st || !p) {
+    set_error("dimcv_batch_status: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  BatchCtx c;
+  CHECK(make_ctx(c, st, st, p, nullptr));
+  hipStream_t s = (hipStream_t)stream;
+  int32_t hc[DIMCV_C_COUNT];
+  int32_t err = 0;
+  DIMCV_HIP(hipMemcpyAsync(hc, st->counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipMemcpyAsync(&err, c.bw.misc + BM_ERROR, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  if (host_counters)
+    for (int k = 0; k < DIMCV_C_COUNT; ++k) host_counters[k] = hc[k];
+  return check_counters(hc, &err, p);
+}
+
+int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t* out_iters,
+                        void* stream) {
+  if (!st || !p || !o) {
+    set_error("dimcv_batch_substep: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  BatchCtx c;
+  CHECK(make_ctx(c, st, st, p, o));
+  int32_t done = 0;
+  CHECK(seg_div_begin(c, s));
+  do {
+    CHECK(seg_iters(c, 0, s));
+    CHECK(read_done(st, &done, s));
+  } while (!done);
+  CHECK(seg_mid(c, s));
+  do {
+    CHECK(seg_iters(c, 1, s));
+    CHECK(read_done(st, &done, s));
+  } while (!done);
+  CHECK(seg_post(c, s));
+  if (out_iters) {
+    int32_t* tmp = new int32_t[(size_t)st->n_scenes * DIMCV_SC_COUNT];
+    int rc = read_iters(st, out_iters, tmp, s);
+    delete[] tmp;
+    return rc;
+  }
+  return DIMCV_OK;
+}
+
+void dimcv_batch_graph_destroy(dimcv_batch_graph* g) {
+  if (!g) return;
+  for (int a = 0; a < 2; ++a)
+    for (int k = 0; k < 5; ++k) {
+      if (g->exec[a][k]) (void)hipGraphExecDestroy(g->exec[a][k]);
+      if (g->graph[a][k]) (void)hipGraphDestroy(g->graph[a][k]);
+    }
+  if (g->pinned) (void)hipHostFree(g->pinned);
+  delete g;
+}
+
+static int capture(dimcv_batch_graph* g, int par, int seg, hipStream_t s) {
+  const dimcv_state* pre = &g->st[par];
+  const dimcv_state* cur = &g->st[par ^ 1];
+  BatchCtx c;
+  CHECK(make_ctx(c, pre, cur, &g->p, &g->o));
+  DIMCV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  int rc = DIMCV_OK;
+  switch (seg) {
+    case 0:
+      rc = seg_grid(c, s);
+      if (!rc) rc = batch_nbr_launch(cur, &g->p, s);
+      if (!rc) rc = seg_div_begin(c, s);
+      if (!rc) rc = seg_iters(c, 0, s);
+      break;
+    case 1: rc = seg_iters(c, 0, s); break;
+    case 2:
+      rc = seg_mid(c, s);
+      if (!rc) rc = seg_iters(c, 1, s);
+      break;
+    case 3: rc = seg_iters(c, 1, s); break;
+    default: rc = seg_post(c, s); break;
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  g->graph[par][seg] = graph;
+  DIMCV_HIP(hipGraphInstantiate(&g->exec[par][seg], graph, nullptr, nullptr, 0));
+  return DIMCV_OK;
+}
+
+int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_odd, const dimcv_params* p,
+                             const dimcv_step_opts* opts, void* stream, dimcv_batch_graph** out) {
+  if (!st_even || !st_odd || !p || !opts || !out) {
+    set_error("dimcv_batch_graph_create: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (!stream) {
+    set_error("dimcv_batch_graph_create: capture needs a non-null stream");
+    return DIMCV_ERR_ARG;
+  }
+  *out = nullptr;
+  dimcv_batch_graph* g = new dimcv_batch_graph();
+  g->st[0] = *st_even;
+  g->st[1] = *st_odd;
+  g->p = *p;
+  g->o = *opts;
+  g->n_scenes = st_even->n_scenes;
+  hipError_t e = hipHostMalloc((void**)&g->pinned, sizeof(int32_t) * (DIMCV_C_COUNT + 1 + (size_t)DIMCV_SC_COUNT *
+                                                                      (size_t)(g->n_scenes > 0 ? g->n_scenes : 1)),
+                               hipHostMallocDefault);
+  if (e != hipSuccess) {
+    g->pinned = nullptr;
+    dimcv_batch_graph_destroy(g);
+    return hip_fail(e, "hipHostMalloc");
+  }
+  for (int par = 0; par < 2; ++par)
+    for (int seg = 0; seg < 5; ++seg) {
+      int rc = capture(g, par, seg, (hipStream_t)stream);
+      if (rc) {
+        dimcv_batch_graph_destroy(g);
+        return rc;
+      }
+    }
+  *out = g;
+  return DIMCV_OK;
+}
+
+int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_iters, void* stream) {
+  if (!g || (parity != 0 && parity != 1)) {
+    set_error("dimcv_batch_graph_step: bad arguments");
+    return DIMCV_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const dimcv_state* cur = &g->st[parity ^ 1];
+  BatchWs bw;
+  CHECK(bind_batch(cur, &g->p, &bw));
+  int32_t* hc = g->pinned;
+  int32_t* herr = g->pinned + DIMCV_C_COUNT;
+  DIMCV_HIP(hipGraphLaunch(g->exec[parity][0], s));
+  DIMCV_HIP(hipMemcpyAsync(hc, cur->counters, sizeof(int32_t) * DIMCV_C_COUNT, hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipMemcpyAsync(herr, bw.misc + BM_ERROR, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  CHECK(check_counters(hc, herr, &g->p));
+  while (!hc[DIMCV_C_JACOBI_DONE]) {
+    DIMCV_HIP(hipGraphLaunch(g->exec[parity][1], s));
+    CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
+  }
+  DIMCV_HIP(hipGraphLaunch(g->exec[parity][2], s));
+  CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
+  while (!hc[DIMCV_C_JACOBI_DONE]) {
+    DIMCV_HIP(hipGraphLaunch(g->exec[parity][3], s));
+    CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
+  }
+  DIMCV_HIP(hipGraphLaunch(g->exec[parity][4], s));
+  if (out_iters) return read_iters(cur, out_iters, g->pinned + DIMCV_C_COUNT + 1, s);
+  return DIMCV_OK;
+}
+
+}  // extern "C"

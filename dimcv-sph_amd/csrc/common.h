@@ -221,6 +221,46 @@ __device__ __forceinline__ v3 rel(const dimcv_params& p, v3 xi, v3 xj) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched scenes.  A batch's fluid list holds one segment per scene starting
+// at a multiple of 256; unused entries hold -1 and every pass skips them.
+// block_scene[b] names the scene of 256-entry block b (-1: none).
+__device__ __forceinline__ int block_scene_of(const dimcv_state& st) {
+  return st.block_scene ? st.block_scene[blockIdx.x] : 0;
+}
+
+// Jacobi gate: the solve of this block's scene (batched: done points at
+// scene_counters + DIMCV_SC_DONE) or the single solve has converged.
+// Launches using it are 256-wide over fluid-list entries.
+__device__ __forceinline__ bool jacobi_gated(const dimcv_state& st, const int32_t* done) {
+  if (!done) return false;
+  if (st.block_scene) {
+    const int s = st.block_scene[blockIdx.x];
+    return s < 0 || done[(int64_t)s * DIMCV_SC_COUNT] != 0;
+  }
+  return *done != 0;
+}
+
+// kden_sum of the scene of this block.
+__device__ __forceinline__ double kden_sum_of(const dimcv_state& st) {
+  if (st.block_scene)
+    return st.scene_dscal[(int64_t)st.block_scene[blockIdx.x] * DIMCV_SD_COUNT + DIMCV_SD_KDEN_SUM];
+  return st.dscal[DIMCV_D_KDEN_SUM];
+}
+
+// Far-field source splits for a target set: about 16 waves per SIMD (256 CUs
+// x 4 SIMDs) when the targets alone would leave the GPU idle, at most 16.
+// A pure function of the target count, so that a scene's summation structure
+// is the same alone and in a batch; splits * targets < 2^21 whenever
+// splits > 1.
+__host__ __device__ __forceinline__ int far_split_count(int64_t targets) {
+  const int64_t want_waves = 256LL * 4 * 16;
+  const int64_t waves = (targets + 63) / 64;
+  int64_t s = waves > 0 ? (want_waves + waves - 1) / waves : 1;
+  if (s > 16) s = 16;
+  return s < 1 ? 1 : (int)s;
+}
+
+// ---------------------------------------------------------------------------
 // canonical FP64 block tree (256 values, a[t] += a[t + s], s = 128..1)
 __device__ inline double block_tree_sum256(double v, double* lds) {
   const int t = threadIdx.x;
@@ -235,6 +275,51 @@ __device__ inline double block_tree_sum256(double v, double* lds) {
   __syncthreads();
   return r;
 }
+
+// Far-field partial of a split source range (A, Kahan compensation,
+// rejected samples).
+struct FarPartial {
+  float a[3];
+  float c[3];
+  int rej;
+  int pad;
+};
+
+// Batch workspace (dimcv_batch_workspace_bytes): unpadded fluid list, far
+// field work order, misc counters, far-field partials (last region).
+enum BatchMisc {
+  BM_UCOUNT = 0,       // unpadded fluid-list length
+  BM_ITEMS = 1,        // far-field work items
+  BM_QUEUE = 2,        // far-field work queue head
+  BM_TILES = 3,        // CDF scan tiles
+  BM_ERROR = 4,        // capacity error flags (1: fluid list, 2: far partials)
+  BM_COUNT = 16
+};
+struct BatchWs {
+  int32_t* ulist;       // [n_max]
+  int32_t* order;       // [n_scenes]: far-field rank -> scene
+  int32_t* item_start;  // [n_scenes + 1]: first work item of each rank
+  int32_t* misc;        // [BM_COUNT]
+  double* btot;         // [tile_bound]: CDF tile totals
+  double* bpre;         // [tile_bound]: their per-scene scans
+  int64_t tile_bound;   // n_max / 4096 + n_scenes + 1
+  FarPartial* part;     // [part_cap]
+  int64_t part_cap;
+  int64_t bytes;
+};
+
+// Last index r in [0, n) with a[r * stride] <= v for non-decreasing a (-1: none).
+__device__ __forceinline__ int last_le(const int32_t* __restrict__ a, int n, int stride, int64_t v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[(int64_t)mid * stride] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs* bw, void* base);
+int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw);
 
 // host helpers implemented in util.hip
 int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream,

@@ -261,6 +261,7 @@ __global__ void k_gather(dimcv_state st, int64_t n_in, const uint32_t* __restric
   st.is_active_buffer[s] = 1;
   st.is_sample_buffer[s] = st.is_sample[i];
   st.life_time_buffer[s] = st.life_time[i];
+  if (st.scene_id) st.scene_id_buffer[s] = st.scene_id[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +420,161 @@ __global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound,
   st.m_V[i] = 1.0f / delta * 3.0f;
 }
 
+// Stable radix sort of (key, slot) over the key bits in use (sentinel =
+// n_cells), the cell table and the gather into the *_buffer fields.  Keys
+// and slots are in ws.keys[0] / ws.vals[0].
+int sort_cells_gather(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int64_t n_in, hipStream_t s) {
+  int nbits = 32 - __builtin_clz((unsigned)p->n_cells);
+  int passes = (nbits + 7) / 8;
+  int cur = 0;
+  if (n_in > 0) {
+    int nblk = (int)((n_in + DIMCV_SORT_TILE - 1) / DIMCV_SORT_TILE);
+    for (int ps = 0; ps < passes; ++ps) {
+      int shift = 8 * ps;
+      hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift,
+                         ws.radix_hist, nblk);
+      int rc = exclusive_scan_u32(ws.radix_hist, ws.radix_hist, (int64_t)256 * nblk, ws.scan_tmp, nullptr, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
+                         ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
+      DIMCV_LAUNCHED("k_radix_scatter");
+      cur ^= 1;
+    }
+  }
+  hipLaunchKernelGGL(k_cell_end, dim3(nblocks(p->n_cells)), dim3(256), 0, s, ws.keys[cur], n_in,
+                     p->n_cells, st->grid_particles_num, st->grid_particles_num_temp);
+  if (n_in > 0)
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(n_in)), dim3(256), 0, s, *st, n_in, ws.keys[cur],
+                       ws.vals[cur]);
+  DIMCV_LAUNCHED("k_gather");
+  return DIMCV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched scenes: keys with per-scene boxes, grids and cell bases; neighbour
+// lists over each scene's own cells.
+
+// update_activity + update_grid_id of the slot's scene (open box, fluid
+// only).  Inactive slots get the sentinel key (past every scene's cells).
+__global__ void k_batch_keys(dimcv_state st, dimcv_params p, int64_t n_in, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_in) return;
+  const uint32_t sentinel = (uint32_t)p.n_cells;
+  vals[i] = (uint32_t)i;
+  if (i >= st.counters[DIMCV_C_PARTICLE_NUM]) {
+    keys[i] = sentinel;
+    return;
+  }
+  const dimcv_scene& sc = st.scenes[st.scene_id[i]];
+  const int32_t mat = st.material[i];
+  const v3 xi = ld3(st.x, i);
+  uint8_t active = st.is_active[i];
+  if (mat == 0) {
+    active = 1;
+  } else if (mat == 1) {
+    const float q[3] = {xi.x, xi.y, xi.z};
+    for (int j = 0; j < 3; ++j) {
+      if (q[j] < sc.domain_start[j] || q[j] > sc.domain_end[j]) {
+        active = 0;
+        atomicSub(&st.counters[DIMCV_C_FLUID_NUM], 1);
+        break;
+      }
+    }
+  }
+  st.is_active[i] = active;
+  if (!active) {
+    keys[i] = sentinel;
+    return;
+  }
+  int c[3];
+  c[0] = (int)(xi.x / p.grid_size);
+  c[1] = (int)(xi.y / p.grid_size);
+  c[2] = (int)(xi.z / p.grid_size);
+  for (int d = 0; d < 3; ++d)
+    if (c[d] < 0 || c[d] >= sc.grid_num[d]) {
+      atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
+      keys[i] = sentinel;
+      return;
+    }
+  keys[i] = (uint32_t)(sc.cell_base + (c[0] * sc.grid_num[1] * sc.grid_num[2] + c[1] * sc.grid_num[2] + c[2]));
+}
+
+// Neighbour lists of fluid-list entry q over the 9 z-runs of its scene's
+// grid; the scene's own cell 0 stays invisible (its run starts at the end of
+// that cell), as in a solo build.
+__global__ void k_batch_nbr_build(dimcv_state st, dimcv_params p, int64_t bound) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  if (i < 0) {
+    st.nbr_count[f] = 0;
+    return;
+  }
+  const dimcv_scene& sc = st.scenes[st.scene_id[i]];
+  const v3 xi = ld3(st.x, i);
+  int cc[3];
+  cc[0] = (int)(xi.x / p.grid_size);
+  cc[1] = (int)(xi.y / p.grid_size);
+  cc[2] = (int)(xi.z / p.grid_size);
+  const int nx = sc.grid_num[0], ny = sc.grid_num[1], nz = sc.grid_num[2];
+  const int32_t* __restrict__ gpn = st.grid_particles_num + sc.cell_base;
+  const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, nz - 1);
+  const float lim = p.nbr_r2_limit;
+  const int kmax = p.nbr_kmax;
+  const int64_t stride = p.n_max;
+  int cnt = 0;
+  for (int o0 = -1; o0 <= 1; ++o0) {
+    const int g0 = cc[0] + o0;
+    if (g0 < 0 || g0 >= nx) continue;
+    for (int o1 = -1; o1 <= 1; ++o1) {
+      const int g1 = cc[1] + o1;
+      if (g1 < 0 || g1 >= ny) continue;
+      const int fl = (g0 * ny + g1) * nz;
+      const int lo = fl + zlo - 1;
+      const int beg = lo >= 0 ? gpn[lo] : gpn[0];
+      const int end = gpn[fl + zhi];
+      for (int j = beg; j < end; ++j) {
+        const v3 r = vsub(xi, ld3(st.x, j));
+        const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+        if (r2 < lim && j != i) {
+          if (cnt < kmax)
+            st.nbr[(int64_t)cnt * stride + f] = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
+          ++cnt;
+        }
+      }
+    }
+  }
+  st.nbr_count[f] = cnt;
+  atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+}
+
+static int g_nbr_block = 256;
+
+void batch_set_nbr_block(int b) { g_nbr_block = b; }
+
+// Activity + keys, sort, cell table, gather (state before the swap) and the
+// unpadded fluid list (ascending slots) into ulist / *ucount.
+int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,
+                      int32_t* ucount, hipStream_t s) {
+  const int64_t n_in = p->n_max;
+  hipLaunchKernelGGL(k_batch_keys, dim3(nblocks(n_in)), dim3(256), 0, s, *st, *p, n_in, ws.keys[0], ws.vals[0]);
+  hipLaunchKernelGGL(k_set_particle_num, dim3(1), dim3(1), 0, s, st->counters, (int32_t)n_in);
+  DIMCV_LAUNCHED("k_batch_keys");
+  int rc = sort_cells_gather(st, p, ws, n_in, s);
+  if (rc) return rc;
+  PredFluid pf{st->material_buffer, st->counters + DIMCV_C_PARTICLE_NUM};
+  return compact(pf, n_in, ulist, ucount, ws, s);
+}
+
+// Neighbour lists of the swapped state (padded fluid list in place).
+int batch_nbr_launch(const dimcv_state* st, const dimcv_params* p, hipStream_t s) {
+  const int b = g_nbr_block;
+  hipLaunchKernelGGL(k_batch_nbr_build, dim3(nblocks(p->n_max, b)), dim3(b), 0, s, *st, *p, (int64_t)p->n_max);
+  DIMCV_LAUNCHED("k_batch_nbr_build");
+  return DIMCV_OK;
+}
+
 }  // namespace dimcv
 
 using namespace dimcv;
@@ -483,30 +639,8 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
   hipLaunchKernelGGL(k_set_particle_num, dim3(1), dim3(1), 0, s, st->counters, (int32_t)n_in);
   DIMCV_LAUNCHED("k_activity_keys");
 
-  // stable radix sort over the key bits in use (sentinel = n_cells)
-  int nbits = 32 - __builtin_clz((unsigned)p->n_cells);
-  int passes = (nbits + 7) / 8;
-  int cur = 0;
-  if (n_in > 0) {
-    int nblk = (int)((n_in + DIMCV_SORT_TILE - 1) / DIMCV_SORT_TILE);
-    for (int ps = 0; ps < passes; ++ps) {
-      int shift = 8 * ps;
-      hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift,
-                         ws.radix_hist, nblk);
-      rc = exclusive_scan_u32(ws.radix_hist, ws.radix_hist, (int64_t)256 * nblk, ws.scan_tmp, nullptr, s);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
-                         ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
-      DIMCV_LAUNCHED("k_radix_scatter");
-      cur ^= 1;
-    }
-  }
-  hipLaunchKernelGGL(k_cell_end, dim3(nblocks(p->n_cells)), dim3(256), 0, s, ws.keys[cur], n_in,
-                     p->n_cells, st->grid_particles_num, st->grid_particles_num_temp);
-  if (n_in > 0)
-    hipLaunchKernelGGL(k_gather, dim3(nblocks(n_in)), dim3(256), 0, s, *st, n_in, ws.keys[cur],
-                       ws.vals[cur]);
-  DIMCV_LAUNCHED("k_gather");
+  rc = sort_cells_gather(st, p, ws, n_in, s);
+  if (rc) return rc;
   PredFluid pf{st->material_buffer, st->counters + DIMCV_C_PARTICLE_NUM};
   rc = compact(pf, n_in, st->fluid_list, st->counters + DIMCV_C_FLUID_LIST, ws, s);
   if (rc) return rc;

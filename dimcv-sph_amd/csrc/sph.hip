@@ -13,6 +13,7 @@ namespace dimcv {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;        \
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;          \
   const int64_t i = st.fluid_list[f];                                      \
+  if (i < 0) return;                                                       \
   const int nn = min(st.nbr_count[f], p.nbr_kmax);                         \
   const uint32_t* __restrict__ nl = st.nbr + f;                            \
   const int64_t ns = p.n_max
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_para
 // dimcv_sph.py:337-374
 __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done) {
-  if (done && *done) return;
+  if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float dadv = 0.0f;
@@ -141,6 +142,7 @@ __global__ void k_multiply(dimcv_state st, float* field, float scale, int32_t bo
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   field[i] *= scale;
 }
 
@@ -148,15 +150,17 @@ __global__ void k_multiply(dimcv_state st, float* field, float scale, int32_t bo
 __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p, float offset, int32_t bound,
                                                   double* __restrict__ partials, const int32_t* __restrict__ done) {
   __shared__ double lds[256];
-  if (done && *done) return;  // uniform per block
+  if (jacobi_gated(st, done)) return;  // uniform per block
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
-    float t;
-    if (!p.has_transit) t = st.density_adv[i] - offset;
-    else t = transit_weight(p, st.x[3 * i]) * (st.density_adv[i] - offset);
-    v = (double)t;
+    if (i >= 0) {
+      float t;
+      if (!p.has_transit) t = st.density_adv[i] - offset;
+      else t = transit_weight(p, st.x[3 * i]) * (st.density_adv[i] - offset);
+      v = (double)t;
+    }
   }
   double s = block_tree_sum256(v, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
 // dimcv_sph.py:421-468
 __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done) {
-  if (done && *done) return;
+  if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -199,6 +203,7 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
 // boundary viscosity 0 contribute exact zeros and are not evaluated).
 __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound) {
   FLUID_ENTRY();
+  const float visc_coef = st.block_scene ? st.scenes[block_scene_of(st)].visc_coef : p.visc_coef;
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
   FOR_NBR(k, e, j) {
@@ -206,7 +211,7 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
       v3 r = rel(p, xi, ld3(st.x, j));
       float v_xy = vdot(vsub(vi, ld3(st.v, j)), r);
       float rn = vnorm(r);
-      v3 fv = smul(p.visc_coef * st.m_V[j] * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
+      v3 fv = smul(visc_coef * st.m_V[j] * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
       dv = vadd(dv, fv);
     }
   }
@@ -246,7 +251,7 @@ __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
 // dimcv_sph.py:470-493
 __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done) {
-  if (done && *done) return;
+  if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float delta = 0.0f;
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
 // dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour)
 __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done) {
-  if (done && *done) return;
+  if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -349,6 +354,7 @@ __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   st3(st.v_star, i, vadd(ld3(st.v, i), vmuls(ld3(st.acceleration, i), p.dt)));
 }
 
@@ -446,7 +452,7 @@ __global__ void k_kvn_sample(dimcv_state st, dimcv_params p, const float* __rest
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
-  if (!in_vortex_domain(p, ld3(st.x, i))) return;
+  if (i < 0 || !in_vortex_domain(p, ld3(st.x, i))) return;
   const float* G = st.grad_v + 9 * i;
   float g[9];
   for (int q = 0; q < 9; ++q) g[q] = G[q];
@@ -494,8 +500,8 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
   double contrib = 0.0;
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
-    v3 xi = ld3(st.x, i);
-    if (in_vortex_domain(p, xi)) {
+    if (i >= 0 && in_vortex_domain(p, ld3(st.x, i))) {
+      const v3 xi = ld3(st.x, i);
       const int nn = min(st.nbr_count[f], p.nbr_kmax);
       const uint32_t* __restrict__ nl = st.nbr + f;
       const int64_t ns = p.n_max;
@@ -519,7 +525,7 @@ __global__ void k_life_time(dimcv_state st, dimcv_params p, int32_t bound) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
-  if (!in_vortex_domain(p, ld3(st.x, i))) return;
+  if (i < 0 || !in_vortex_domain(p, ld3(st.x, i))) return;
   st.life_time[i] += p.dt;
 }
 
@@ -569,6 +575,21 @@ int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, 
     DIMCV_LAUNCHED("k_residual");
   }
   return reduce_partials_f64(partials, nb, st->dscal + DIMCV_D_RESIDUAL, s, done);
+}
+
+// Per-block FP64 partials of the residual (which = 0, with offset) or of
+// kden (which = 1), 256 fluid-list entries per block; the batched path
+// reduces them per scene.
+int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,
+                          double* partials, const int32_t* done, hipStream_t s) {
+  const unsigned nb = nblocks(bound);
+  if (nb == 0) return DIMCV_OK;
+  if (which == 0)
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, bound, partials, done);
+  else
+    hipLaunchKernelGGL(k_kernel_density, dim3(nb), dim3(256), 0, s, *st, *p, bound, partials);
+  DIMCV_LAUNCHED("block partials");
+  return DIMCV_OK;
 }
 
 }  // namespace dimcv

@@ -51,8 +51,9 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.block_pre = (double*)take(8 * (scan_blocks + 16));
   w.near3 = (float*)take(4 * 3 * n);
   w.scale_s = (float*)take(4 * n);
-  // split far-field partials: enough for 8 splits of the whole set, at most 2M
-  w.far_partial_cap = n * 8 < (int64_t)(1 << 21) ? n * 8 : (int64_t)(1 << 21);
+  // split far-field partials: splits * targets of far_split_count for any
+  // target count up to n (16 n when n < 65536, below 2^21 otherwise)
+  w.far_partial_cap = n * 16 < (int64_t)(1 << 21) ? n * 16 : (int64_t)(1 << 21);
   w.far_partials = take(32 * w.far_partial_cap);
   int64_t ms = max_samples > 0 ? max_samples : 1;
   w.samples = (float4*)take(32 * ms);

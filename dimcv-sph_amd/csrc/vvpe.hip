@@ -25,6 +25,7 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const int nn = min(st.nbr_count[f], p.nbr_kmax);
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   }
   vv = vmuls(vv, p.vp_factor);
   st3(near3, f, vv);
-  scale_s[f] = (float)(st.dscal[DIMCV_D_KDEN_SUM] - (double)kns);
+  scale_s[f] = (float)(kden_sum_of(st) - (double)kns);
 }
 
 // Final combination (dimcv_sph.py:711-719).
@@ -66,14 +67,23 @@ __global__ void __launch_bounds__(256) k_vv_far_exact(dimcv_state st, dimcv_para
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const float S = scale_s[f];
-  const int M = st.counters[DIMCV_C_NUM_SAMPLES];
+  int M;
+  const int32_t* __restrict__ sidx = st.sample_idx;
+  if (st.block_scene) {  // batched: the scene's own samples
+    const int sc = block_scene_of(st);
+    M = st.scene_counters[(int64_t)sc * DIMCV_SC_COUNT + DIMCV_SC_NUM_SAMPLES];
+    sidx += st.scenes[sc].sample_base;
+  } else {
+    M = st.counters[DIMCV_C_NUM_SAMPLES];
+  }
   v3 vs = mk(0.0f, 0.0f, 0.0f);
   int added = 0;
   for (int js = 0; js < M; ++js) {
-    const int64_t pj = st.sample_idx[js];
+    const int64_t pj = sidx[js];
     v3 r2 = rel(p, xi, ld3(st.x, pj));
     float rn = vnorm(r2);
     if (rn > p.support_radius) {
@@ -212,12 +222,6 @@ __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v
 // [s*chunk, (s+1)*chunk) for targets of block b and stores the partial
 // (A, compensation, rejections); k_far_combine adds the partials in split
 // order.
-struct FarPartial {
-  float a[3];
-  float c[3];
-  int rej;
-  int pad;
-};
 
 template <bool KAHAN, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params p, int32_t bound,
@@ -226,6 +230,7 @@ __global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params 
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const int M = min(*msrc, cap);
@@ -254,6 +259,7 @@ __global__ void k_far_combine(dimcv_state st, dimcv_params p, int32_t bound, con
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
@@ -284,6 +290,7 @@ __global__ void __launch_bounds__(256) k_vv_far_fast(dimcv_state st, dimcv_param
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
@@ -304,6 +311,7 @@ __global__ void __launch_bounds__(256) k_direct_f32(dimcv_state st, dimcv_params
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   v3 acc;
@@ -330,6 +338,7 @@ __global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const double X = xi.x, Y = xi.y, Z = xi.z;
@@ -435,7 +444,7 @@ __global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, dou
   double v = 0.0;
   if (f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
-    if (in_vortex_domain(p, ld3(st.x, i))) v = (double)st.kernel_density[i];
+    if (i >= 0 && in_vortex_domain(p, ld3(st.x, i))) v = (double)st.kernel_density[i];
   }
   w[f] = v;
 }
@@ -563,15 +572,12 @@ __global__ void __launch_bounds__(256) k_compact_write2(Pred pred, int64_t n, co
   if (f) out[pre + (uint32_t)__popcll(b & lt)] = (int32_t)i;
 }
 
-// Source splits so that a small target set still fills the GPU: about
-// 16 waves per SIMD (256 CUs x 4 SIMDs), bounded by the partials workspace.
+// Source splits so that a small target set still fills the GPU
+// (far_split_count).
 static int far_splits(int64_t targets, const Workspace& ws) {
-  const int64_t want_waves = 256LL * 4 * 16;
-  int64_t waves = (targets + 63) / 64;
-  int64_t s = waves > 0 ? (want_waves + waves - 1) / waves : 1;
-  if (s > 16) s = 16;
-  while (s > 1 && s * targets > ws.far_partial_cap) --s;
-  return s < 1 ? 1 : (int)s;
+  const int s = far_split_count(targets);
+  // never binds: far_partial_cap >= min(16 n_max, 2^21) > s * targets
+  return (int64_t)s * targets > ws.far_partial_cap ? 1 : s;
 }
 
 static int launch_far_split(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, const float4* src,
@@ -586,6 +592,241 @@ static int launch_far_split(const dimcv_state* st, const dimcv_params* p, int32_
   hipLaunchKernelGGL(comb, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, msrc, cap, splits,
                      (const FarPartial*)part, ws.near3, ws.scale_s, mode);
   DIMCV_LAUNCHED("far field (split)");
+  return DIMCV_OK;
+}
+
+// ---- batched scenes ------------------------------------------------------------
+__device__ __forceinline__ int scene_of_sample(const dimcv_state& st, int64_t g) {
+  int lo = 0, hi = st.n_scenes;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (st.scenes[mid].sample_base <= g) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+// Canonical scan tiles of every scene's CDF weights: tile t of scene s covers
+// entries [4096 t, 4096 t + 4096) of the scene's segment, as in a solo scan.
+__global__ void __launch_bounds__(256) k_batch_scan_tiles(dimcv_state st, const double* __restrict__ w,
+                                                          double* __restrict__ L, double* __restrict__ btot,
+                                                          const int32_t* __restrict__ misc) {
+  __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
+  const int64_t T = blockIdx.x;
+  if (T >= misc[BM_TILES]) return;  // uniform per block
+  const int s = last_le(st.scene_counters + DIMCV_SC_TILE_START, st.n_scenes, DIMCV_SC_COUNT, T);
+  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  const int64_t t = T - c[DIMCV_SC_TILE_START];
+  const int64_t start = c[DIMCV_SC_FLUID_START] + t * DIMCV_SCAN_TILE;
+  const int64_t m = min((int64_t)DIMCV_SCAN_TILE, (int64_t)c[DIMCV_SC_FLUID] - t * DIMCV_SCAN_TILE);
+  block_local_scan_f64(w + start, L + start, m, tt, l1s, L2s, L1s);
+  if (threadIdx.x == 0) btot[T] = L[start + m - 1];
+}
+
+// Scan of each scene's tile totals (one workgroup per scene).
+__global__ void __launch_bounds__(256) k_batch_scan_totals(dimcv_state st, const double* __restrict__ btot,
+                                                           double* __restrict__ bpre) {
+  __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
+  const int32_t* c = st.scene_counters + (int64_t)blockIdx.x * DIMCV_SC_COUNT;
+  const int64_t nt = ((int64_t)c[DIMCV_SC_FLUID] + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
+  if (nt <= 0 || nt > DIMCV_MAX_SCAN_BLOCKS) return;  // scenes are limited to 4096^2 particles
+  block_local_scan_f64(btot + c[DIMCV_SC_TILE_START], bpre + c[DIMCV_SC_TILE_START], nt, tt, l1s, L2s, L1s);
+}
+
+__global__ void k_batch_scan_add(dimcv_state st, double* __restrict__ L, const double* __restrict__ bpre,
+                                 int64_t bound) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= bound || q >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int s = st.block_scene[q >> 8];
+  if (s < 0) return;
+  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  const int64_t lf = q - c[DIMCV_SC_FLUID_START];
+  if (lf >= c[DIMCV_SC_FLUID]) return;
+  const int64_t t = lf / DIMCV_SCAN_TILE;
+  if (t > 0) L[q] = bpre[c[DIMCV_SC_TILE_START] + t - 1] + L[q];
+  if (lf == c[DIMCV_SC_FLUID] - 1) st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + DIMCV_SD_CDF_TOTAL] = L[q];
+}
+
+// Inverse-CDF draws of every scene (sample g of the batch is draw
+// g - sample_base of its scene), then the packed far-field source of the
+// draw (k_pack_sources, weighted).
+__global__ void k_batch_cdf_draw(dimcv_state st, const double* __restrict__ L, const double* __restrict__ u,
+                                 int64_t m_total, float4* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m_total) return;
+  const int s = scene_of_sample(st, g);
+  if (s < 0) return;
+  const dimcv_scene& sc = st.scenes[s];
+  const int64_t k = g - sc.sample_base;
+  if (k >= sc.m) return;
+  int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  const int64_t F = c[DIMCV_SC_FLUID];
+  const double* cdf = L + c[DIMCV_SC_FLUID_START];
+  const double total = F > 0 ? cdf[F - 1] : 0.0;
+  const bool ok = F > 0 && total > 0.0;
+  if (k == 0) c[DIMCV_SC_NUM_SAMPLES] = ok ? sc.m : 0;
+  if (!ok) {
+    st.sample_idx[g] = c[DIMCV_SC_SLOT_START];
+    return;
+  }
+  const double t = u[g] * total;
+  int64_t lo = 0, hi = F;  // first cdf > t
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > t) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= F) {  // first cdf >= total
+    int64_t a = 0, b = F;
+    while (a < b) {
+      int64_t mid = (a + b) >> 1;
+      if (cdf[mid] >= total) b = mid;
+      else a = mid + 1;
+    }
+    lo = a;
+  }
+  const int64_t slot = st.fluid_list[c[DIMCV_SC_FLUID_START] + lo];
+  st.sample_idx[g] = (int32_t)slot;
+  v3 w = ld3(st.d_vort_smoothed, slot);
+  const float den = st.kernel_density[slot] + 1e-8f;
+  w = mk(w.x / den, w.y / den, w.z / den);
+  out[2 * g] = make_float4(st.x[3 * slot], st.x[3 * slot + 1], st.x[3 * slot + 2], 0.0f);
+  out[2 * g + 1] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+// Persistent far field over a work queue.  Work item q = (scene, 256-target
+// block, source split) in the order set up by the scene set-up (largest
+// per-item cost first); a workgroup takes the next item until none is left.
+// A target's sum is the one of k_vv_far_fast (splits = 1) or k_far_split
+// (splits > 1) for the scene alone: the same 64-lane waves over the scene's
+// fluid entries and the same source ranges.
+template <bool KAHAN>
+__global__ void __launch_bounds__(256) k_batch_far(dimcv_state st, dimcv_params p, const float* __restrict__ near3,
+                                                   const float* __restrict__ scale_s, const float4* __restrict__ src,
+                                                   int32_t* __restrict__ misc, const int32_t* __restrict__ order,
+                                                   const int32_t* __restrict__ item_start,
+                                                   FarPartial* __restrict__ part) {
+  __shared__ int q_sh;
+  const int total = __builtin_amdgcn_readfirstlane(misc[BM_ITEMS]);
+  const float T = p.far_r2_max_rejected;
+  const v3 L0 = mk(0.0f, 0.0f, 0.0f);
+  // The queue grab and its barriers must stay in uniform control flow: the
+  // per-target work below is an if-region (no continue / return), so every
+  // wave reaches the next grab together.
+  for (;;) {
+    if (threadIdx.x == 0) q_sh = atomicAdd(&misc[BM_QUEUE], 1);
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(q_sh);
+    __syncthreads();
+    if (q >= total) break;
+    const int r = last_le(item_start, st.n_scenes, 1, q);
+    const int s = order[r];
+    const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+    const int local = q - item_start[r];
+    const int splits = c[DIMCV_SC_SPLITS];
+    const int tb = local / splits, k = local - tb * splits;
+    const int F = c[DIMCV_SC_FLUID];
+    const int lf = tb * 256 + (int)threadIdx.x;
+    if (lf < F) {
+      const int64_t f = (int64_t)c[DIMCV_SC_FLUID_START] + lf;
+      const int64_t i = st.fluid_list[f];
+      const v3 xi = ld3(st.x, i);
+      if (in_vortex_domain(p, xi)) {
+        const int M = c[DIMCV_SC_NUM_SAMPLES];
+        const float4* sp = src + 2 * (int64_t)st.scenes[s].sample_base;
+        if (splits == 1) {
+          v3 acc;
+          int added;
+          far_sum<KAHAN, false>(sp, M, xi, T, L0, acc, added);
+          vv_finish(st, p, i, ld3(near3, f), vmuls(acc, scale_s[f]), added);
+        } else {
+          const int chunk = (M + splits - 1) / splits;
+          const int j0 = min(k * chunk, M), j1 = min(j0 + chunk, M);
+          float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+          int rej = 0;
+          far_sum_range<KAHAN, false>(sp, j0, j1, xi, T, L0, A, C, rej);
+          FarPartial& o = part[(int64_t)c[DIMCV_SC_PART_BASE] + (int64_t)k * F + lf];
+          for (int d = 0; d < 3; ++d) {
+            o.a[d] = A[d];
+            o.c[d] = C[d];
+          }
+          o.rej = rej;
+        }
+      }
+    }
+  }
+}
+
+// k_far_combine for the split scenes of a batch.
+template <bool KAHAN>
+__global__ void k_batch_far_combine(dimcv_state st, dimcv_params p, int64_t bound, const FarPartial* __restrict__ part,
+                                    const float* __restrict__ near3, const float* __restrict__ scale_s) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= bound || q >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int s = st.block_scene[q >> 8];
+  if (s < 0) return;
+  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  const int splits = c[DIMCV_SC_SPLITS];
+  if (splits == 1) return;
+  const int64_t F = c[DIMCV_SC_FLUID];
+  const int64_t lf = q - c[DIMCV_SC_FLUID_START];
+  if (lf >= F) return;
+  const int64_t i = st.fluid_list[q];
+  const v3 xi = ld3(st.x, i);
+  if (!in_vortex_domain(p, xi)) return;
+  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+  int rej = 0;
+  for (int s2 = 0; s2 < splits; ++s2) {
+    const FarPartial& o = part[(int64_t)c[DIMCV_SC_PART_BASE] + (int64_t)s2 * F + lf];
+    for (int d = 0; d < 3; ++d) {
+      if (KAHAN) {
+        acc_add<true>(A[d], C[d], o.a[d]);
+        acc_add<true>(A[d], C[d], -o.c[d]);
+      } else {
+        A[d] += o.a[d];
+      }
+    }
+    rej += o.rej;
+  }
+  const int M = c[DIMCV_SC_NUM_SAMPLES];
+  vv_finish(st, p, i, ld3(near3, q), vmuls(mk(A[0], A[1], A[2]), scale_s[q]), M - rej);
+}
+
+static int g_far_wgs = 2048;
+void batch_set_far_workgroups(int w) { g_far_wgs = w > 0 ? w : 2048; }
+
+// CDF of every scene, draws and packed sources.
+int batch_cdf_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, BatchWs& bw, const double* u,
+                     int64_t m_total, hipStream_t s) {
+  const int64_t n = p->n_max;
+  hipLaunchKernelGGL(k_cdf_weights, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, (int32_t)n, ws.cdf_weights);
+  hipLaunchKernelGGL(k_batch_scan_tiles, dim3((unsigned)bw.tile_bound), dim3(256), 0, s, *st, ws.cdf_weights,
+                     ws.cdf, bw.btot, bw.misc);
+  hipLaunchKernelGGL(k_batch_scan_totals, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, bw.btot, bw.bpre);
+  hipLaunchKernelGGL(k_batch_scan_add, dim3(nblocks(n)), dim3(256), 0, s, *st, ws.cdf, bw.bpre, n);
+  if (m_total > 0)
+    hipLaunchKernelGGL(k_batch_cdf_draw, dim3(nblocks(m_total)), dim3(256), 0, s, *st, ws.cdf, u, m_total,
+                       ws.samples);
+  DIMCV_LAUNCHED("batched CDF");
+  return DIMCV_OK;
+}
+
+// vort2vel of every scene: near field + S_i, then the far field.
+int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, BatchWs& bw, hipStream_t s) {
+  const int64_t n = p->n_max;
+  hipLaunchKernelGGL(k_vv_near, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, (int32_t)n, ws.near3, ws.scale_s);
+  if (p->far_mode == DIMCV_FAR_EXACT) {
+    hipLaunchKernelGGL(k_vv_far_exact, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, (int32_t)n, ws.near3, ws.scale_s);
+  } else {
+    const bool kahan = p->far_mode == DIMCV_FAR_KAHAN;
+    auto far = kahan ? k_batch_far<true> : k_batch_far<false>;
+    auto comb = kahan ? k_batch_far_combine<true> : k_batch_far_combine<false>;
+    hipLaunchKernelGGL(far, dim3((unsigned)g_far_wgs), dim3(256), 0, s, *st, *p, ws.near3, ws.scale_s, ws.samples,
+                       bw.misc, bw.order, bw.item_start, bw.part);
+    hipLaunchKernelGGL(comb, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, n, (const FarPartial*)bw.part, ws.near3,
+                       ws.scale_s);
+  }
+  DIMCV_LAUNCHED("batched vort2vel");
   return DIMCV_OK;
 }
 

@@ -100,18 +100,16 @@ def ring_velocity(x: torch.Tensor, table: np.ndarray, sigma: float) -> torch.Ten
     return vel
 
 
-def build_ring_system(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.01, seed: int = 2,
-                      mc_samples: int | None = 4096, far_field: str = "fast", dt: float = 2e-4,
-                      viscosity: float = 1e-3, sigma: float | None = None, K: int = 256,
-                      periodic: bool = False, device: str = "cuda"):
-    """ParticleSystem + KarmanVortexSolver for a ring-lattice scene, initialised.
+def make_ring_scene(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.01, seed: int = 2,
+                    mc_samples: int | None = 4096, far_field: str = "fast", dt: float = 2e-4,
+                    viscosity: float = 1e-3, sigma: float | None = None, K: int = 256,
+                    periodic: bool = False, trim: float = 0.0, device: str = "cuda") -> dict:
+    """Configuration, positions and initial velocities of one ring scene.
 
-    Returns (ps, solver, info) where info holds the positions, ring table and
-    initial velocities (host copies) for feeding another implementation.
+    ``trim`` > 0 moves the box's upper x face ``trim`` particle diameters
+    inwards, so that the outermost lattice layer starts outside the box and
+    is deactivated by the first grid build (update_activity).
     """
-    from config_builder import SimConfig
-    from particle_system import ParticleSystem
-
     rng = np.random.default_rng(seed)
     pos = jittered_lattice(nx, ny, nz, r, rng)
     box = np.array([nx, ny, nz], dtype=np.float64) * 2 * r
@@ -120,14 +118,38 @@ def build_ring_system(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.
     table = ring_points(*rings, K=K)
     scene = ring_scene_config(nx, ny, nz, r=r, dt=dt, mc_samples=mc_samples, far_field=far_field,
                               viscosity=viscosity, seed=seed, periodic=periodic)
-    cfg = SimConfig(scene=scene)
-    ps = ParticleSystem(cfg, device=device)
-    xt = torch.as_tensor(pos, device=ps.device)
+    if trim:
+        scene["Configuration"]["domainEnd"][0] = (nx - trim) * 2 * r
+    xt = torch.as_tensor(pos, device=device)
     vel = ring_velocity(xt, table, sigma).cpu().numpy()
+    return {"pos": pos, "vel": vel, "rings": rings, "table": table, "sigma": sigma, "scene": scene}
+
+
+def add_ring_particles(ps, pos: np.ndarray, vel: np.ndarray) -> int:
+    """Append a ring scene's fluid particles (object 0, rest density, colour)."""
     n = pos.shape[0]
-    ps.add_particles(0, n, pos, vel, np.full(n, 1000.0, np.float32), np.zeros(n, np.float32),
-                     np.ones(n, np.int32), np.ones(n, np.int32), np.tile(np.array([[50, 100, 200]], np.int32), (n, 1)))
+    return ps.add_particles(0, n, pos, vel, np.full(n, 1000.0, np.float32), np.zeros(n, np.float32),
+                            np.ones(n, np.int32), np.ones(n, np.int32),
+                            np.tile(np.array([[50, 100, 200]], np.int32), (n, 1)))
+
+
+def build_ring_system(nx: int, ny: int, nz: int, n_rings: int = 8, r: float = 0.01, seed: int = 2,
+                      mc_samples: int | None = 4096, far_field: str = "fast", dt: float = 2e-4,
+                      viscosity: float = 1e-3, sigma: float | None = None, K: int = 256,
+                      periodic: bool = False, trim: float = 0.0, device: str = "cuda"):
+    """ParticleSystem + KarmanVortexSolver for a ring-lattice scene, initialised.
+
+    Returns (ps, solver, info) where info holds the positions, ring table and
+    initial velocities (host copies) for feeding another implementation.
+    """
+    from config_builder import SimConfig
+    from particle_system import ParticleSystem
+
+    info = make_ring_scene(nx, ny, nz, n_rings=n_rings, r=r, seed=seed, mc_samples=mc_samples,
+                           far_field=far_field, dt=dt, viscosity=viscosity, sigma=sigma, K=K,
+                           periodic=periodic, trim=trim, device=device)
+    ps = ParticleSystem(SimConfig(scene=info["scene"]), device=device)
+    add_ring_particles(ps, info["pos"], info["vel"])
     solver = ps.build_solver()
     solver.initialize()
-    info = {"pos": pos, "vel": vel, "rings": rings, "table": table, "sigma": sigma, "scene": scene}
     return ps, solver, info

@@ -118,21 +118,51 @@ typedef struct dimcv_params {
   double vp_factor_f64;         /* 0.1592 * volumeScale * m_V0 in FP64 (direct FP64 sum) */
 } dimcv_params;
 
-/* One scene of a batch (build-defined batching).  All scenes of a batch
- * share the particle radius, rest density, time step and far-field mode;
- * each has its own box, cell grid, viscosity and sample count. */
+/* One scene of a batch (build-defined batching, dimcv_batch_*).  All scenes
+ * of a batch share the particle radius, rest density, time step, DFSPH
+ * tolerances and far-field mode (the dimcv_params of the batch); each has its
+ * own box [0, domain_end] (pos_to_index ignores domain_start, so boxes start
+ * at the origin), cell grid, viscosity and sample count.  The reference has
+ * no batching; viscosity is a module constant there (sph_base.py:14). */
 typedef struct dimcv_scene {
-  int32_t grid_num[3];
-  int32_t cell_base;        /* first global cell of the scene */
-  int32_t n_cells;
-  int32_t m;                /* importance samples per step */
-  int32_t sample_base;      /* offset of the scene's samples in sample_idx / u */
+  int32_t grid_num[3];      /* ceil(domain size / grid_size) (particle_system.py:52-56) */
+  int32_t cell_base;        /* first global cell key of the scene */
+  int32_t n_cells;          /* grid_num[0] * grid_num[1] * grid_num[2] */
+  int32_t m;                /* importance samples per step (> 0) */
+  int32_t sample_base;      /* offset of the scene's samples in sample_idx and u_cdf */
   int32_t pad0;
-  float domain_start[3];
+  float domain_start[3];    /* activity bounds (particle_system.py:413-427) */
   float domain_end[3];
-  float visc_coef;          /* f32(2 (dim+2) nu_scene) */
+  float visc_coef;          /* f32(2 (dim + 2) nu_scene) (sph_base.py:14, dimcv_sph.py:186) */
   float pad1;
 } dimcv_scene;
+
+/* Per-scene device counters of a batch: scene_counters[s * DIMCV_SC_COUNT + k]. */
+typedef enum dimcv_scene_counter {
+  DIMCV_SC_DONE = 0,          /* Jacobi: converged */
+  DIMCV_SC_ITERS = 1,         /* Jacobi: iterations of the current solve */
+  DIMCV_SC_NUM_SAMPLES = 2,   /* samples drawn this step (m, or 0 when kden_sum is 0) */
+  DIMCV_SC_FLUID = 3,         /* fluid-list entries of the scene (fluid_particle_num) */
+  DIMCV_SC_FLUID_START = 4,   /* first fluid-list entry (multiple of 256) */
+  DIMCV_SC_SLOT_START = 5,    /* first slot */
+  DIMCV_SC_SLOTS = 6,         /* active slots (particle_num of the scene) */
+  DIMCV_SC_TILE_START = 7,    /* first CDF scan tile */
+  DIMCV_SC_SPLITS = 8,        /* far-field source splits */
+  DIMCV_SC_PART_BASE = 9,     /* first far-field partial */
+  DIMCV_SC_ITEM_START = 10,   /* first far-field work item */
+  DIMCV_SC_ITERS_V = 11,      /* divergence iterations of the last substep */
+  DIMCV_SC_ITERS_P = 12,      /* pressure iterations of the last substep */
+  DIMCV_SC_FLUID_SRC = 13,    /* first entry in the build's unpadded fluid list */
+  DIMCV_SC_COUNT = 16
+} dimcv_scene_counter;
+
+/* Per-scene FP64 scalars: scene_dscal[s * DIMCV_SD_COUNT + k]. */
+typedef enum dimcv_scene_dscalar {
+  DIMCV_SD_KDEN_SUM = 0,
+  DIMCV_SD_RESIDUAL = 1,
+  DIMCV_SD_CDF_TOTAL = 2,
+  DIMCV_SD_COUNT = 4
+} dimcv_scene_dscalar;
 
 /* Device buffers.  Field names follow the reference (particle_system.py:143-220,
  * dimcv_sph.py:30-53).  Fields marked [sorted] are permuted by the grid build:
@@ -204,18 +234,17 @@ typedef struct dimcv_state {
   double* dscal;             /* [DIMCV_D_COUNT] */
   void* workspace;
   int64_t workspace_bytes;
-  /* batching (all NULL / 0 for a single scene) */
+  /* batching (all NULL / 0 outside dimcv_batch_*) */
   int32_t* scene_id;           /* [sorted] scene of each slot */
   int32_t* scene_id_buffer;
   const dimcv_scene* scenes;   /* [n_scenes] (device) */
   int32_t n_scenes;
   int32_t pad1;
-  int32_t* scene_fluid_start;  /* [n_scenes + 1] fluid-list segment of each scene */
-  int32_t* scene_counters;     /* [n_scenes][4]: done, iterations, num_samples, fluid count */
-  double* scene_dscal;         /* [n_scenes][4]: kden_sum, residual, CDF total, - */
-  const int32_t* seg_blocks;   /* [n_seg_blocks][2]: scene, first fluid entry (256-entry blocks) */
-  int32_t n_seg_blocks;
-  int32_t pad2;
+  int32_t* scene_counters;     /* [n_scenes][DIMCV_SC_COUNT] */
+  double* scene_dscal;         /* [n_scenes][DIMCV_SD_COUNT] */
+  int32_t* block_scene;        /* [n_max / 256 + 1]: scene of each 256-entry fluid-list block, -1 none */
+  void* batch_workspace;       /* dimcv_batch_workspace_bytes */
+  int64_t batch_workspace_bytes;
 } dimcv_state;
 
 /* ---- library ---------------------------------------------------------------- */
@@ -376,6 +405,60 @@ int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precis
  * FP64 accumulation, written as f32. */
 int dimcv_ring_velocity(int64_t npts, const double* pts, double inv_sigma2, int64_t n,
                         const float* x, float* vel, void* stream);
+
+/* ---- batched scenes (build-defined) ----------------------------------------- */
+/* Many independent scenes advance together in one set of launches.  Slots of
+ * scene s are contiguous and ordered by scene (inactive slots of every scene
+ * follow the last scene); cell keys are cell_base + the scene's own flattened
+ * cell index, so the stable sort keeps each scene in its solo order.  The
+ * fluid list gives each scene a segment starting at a multiple of 256 (unused
+ * entries hold -1), so every per-block FP64 partial, every canonical
+ * reduction and scan, every Jacobi convergence test and the far-field source
+ * splitting of a scene are those of the same scene run alone: each scene's
+ * results are bitwise equal to its solo run.  Batches are open boxes with no
+ * solids, emitters, vortex domain or transit area, in fixed-M (inverse-CDF)
+ * sampling mode.  Nothing here synchronises the host except
+ * dimcv_batch_substep's convergence checks and dimcv_batch_status.
+ *
+ * Capacities: n_max >= total particles + 256 * n_scenes; block_scene holds
+ * n_max / 256 + 1 entries; n_cells (params) = sum of scene cells.
+ * scene_particles: host array of the scenes' particle counts (upper bounds). */
+int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_particles, int64_t n_max);
+/* Grid build of every scene (activity, keys, sort, cell table, gather into
+ * the *_buffer fields including scene_id, padded fluid list, per-scene
+ * counters).  Every slot below n_max is scanned (slots at or past
+ * particle_num get the sentinel key).  The caller swaps the sorted buffers
+ * afterwards, then builds the neighbour lists. */
+int dimcv_batch_grid_build(const dimcv_state* st, const dimcv_params* p, void* stream);
+/* Neighbour lists of the current (swapped) buffers. */
+int dimcv_batch_neighbor_build(const dimcv_state* st, const dimcv_params* p, void* stream);
+/* Reads the error counters of the last build (key outside the grid,
+ * neighbour-list overflow) and the counters; synchronises `stream`. */
+int dimcv_batch_status(const dimcv_state* st, const dimcv_params* p, int32_t* host_counters, void* stream);
+/* The substep of every scene (dimcv_substep per scene).  opts->u_cdf holds the
+ * uniforms of all scenes (scene s at scenes[s].sample_base); opts->m is the
+ * total sample count; fluid_count and the u1/u2 inputs are unused.
+ * out_iters (host, optional): [n_scenes][2] divergence / pressure iterations. */
+int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
+                        int32_t* out_iters, void* stream);
+
+/* Launch shapes of the batched far field (result-invariant; chosen by the
+ * autotuner of the Python layer).  workgroups = 0 restores the default. */
+int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block);
+
+/* Step graphs: the batched step (grid build, neighbour lists, substep)
+ * captured into HIP graphs, one per segment between host convergence checks
+ * (before/inside/after the two Jacobi solves), for both parities of the
+ * sorted-buffer swap.  st_even / st_odd: the state before an even / odd step
+ * (buffers swapped).  `stream` must not be the null stream.  The uniforms are
+ * read from opts->u_cdf at replay time, so the caller refreshes that buffer
+ * before each step. */
+typedef struct dimcv_batch_graph dimcv_batch_graph;
+int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_odd, const dimcv_params* p,
+                             const dimcv_step_opts* opts, void* stream, dimcv_batch_graph** out);
+/* One step from the captured graphs; parity selects st_even / st_odd. */
+int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_iters, void* stream);
+void dimcv_batch_graph_destroy(dimcv_batch_graph* g);
 
 #ifdef __cplusplus
 }

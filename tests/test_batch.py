@@ -1,0 +1,93 @@
+"""Batched scenes: every scene of a batch is bitwise its solo run.
+
+The solo run of a scene is ``scenes.build_ring_system`` + the native
+``KarmanVortexSolver.step`` (itself bit-exact against the oracle in the exact
+far-field mode, test_gpu_parity.py).  The batch holds scenes of uneven sizes
+and viscosities, one of them in a trimmed box whose outer lattice layer is
+deactivated by the first grid build (so the scenes' slot ranges move).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PS_COMPARE = ["x", "v", "density", "pressure", "dfsph_factor", "density_adv", "vorticity", "vorticity_vis",
+              "life_time", "material", "object_id", "grid_ids"]
+SOLVER_COMPARE = ["kernel_density", "kinematic_vort_num", "d_vort_smoothed", "d_vort", "v_star", "grad_v",
+                  "x_temp", "v_vort"]
+
+
+def _specs():
+    from batch import RingSpec
+    return [RingSpec(12, 10, 8, n_rings=2, seed=11, viscosity=3e-4),
+            RingSpec(20, 14, 12, n_rings=3, seed=12, viscosity=5e-3, trim=0.6),
+            RingSpec(16, 16, 16, n_rings=4, seed=13, viscosity=1e-3, mc_samples=300)]
+
+
+def _grid_ids_local(batch, s, a):
+    return a - batch.cell_base[s]
+
+
+def _compare_scene(batch, s, ps, sv):
+    import _native as N
+    sc = batch.scene_counters_host()[s]
+    k = int(sc[N.SC_SLOTS])
+    assert k == ps.particle_num[None], (s, k, ps.particle_num[None])
+    assert int(sc[N.SC_FLUID]) == ps.fluid_particle_num[None]
+    bad = []
+    for name in PS_COMPARE:
+        b = batch.scene_field(name, s)
+        r = getattr(ps, name).to_numpy()[:k]
+        if name == "grid_ids":
+            b = _grid_ids_local(batch, s, b)
+        if not np.array_equal(b.view(np.uint8), np.ascontiguousarray(r).view(np.uint8)):
+            bad.append(name)
+    for name in SOLVER_COMPARE:
+        b = batch.scene_field(name, s)
+        r = getattr(sv, name).to_numpy()[:k]
+        if not np.array_equal(b.view(np.uint8), np.ascontiguousarray(r).view(np.uint8)):
+            bad.append(name)
+    assert not bad, (s, bad)
+    m = int(sc[N.SC_NUM_SAMPLES])
+    assert m == sv.num_samples[None]
+    assert np.array_equal(batch.scene_samples(s), sv.sample_idx.to_numpy()[:m])
+    assert tuple(batch.last_iterations[s]) == (sv.last_iterations_v, sv.last_iterations_p)
+
+
+@pytest.mark.parametrize("far", ["fast", "kahan", "exact"])
+def test_batch_scenes_equal_solo_runs(far):
+    from batch import SceneBatch, solo_system
+    specs = _specs()
+    batch = SceneBatch(specs, far_field=far)
+    solos = [solo_system(sp, far_field=far) for sp in specs]
+    for s, (ps, sv) in enumerate(solos):
+        assert batch.scene_counters_host()[s][3] == ps.fluid_particle_num[None]
+    for step in range(3):
+        batch.step()
+        for ps, sv in solos:
+            sv.step()
+        for s, (ps, sv) in enumerate(solos):
+            _compare_scene(batch, s, ps, sv)
+    # the trimmed scene lost its outer layer; the others kept every particle
+    assert batch.scene_counters_host()[1][6] == 19 * 14 * 12
+    assert batch.scene_counters_host()[0][6] == 12 * 10 * 8
+
+
+def test_graph_steps_equal_eager_steps_and_autotune_keeps_state():
+    from batch import SceneBatch, autotune
+    specs = _specs()
+    eager = SceneBatch(specs)
+    graph = SceneBatch(specs)
+    cfg = autotune(graph, far_workgroups=(256, 2048), nbr_blocks=(64, 256), steps=1)
+    assert cfg["far_workgroups"] in (256, 2048) and cfg["nbr_block"] in (64, 256)
+    graph.capture()
+    for step in range(4):   # both parities of the buffer swap, twice
+        eager.step()
+        graph.step_graph()
+        for s in range(len(specs)):
+            for name in ["x", "v", "density", "density_adv", "dfsph_factor"]:
+                a, b = eager.scene_field(name, s), graph.scene_field(name, s)
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (step, s, name)
+            assert np.array_equal(eager.scene_samples(s), graph.scene_samples(s))
+        assert np.array_equal(eager.last_iterations, graph.last_iterations)
+    graph.release_graph()
