@@ -218,6 +218,8 @@ def run_batching(n_scenes=64, steps=10, warmup=1, seed=7):
                                   getattr(ps, f).to_numpy()[:k].view(np.uint8)) for f in ("x", "v"))
         equal += int(same)
         del ps, sv
+        if (s + 1) % 16 == 0:
+            log(f"sequential: {s + 1}/{n_scenes} scenes, {t_seq * 1e3:.1f} ms so far, {equal} bitwise equal")
     batch.release_graph()
     out = {"workload": f"{n_scenes} ring scenes, N log-uniform [8192, 131072], M = 2% N",
            "n_scenes": n_scenes, "n_particles": n_total, "steps": steps,
