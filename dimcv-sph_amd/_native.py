@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
 SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip",
-           "csrc/batch.hip"]
+           "csrc/batch.hip", "csrc/adjoint.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -94,7 +94,7 @@ STATE_POINTERS_2 = ["fluid_list", "nbr_count", "nbr", "v_vort", "counters", "dsc
 # Fields the grid build permutes into their *_buffer twin (dimcv_state [sorted]).
 SORTED_FIELDS = ["object_id", "x", "x_0", "v", "m_V", "m", "density", "pressure", "material", "color",
                  "is_dynamic", "dfsph_factor", "density_adv", "is_active", "is_sample", "life_time",
-                 "grid_ids"]
+                 "grid_ids", "particle_id"]
 
 
 class StepOpts(ctypes.Structure):
@@ -108,6 +108,19 @@ class StepOpts(ctypes.Structure):
         ("out_kv", ctypes.c_int32), ("out_kp", ctypes.c_int32),
         ("time_far", ctypes.c_int32), ("out_far_ms", ctypes.c_float),
     ]
+
+
+# dimcv_adj_pass_id
+(ADJ_DENSITY, ADJ_FACTOR, ADJ_DIV, ADJ_JACOBI, ADJ_NONPRESSURE, ADJ_CURL, ADJ_GRAD_V, ADJ_SMOOTH, ADJ_KVN,
+ ADJ_KDEN, ADJ_GATE, ADJ_NEAR, ADJ_FAR_TARGETS, ADJ_FAR_SOURCES) = range(14)
+
+
+class AdjArgs(ctypes.Structure):
+    """dimcv_adj_args (include/dimcv.h)."""
+    _fields_ = [("f0", ctypes.c_void_p), ("f1", ctypes.c_void_p), ("f2", ctypes.c_void_p),
+                ("g_in", ctypes.c_void_p), ("x_bar", ctypes.c_void_p), ("b0", ctypes.c_void_p),
+                ("b1", ctypes.c_void_p), ("b2", ctypes.c_void_p), ("dscalar", ctypes.c_double),
+                ("scalar", ctypes.c_float), ("mode", ctypes.c_int32)]
 
 
 class Scene(ctypes.Structure):
@@ -128,7 +141,8 @@ class State(ctypes.Structure):
                    ("scenes", ctypes.c_void_p), ("n_scenes", ctypes.c_int32), ("pad1", ctypes.c_int32),
                    ("scene_counters", ctypes.c_void_p), ("scene_dscal", ctypes.c_void_p),
                    ("block_scene", ctypes.c_void_p), ("batch_workspace", ctypes.c_void_p),
-                   ("batch_workspace_bytes", ctypes.c_int64)])
+                   ("batch_workspace_bytes", ctypes.c_int64),
+                   ("particle_id", ctypes.c_void_p), ("particle_id_buffer", ctypes.c_void_p)])
 
 
 def lib():
@@ -171,6 +185,7 @@ def lib():
         "dimcv_batch_graph_create": (I32, [P, P, P, P, P, P]),
         "dimcv_batch_graph_step": (I32, [P, I32, P, P]),
         "dimcv_batch_graph_destroy": (None, [P]),
+        "dimcv_adj_pass": (I32, [P, P, I32, P, I32, P]),
     }
     for name in ["dimcv_compute_densities", "dimcv_compute_DFSPH_factor",
                  "dimcv_compute_density_change", "dimcv_divergence_solver_iteration",

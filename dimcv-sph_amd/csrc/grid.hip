@@ -262,6 +262,7 @@ __global__ void k_gather(dimcv_state st, int64_t n_in, const uint32_t* __restric
   st.is_sample_buffer[s] = st.is_sample[i];
   st.life_time_buffer[s] = st.life_time[i];
   if (st.scene_id) st.scene_id_buffer[s] = st.scene_id[i];
+  if (st.particle_id) st.particle_id_buffer[s] = st.particle_id[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -128,7 +128,7 @@ class ParticleSystem:
             "vorticity": (3, F32, False), "vorticity_vis": (3, F32, False),
             "dfsph_factor": (0, F32, False), "density_adv": (0, F32, False),
             "is_active": (0, U8, True), "is_sample": (0, U8, True), "life_time": (0, F32, False),
-            "grid_ids": (0, I32, False),
+            "grid_ids": (0, I32, False), "particle_id": (0, I32, False),
         }
         for name, (c, dt, b) in spec.items():
             setattr(self, name, Field(n, c, dt, dev, as_bool=b))
@@ -146,6 +146,7 @@ class ParticleSystem:
         self._workspace = torch.zeros(int(ws), dtype=torch.uint8, device=dev)
         self._host_counters = np.zeros(N.C_COUNT, dtype=np.int32)
         self._pn = 0          # exact particle_num, mirrored on the host
+        self._n_inserted = 0  # particles inserted so far (next particle_id)
         self._nfl = 0         # exact fluid-list length after the last grid build
         self._nbr_valid = False
         self.solver = None
@@ -226,6 +227,8 @@ class ParticleSystem:
         s.nbr = self.nbr.data_ptr()
         vv = getattr(self.solver, "v_vort", None) if self.solver is not None else None
         s.v_vort = vv.ptr() if vv is not None else None
+        s.particle_id = self.particle_id.ptr()
+        s.particle_id_buffer = self.particle_id_buffer.ptr()
         s.counters = self._counters.data_ptr()
         s.dscal = self._dscal.data_ptr()
         s.workspace = self._workspace.data_ptr()
@@ -278,6 +281,11 @@ class ParticleSystem:
                   den.data_ptr(), pre.data_ptr(), mat.data_ptr(), dyn.data_ptr(), col.data_ptr(),
                   dropped.ctypes.data, self.stream())
         added = n - int(dropped[0])
+        # build-owned identity: insertion index of each particle, carried
+        # through every sort (particle_id)
+        self.particle_id.t[self._pn:self._pn + added] = torch.arange(
+            self._n_inserted, self._n_inserted + added, dtype=torch.int32, device=self.device)
+        self._n_inserted += added
         self._pn += added
         self._nbr_valid = False
         return added

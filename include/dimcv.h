@@ -245,6 +245,10 @@ typedef struct dimcv_state {
   int32_t* block_scene;        /* [n_max / 256 + 1]: scene of each 256-entry fluid-list block, -1 none */
   void* batch_workspace;       /* dimcv_batch_workspace_bytes */
   int64_t batch_workspace_bytes;
+  /* optional [sorted] particle identity (insertion index), carried by the
+   * grid build's gather like every sorted field; NULL = not tracked */
+  int32_t* particle_id;
+  int32_t* particle_id_buffer;
 } dimcv_state;
 
 /* ---- library ---------------------------------------------------------------- */
@@ -459,6 +463,70 @@ int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_o
 /* One step from the captured graphs; parity selects st_even / st_odd. */
 int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_iters, void* stream);
 void dimcv_batch_graph_destroy(dimcv_batch_graph* g);
+
+/* ---- adjoint passes (build-defined differentiation) ------------------------ */
+/* Vector-Jacobian products of the step's passes with the step's discrete
+ * structure frozen (neighbour lists, samples, iteration counts and every gate
+ * keep their forward values).  One entry, dimcv_adj_pass, runs the VJP of one
+ * pass over the fluid list of `st` (the forward's neighbour lists and, for the
+ * far field, its samples), reading forward values and the upstream adjoint
+ * and ADDING into the adjoint outputs.  x_bar is the adjoint of the positions
+ * x of `st`.  Open single-scene systems only.
+ *
+ *  pass                 reference          f0 / f1 / f2                  g_in        outputs
+ *  DENSITY              dimcv_sph.py:126   -                             rho_bar     x_bar
+ *  FACTOR               :86-124            alpha                         alpha_bar   x_bar
+ *  DIV       mode 0     :337-374           v / b (gate b > 0)            b_bar       x_bar, b1 = v_bar
+ *            mode 1     :470-493           v / da (gate da > rho0)       da_bar      x_bar, b1 = v_bar, b0 = rho_bar
+ *  JACOBI    mode 0     :421-468           b / alpha / rho (k = b a rho) P_bar       x_bar, b0 = kappa_bar
+ *            mode 1     :495-549           da / alpha (k = (da-rho0) a)  P_bar       x_bar, b0 = kappa_bar
+ *  NONPRESSURE          :150-208           v                             a_bar       x_bar, b1 = v_bar
+ *  CURL                 :228-248, :551-583 u (v or v*)                   C_bar       x_bar, b1 = u_bar
+ *  GRAD_V               :250-262           v                             G_bar (9)   x_bar, b1 = v_bar
+ *  SMOOTH               :660-677           d_vort                        w_bar       x_bar, b1 = d_vort_bar
+ *  KVN                  :295-312           grad_v (9)                    kvn_bar     b1 = grad_v_bar (9)
+ *  KDEN                 :721-751           kvn                           kden_bar    x_bar, b1 = kvn_bar
+ *  GATE                 :717-719           v_vort / v                    -           b2 = gate (1 / 0)
+ *  FAR_TARGETS          :693-716           w / kden; dscalar = kden_sum  u_bar       x_bar, b0 = S_bar, b2 = weights
+ *  FAR_SOURCES          :693-716           w / kden / weights; mode = M  -           x_bar, b0 = w_bar, b1 = kden_bar
+ *  NEAR                 :649-658, 687-692  w / S_bar                     u_bar       x_bar, b0 = w_bar, b1 = kden_bar
+ *
+ * P_bar is the adjoint of the sum a Jacobi iteration subtracts from v (minus
+ * the adjoint of the iteration's output velocity); u_bar is the gated
+ * adjoint of the recovered velocity. */
+typedef enum dimcv_adj_pass_id {
+  DIMCV_ADJ_DENSITY = 0,
+  DIMCV_ADJ_FACTOR = 1,
+  DIMCV_ADJ_DIV = 2,
+  DIMCV_ADJ_JACOBI = 3,
+  DIMCV_ADJ_NONPRESSURE = 4,
+  DIMCV_ADJ_CURL = 5,
+  DIMCV_ADJ_GRAD_V = 6,
+  DIMCV_ADJ_SMOOTH = 7,
+  DIMCV_ADJ_KVN = 8,
+  DIMCV_ADJ_KDEN = 9,
+  DIMCV_ADJ_GATE = 10,
+  DIMCV_ADJ_NEAR = 11,
+  DIMCV_ADJ_FAR_TARGETS = 12,
+  DIMCV_ADJ_FAR_SOURCES = 13
+} dimcv_adj_pass_id;
+
+typedef struct dimcv_adj_args {
+  const float* f0;
+  const float* f1;
+  const float* f2;
+  const float* g_in;
+  float* x_bar;
+  float* b0;
+  float* b1;
+  float* b2;
+  double dscalar;
+  float scalar;
+  int32_t mode;
+} dimcv_adj_args;
+
+int dimcv_adj_pass(const dimcv_state* st, const dimcv_params* p, int32_t pass, const dimcv_adj_args* args,
+                   int32_t fluid_bound, void* stream);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ int main(void) {
   O(dimcv_state, n_emitters) O(dimcv_state, fluid_list) O(dimcv_state, workspace)
   O(dimcv_state, workspace_bytes) O(dimcv_state, scene_id) O(dimcv_state, n_scenes)
   O(dimcv_state, scene_counters) O(dimcv_state, block_scene) O(dimcv_state, batch_workspace_bytes)
+  O(dimcv_state, particle_id_buffer)
   O(dimcv_scene, cell_base) O(dimcv_scene, domain_start) O(dimcv_scene, visc_coef)
   return 0;
 }
