@@ -20,10 +20,12 @@ Also measured in the same run:
     bounded sample of the same 1M workload.
 
   * batched scenes: 64 uneven ring scenes stepped by one batched, graph-
-    replayed step against the same scenes stepped one after another.
+    replayed step against the same scenes stepped one after another;
+  * differentiation: forward and adjoint times of a 32-step, 65,536-particle
+    ring rollout with 16 circulations as parameters.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                       [--no-direct] [--no-batching] [--batch-steps K]
+                       [--no-direct] [--no-batching] [--batch-steps K] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -234,6 +236,42 @@ def run_batching(n_scenes=64, steps=10, warmup=1, seed=7):
     return out
 
 
+def run_adjoint(steps=32, checkpoint=8):
+    """Differentiation workload: 65,536 particles (64 x 32 x 32 lattice, two
+    spacings clear of every wall), 16
+    rings whose circulations are the parameters, a 32-step FP32 rollout with
+    checkpoints every 8 steps, L2 loss against the rollout of seeded
+    perturbed circulations; forward and adjoint (backward) times."""
+    import numpy as np
+    import torch
+    from adjoint import RingRollout, rollout
+
+    t0 = time.time()
+    model = RingRollout(64, 32, 32, n_rings=16, steps=steps, checkpoint=checkpoint, mc_samples=1024, seed=5, pad=2)
+    g0 = model.gamma0.copy()
+    gt = g0 + np.random.default_rng(3).normal(0.0, 0.1, g0.shape)
+    v_target = model.forward(gt).double()
+    log(f"adjoint scene ready in {time.time() - t0:.1f}s ({model.n} particles)")
+    gamma = torch.tensor(g0, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    v = rollout(gamma, model)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    loss = ((v.double() - v_target) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    g = gamma.grad.numpy()
+    out = {"workload": "65,536 particles, 16 rings (circulations as parameters), 32 steps, checkpoint 8",
+           "n_particles": model.n, "steps": steps, "checkpoint": checkpoint, "mc_samples": 1024,
+           "forward_ms": round((t2 - t1) * 1e3, 2), "backward_ms": round((t3 - t2) * 1e3, 2),
+           "backward_over_forward": round((t3 - t2) / (t2 - t1), 2), "loss": loss.item(),
+           "grad_norm": float(np.linalg.norm(g)), "grad": [round(float(x), 6) for x in g]}
+    log(f"adjoint: {json.dumps(out)}")
+    return out
+
+
 def run_cpu_baseline(info, steps=1):
     """The oracle on a bounded sample of the 1M workload: one step."""
     import copy
@@ -265,6 +303,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -280,6 +319,7 @@ def main():
         direct = {"comparator_131072": run_recovery_comparison(64, 64, 32, [4096]),
                   "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536])}
     batching = None if args.no_batching else run_batching(steps=args.batch_steps)
+    adjoint = None if args.no_adjoint else run_adjoint()
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(info)
     value = res["n_fluid"] * args.steps / res["sec"]
     traffic = None
@@ -313,6 +353,7 @@ def main():
         "cpu_baseline": cpu,
         "mc_vs_direct": direct,
         "batching": batching,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 
