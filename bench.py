@@ -93,6 +93,15 @@ def run_gpu(args):
         "n_fluid": n_fluid, "sec": sec, "far_ms": far_ms, "achieved_tflops": achieved,
         "pairs": pairs, "kv": float(np.mean(kv)), "kp": float(np.mean(kp)), "m": m,
     }
+    if args.traffic_probe:
+        # a plain copy of known size (64 Mi f32 read, 64 Mi f32 written): the
+        # FETCH_SIZE / WRITE_SIZE counters of this kernel scale the far field's
+        a = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log("traffic probe: 3 x elementwise copy of 256 MiB done")
     del solver, ps
     torch.cuda.empty_cache()
     return res, info
@@ -304,6 +313,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="after the timed steps, run a 256 MiB elementwise copy (PMC scale reference)")
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

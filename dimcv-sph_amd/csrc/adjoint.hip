@@ -91,7 +91,7 @@ __device__ __forceinline__ float a_kernel(const dimcv_params& p, float s, float&
 }
 
 #define ADJ_ENTRY()                                                    \
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   \
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;   \
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;     \
   const int64_t i = st.fluid_list[f];                                  \
   if (i < 0) return;                                                   \
@@ -323,7 +323,7 @@ __global__ void k_adj_smooth(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
 // kvn = sqrt(Wn) / (sqrt(Sn) + 1e-8) of grad_v (dimcv_sph.py:295-312),
 // elementwise.  in: f0 = grad_v (9), g_in = kvn_bar.  out: b1 = grad_v_bar (9).
 __global__ void k_adj_kvn(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -370,7 +370,7 @@ __global__ void k_adj_kden(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
 // The update gate of vort2vel: 1 where |v_vort| < |v| (dimcv_sph.py:717-719).
 // in: f0 = v_vort, f1 = v (before the update).  out: b2 = gate (1.0 / 0.0).
 __global__ void k_adj_gate(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;

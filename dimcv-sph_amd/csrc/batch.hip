@@ -434,8 +434,7 @@ int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_parti
 
 int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block) {
   if (far_workgroups < 0 || far_workgroups > 65535 ||
-      !(nbr_block == 0 || nbr_block == 64 || nbr_block == 128 || nbr_block == 256 || nbr_block == 512 ||
-        nbr_block == 1024)) {
+      !(nbr_block == 0 || nbr_block == 64 || nbr_block == 128 || nbr_block == 256)) {
     set_error("dimcv_batch_set_launch: workgroups %d or block %d out of range", far_workgroups, nbr_block);
     return DIMCV_ERR_ARG;
   }

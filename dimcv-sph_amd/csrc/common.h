@@ -221,20 +221,34 @@ __device__ __forceinline__ v3 rel(const dimcv_params& p, v3 xi, v3 xj) {
 }
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order.  gfx950 dispatches workgroups round-robin over its 8
+// XCDs (workgroup b runs on XCD b % 8), each with its own L2.  Passes over the
+// fluid list use the logical block xcd_block() instead of blockIdx.x, which
+// hands every XCD one contiguous range of blocks: consecutive fluid-list
+// entries are spatial neighbours (cell-sorted), so a particle's neighbour data
+// is mostly already in that XCD's L2.  The map is a bijection, so every
+// entry is processed exactly once whatever the dispatch order.
+__device__ __forceinline__ int64_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+  return (int64_t)(x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// ---------------------------------------------------------------------------
 // Batched scenes.  A batch's fluid list holds one segment per scene starting
 // at a multiple of 256; unused entries hold -1 and every pass skips them.
 // block_scene[b] names the scene of 256-entry block b (-1: none).
 __device__ __forceinline__ int block_scene_of(const dimcv_state& st) {
-  return st.block_scene ? st.block_scene[blockIdx.x] : 0;
+  return st.block_scene ? st.block_scene[xcd_block()] : 0;
 }
 
 // Jacobi gate: the solve of this block's scene (batched: done points at
 // scene_counters + DIMCV_SC_DONE) or the single solve has converged.
-// Launches using it are 256-wide over fluid-list entries.
+// Launches using it are 256-wide over fluid-list entries, in xcd_block order.
 __device__ __forceinline__ bool jacobi_gated(const dimcv_state& st, const int32_t* done) {
   if (!done) return false;
   if (st.block_scene) {
-    const int s = st.block_scene[blockIdx.x];
+    const int s = st.block_scene[xcd_block()];
     return s < 0 || done[(int64_t)s * DIMCV_SC_COUNT] != 0;
   }
   return *done != 0;
@@ -243,7 +257,7 @@ __device__ __forceinline__ bool jacobi_gated(const dimcv_state& st, const int32_
 // kden_sum of the scene of this block.
 __device__ __forceinline__ double kden_sum_of(const dimcv_state& st) {
   if (st.block_scene)
-    return st.scene_dscal[(int64_t)st.block_scene[blockIdx.x] * DIMCV_SD_COUNT + DIMCV_SD_KDEN_SUM];
+    return st.scene_dscal[(int64_t)st.block_scene[xcd_block()] * DIMCV_SD_COUNT + DIMCV_SD_KDEN_SUM];
   return st.dscal[DIMCV_D_KDEN_SUM];
 }
 

@@ -390,22 +390,47 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __
   }
 }
 
-__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
-  int64_t i = st.fluid_list[f];
-  v3 xi = ld3(st.x, i);
-  int cnt = 0;
-  const int kmax = p.nbr_kmax;
+// Neighbour lists are k-major (nbr[k * n_max + f]) so that the passes read
+// 64 consecutive entries per wave and row.  Lanes find their k-th neighbour
+// at different times, so writing entries as found scatters 4-byte stores
+// over many rows and partially written lines leave L2 before they fill.  The
+// first NBR_STAGE entries of each lane are therefore staged in LDS and
+// written row by row by the whole wave (one coalesced 256-byte row store);
+// entries past NBR_STAGE are written directly.
+#define NBR_STAGE 32
+
+__device__ __forceinline__ void nbr_flush(dimcv_state& st, const dimcv_params& p, uint32_t (*stage)[64], int lane,
+                                          int64_t f, int cnt) {
+  const int c = min(cnt, min(NBR_STAGE, p.nbr_kmax));
+  int cw = c;
+  for (int o = 32; o > 0; o >>= 1) cw = max(cw, __shfl_xor(cw, o, 64));
   const int64_t stride = p.n_max;
-  auto emit = [&](int j) {
-    if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
-    ++cnt;
-  };
-  if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, emit);
-  else scan_runs(p, st.x, st.grid_particles_num, i, xi, emit);
-  st.nbr_count[f] = cnt;
-  atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+  for (int k = 0; k < cw; ++k)
+    if (k < c) st.nbr[(int64_t)k * stride + f] = stage[k][lane];
+}
+
+__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
+  __shared__ uint32_t stage[4][NBR_STAGE][64];
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int cnt = 0;
+  if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
+    const int64_t i = st.fluid_list[f];
+    const v3 xi = ld3(st.x, i);
+    const int kmax = p.nbr_kmax;
+    const int64_t stride = p.n_max;
+    auto emit = [&](int j) {
+      const uint32_t e = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
+      if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
+      else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
+      ++cnt;
+    };
+    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, emit);
+    else scan_runs(p, st.x, st.grid_particles_num, i, xi, emit);
+    st.nbr_count[f] = cnt;
+    atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+  }
+  nbr_flush(st, p, stage[w], lane, f, cnt);
 }
 
 // sph_base.py:91-112: Akinci boundary volume over solid neighbours.
@@ -503,51 +528,54 @@ __global__ void k_batch_keys(dimcv_state st, dimcv_params p, int64_t n_in, uint3
 
 // Neighbour lists of fluid-list entry q over the 9 z-runs of its scene's
 // grid; the scene's own cell 0 stays invisible (its run starts at the end of
-// that cell), as in a solo build.
-__global__ void k_batch_nbr_build(dimcv_state st, dimcv_params p, int64_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
-  const int64_t i = st.fluid_list[f];
-  if (i < 0) {
-    st.nbr_count[f] = 0;
-    return;
-  }
-  const dimcv_scene& sc = st.scenes[st.scene_id[i]];
-  const v3 xi = ld3(st.x, i);
-  int cc[3];
-  cc[0] = (int)(xi.x / p.grid_size);
-  cc[1] = (int)(xi.y / p.grid_size);
-  cc[2] = (int)(xi.z / p.grid_size);
-  const int nx = sc.grid_num[0], ny = sc.grid_num[1], nz = sc.grid_num[2];
-  const int32_t* __restrict__ gpn = st.grid_particles_num + sc.cell_base;
-  const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, nz - 1);
-  const float lim = p.nbr_r2_limit;
-  const int kmax = p.nbr_kmax;
-  const int64_t stride = p.n_max;
+// that cell), as in a solo build.  Entries are staged as in k_nbr_build
+// (workgroups of at most 256 threads).
+__global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_params p, int64_t bound) {
+  __shared__ uint32_t stage[4][NBR_STAGE][64];
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int cnt = 0;
-  for (int o0 = -1; o0 <= 1; ++o0) {
-    const int g0 = cc[0] + o0;
-    if (g0 < 0 || g0 >= nx) continue;
-    for (int o1 = -1; o1 <= 1; ++o1) {
-      const int g1 = cc[1] + o1;
-      if (g1 < 0 || g1 >= ny) continue;
-      const int fl = (g0 * ny + g1) * nz;
-      const int lo = fl + zlo - 1;
-      const int beg = lo >= 0 ? gpn[lo] : gpn[0];
-      const int end = gpn[fl + zhi];
-      for (int j = beg; j < end; ++j) {
-        const v3 r = vsub(xi, ld3(st.x, j));
-        const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-        if (r2 < lim && j != i) {
-          if (cnt < kmax)
-            st.nbr[(int64_t)cnt * stride + f] = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
-          ++cnt;
+  const int64_t i = (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) ? st.fluid_list[f] : -2;
+  if (i == -1) st.nbr_count[f] = 0;
+  if (i >= 0) {
+    const dimcv_scene& sc = st.scenes[st.scene_id[i]];
+    const v3 xi = ld3(st.x, i);
+    int cc[3];
+    cc[0] = (int)(xi.x / p.grid_size);
+    cc[1] = (int)(xi.y / p.grid_size);
+    cc[2] = (int)(xi.z / p.grid_size);
+    const int nx = sc.grid_num[0], ny = sc.grid_num[1], nz = sc.grid_num[2];
+    const int32_t* __restrict__ gpn = st.grid_particles_num + sc.cell_base;
+    const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, nz - 1);
+    const float lim = p.nbr_r2_limit;
+    const int kmax = p.nbr_kmax;
+    const int64_t stride = p.n_max;
+    for (int o0 = -1; o0 <= 1; ++o0) {
+      const int g0 = cc[0] + o0;
+      if (g0 < 0 || g0 >= nx) continue;
+      for (int o1 = -1; o1 <= 1; ++o1) {
+        const int g1 = cc[1] + o1;
+        if (g1 < 0 || g1 >= ny) continue;
+        const int fl = (g0 * ny + g1) * nz;
+        const int lo = fl + zlo - 1;
+        const int beg = lo >= 0 ? gpn[lo] : gpn[0];
+        const int end = gpn[fl + zhi];
+        for (int j = beg; j < end; ++j) {
+          const v3 r = vsub(xi, ld3(st.x, j));
+          const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+          if (r2 < lim && j != i) {
+            const uint32_t e = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
+            if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
+            else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
+            ++cnt;
+          }
         }
       }
     }
+    st.nbr_count[f] = cnt;
+    atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
   }
-  st.nbr_count[f] = cnt;
-  atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+  nbr_flush(st, p, stage[w], lane, f, cnt);
 }
 
 static int g_nbr_block = 256;

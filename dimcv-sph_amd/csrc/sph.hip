@@ -10,7 +10,7 @@
 namespace dimcv {
 
 #define FLUID_ENTRY()                                                       \
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;        \
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;        \
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;          \
   const int64_t i = st.fluid_list[f];                                      \
   if (i < 0) return;                                                       \
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
 
 // dimcv_sph.py:395-399
 __global__ void k_multiply(dimcv_state st, float* field, float scale, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
                                                   double* __restrict__ partials, const int32_t* __restrict__ done) {
   __shared__ double lds[256];
   if (jacobi_gated(st, done)) return;  // uniform per block
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
     }
   }
   double s = block_tree_sum256(v, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) partials[xcd_block()] = s;
 }
 
 // dimcv_sph.py:421-468
@@ -351,7 +351,7 @@ __global__ void k_bounce(dimcv_state st, dimcv_params p, int32_t bound) {
 
 // dimcv_sph.py:264-271
 __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(256) k_smooth_vis(dimcv_state st, dimcv_params
 // dimcv_sph.py:295-324 with host-fed uniforms u1/u2 per slot.
 __global__ void k_kvn_sample(dimcv_state st, dimcv_params p, const float* __restrict__ u1,
                              const float* __restrict__ u2, int do_bernoulli, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0 || !in_vortex_domain(p, ld3(st.x, i))) return;
@@ -496,7 +496,7 @@ __device__ __forceinline__ float epan(const dimcv_params& p, v3 r) {
 __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_params p, int32_t bound,
                                                         double* __restrict__ partials) {
   __shared__ double lds[256];
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   double contrib = 0.0;
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
@@ -517,12 +517,12 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
     }
   }
   double s = block_tree_sum256(contrib, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) partials[xcd_block()] = s;
 }
 
 // dimcv_sph.py:753-760
 __global__ void k_life_time(dimcv_state st, dimcv_params p, int32_t bound) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0 || !in_vortex_domain(p, ld3(st.x, i))) return;

@@ -22,7 +22,7 @@ namespace dimcv {
 // ---- near field + S_i (one thread per fluid-list entry) --------------------
 __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p, int32_t bound,
                                                  float* __restrict__ near3, float* __restrict__ scale_s) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void vv_finish(dimcv_state& st, const dimcv_params& p
 __global__ void __launch_bounds__(256) k_vv_far_exact(dimcv_state st, dimcv_params p, int32_t bound,
                                                       const float* __restrict__ near3,
                                                       const float* __restrict__ scale_s) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -227,7 +227,7 @@ template <bool KAHAN, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params p, int32_t bound,
                                                    const float4* __restrict__ src, const int32_t* __restrict__ msrc,
                                                    int cap, int splits, FarPartial* __restrict__ part) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -256,7 +256,7 @@ template <bool KAHAN>
 __global__ void k_far_combine(dimcv_state st, dimcv_params p, int32_t bound, const int32_t* __restrict__ msrc, int cap,
                               int splits, const FarPartial* __restrict__ part, const float* __restrict__ near3,
                               const float* __restrict__ scale_s, int mode) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_vv_far_fast(dimcv_state st, dimcv_param
                                                      const float* __restrict__ near3,
                                                      const float* __restrict__ scale_s,
                                                      const float4* __restrict__ src, int cap) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(256) k_direct_f32(dimcv_state st, dimcv_params
                                                     const float* __restrict__ near3,
                                                     const float4* __restrict__ src,
                                                     const int32_t* __restrict__ nsrc) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params
                                                     const float4* __restrict__ src,
                                                     const int32_t* __restrict__ nsrc, double* __restrict__ out,
                                                     double vp_factor_d) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -439,7 +439,7 @@ __device__ void block_local_scan_f64(const double* __restrict__ y, double* __res
 
 // weights: in-domain fluid kden (fluid-list order), 0 otherwise
 __global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, double* __restrict__ w) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound) return;
   double v = 0.0;
   if (f < st.counters[DIMCV_C_FLUID_LIST]) {

@@ -446,8 +446,10 @@ int dimcv_batch_status(const dimcv_state* st, const dimcv_params* p, int32_t* ho
 int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
                         int32_t* out_iters, void* stream);
 
-/* Launch shapes of the batched far field (result-invariant; chosen by the
- * autotuner of the Python layer).  workgroups = 0 restores the default. */
+/* Launch shapes of the batched far field and neighbour build
+ * (result-invariant; chosen by the autotuner of the Python layer):
+ * far_workgroups in [0, 65535] (0 = default 2048), nbr_block in {0 (= 256),
+ * 64, 128, 256}. */
 int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block);
 
 /* Step graphs: the batched step (grid build, neighbour lists, substep)
