@@ -25,7 +25,8 @@ namespace dimcv {
 
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        const int32_t* done, hipStream_t s);
-int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
+                 double* partials = nullptr);
 int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,
                           double* partials, const int32_t* done, hipStream_t s);
 int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,
@@ -296,8 +297,7 @@ static int seg_grid(BatchCtx& c, hipStream_t s) {
 static int seg_div_begin(BatchCtx& c, hipStream_t s) {
   const dimcv_state* st = c.st;
   const int32_t nb = (int32_t)c.p->n_max;
-  CHECK(launch_fused(0, st, c.p, nb, s));
-  CHECK(dimcv_compute_density_change(st, c.p, nb, s));
+  CHECK(launch_fused(3, st, c.p, nb, s));
   CHECK(dimcv_multiply_time_step(st, c.p, st->dfsph_factor, c.o->inv_dt, nb, s));
   hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
@@ -330,9 +330,8 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, c.o->dt, nb, s));
   CHECK(dimcv_compute_non_pressure_forces(st, p, nb, nb, s));
   CHECK(launch_fused(1, st, p, nb, s));
-  CHECK(launch_fused(2, st, p, nb, s));
   CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, nb, s));
-  CHECK(launch_block_partials(1, st, p, 0.0f, nb, c.ws.partials, nullptr, s));
+  CHECK(launch_fused(4, st, p, nb, s, c.ws.partials));
   hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
                      (int)DIMCV_SD_KDEN_SUM, (const int32_t*)nullptr);
   DIMCV_LAUNCHED("batched kden");

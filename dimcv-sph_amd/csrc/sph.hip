@@ -59,17 +59,23 @@ __global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params 
 
 // compute_densities (dimcv_sph.py:126-148) and compute_DFSPH_factor (:86-124)
 // in one neighbour loop: both read only geometry and masses, and each sum is
-// accumulated exactly as in its own kernel.
+// accumulated exactly as in its own kernel.  CHANGE also accumulates the
+// first compute_density_change of the divergence solve (:337-374), which
+// reads only x and v and the same kernel gradients.
+template <bool CHANGE>
 __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_params p, int32_t bound) {
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i);
+  const v3 vi = CHANGE ? ld3(st.v, i) : mk(0.0f, 0.0f, 0.0f);
   float dens = st.m_V[i] * cubic_W(p, 0.0f);
   float den = 0.0f;
   float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+  float dadv = 0.0f;
   FOR_NBR(k, e, j) {
     v3 r = rel(p, xi, ld3(st.x, j));
     den += st.m_V[j] * cubic_W(p, vnorm(r));
     v3 gw = cubic_dW(p, r);
+    if (CHANGE) dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), gw);
     if (nb_fluid(e)) {
       v3 g = smul(st.m[j], gw);
       r3 += vdot(g, g);
@@ -92,6 +98,11 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
   float factor = 0.0f;
   if (sum > 1e-6f) factor = 1.0f / sum;
   st.dfsph_factor[i] = factor;
+  if (CHANGE) {
+    float d = fmaxf(dadv, 0.0f);
+    if (st.nbr_count[f] < 20) d = 0.0f;
+    st.density_adv[i] = d;
+  }
 }
 
 // dimcv_sph.py:86-124
@@ -492,6 +503,50 @@ __device__ __forceinline__ float epan(const dimcv_params& p, v3 r) {
   return ret;
 }
 
+// k_smooth_vis and compute_kernel_density (dimcv_sph.py:721-751) in one
+// neighbour loop, with the block's canonical FP64 partial of kden_sum.  KVN
+// must be current (it depends only on grad_v, so computing it before the
+// smoothing changes no value).
+__global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_params p, int32_t bound,
+                                                         double* __restrict__ partials) {
+  __shared__ double lds[256];
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  const int64_t i = (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) ? st.fluid_list[f] : -1;
+  if (i >= 0) {
+    const int nn = min(st.nbr_count[f], p.nbr_kmax);
+    const uint32_t* __restrict__ nl = st.nbr + f;
+    const int64_t ns = p.n_max;
+    const v3 xi = ld3(st.x, i);
+    const bool dom = in_vortex_domain(p, xi);
+    const float w0 = cubic_W(p, 0.0f);
+    v3 acc = vmuls(smul(st.m_V[i], ld3(st.d_vort, i)), w0);
+    float w = st.m_V[i] * w0;
+    v3 vort = smul(w, ld3(st.vorticity, i));
+    float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
+    FOR_NBR(k, e, j) {
+      const v3 r = rel(p, xi, ld3(st.x, j));
+      float Wr = cubic_W(p, vnorm(r));
+      float mv = st.m_V[j];
+      if (dom && nb_fluid(e)) {
+        acc = vadd(acc, vmuls(smul(mv, ld3(st.d_vort, j)), Wr));
+        kden += st.kinematic_vort_num[j] * epan(p, r);
+      }
+      float wj = mv * Wr;
+      vort = vadd(vort, smul(wj, ld3(st.vorticity, j)));
+      w += wj;
+    }
+    if (dom) {
+      st3(st.d_vort_smoothed, i, acc);
+      st.kernel_density[i] = kden;
+      contrib = (double)kden;
+    }
+    st3(st.vorticity_vis, i, vdivs(vort, w + 1e-6f));
+  }
+  const double sum = block_tree_sum256(contrib, lds);
+  if (threadIdx.x == 0) partials[xcd_block()] = sum;
+}
+
 // dimcv_sph.py:721-751; kden_sum partials in canonical tree order.
 __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_params p, int32_t bound,
                                                         double* __restrict__ partials) {
@@ -547,20 +602,31 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
   return DIMCV_OK;
 }
 
-int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s) {
+// which: 0 density + factor; 1 v*, vorticity, grad_v, d_vort (zeroing the
+// non-fluid slots first); 2 smoothing + vorticity_vis; 3 density + factor +
+// the first density change; 4 smoothing + vorticity_vis + kernel density,
+// with kden_sum block partials into `partials` (KVN computed before).
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
+                 double* partials) {
   if (which == 1) {  // v*, vorticity, grad_v, d_vort: zero the non-fluid slots first
     int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
     if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
     if (rc) return rc;
-  } else if (which == 2) {
+  } else if (which == 2 || which == 4) {
     int rc = zero_nonfluid(st, p, s, st->vorticity_vis, 3);
     if (rc) return rc;
+  }
+  if (which == 4 && !partials) {
+    set_error("launch_fused: kden partials required");
+    return DIMCV_ERR_ARG;
   }
   if (fluid_bound <= 0) return DIMCV_OK;
   dim3 g(nblocks(fluid_bound)), b(256);
   switch (which) {
-    case 0: hipLaunchKernelGGL(k_density_factor, g, b, 0, s, *st, *p, fluid_bound); break;
+    case 0: hipLaunchKernelGGL(k_density_factor<false>, g, b, 0, s, *st, *p, fluid_bound); break;
     case 1: hipLaunchKernelGGL(k_vort_gradv<7>, g, b, 0, s, *st, *p, fluid_bound); break;
+    case 3: hipLaunchKernelGGL(k_density_factor<true>, g, b, 0, s, *st, *p, fluid_bound); break;
+    case 4: hipLaunchKernelGGL(k_smooth_vis_kden, g, b, 0, s, *st, *p, fluid_bound, partials); break;
     default: hipLaunchKernelGGL(k_smooth_vis, g, b, 0, s, *st, *p, fluid_bound); break;
   }
   DIMCV_LAUNCHED("fused pass");

@@ -17,7 +17,8 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, int32_t fluid_bound,
                     double* partials, const int32_t* done, hipStream_t s);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
-int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
+int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
+                 double* partials = nullptr);
 
 // Events for the optional far-field timing (created on first use, kept for
 // the life of the process).
@@ -98,28 +99,25 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
   Workspace ws;
   CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
 
-  // compute_densities + compute_DFSPH_factor in one pass
-  CHECK(launch_fused(0, st, p, fluid_bound, s));
-  // divergence_solve (dimcv_sph.py:376-393)
-  CHECK(dimcv_compute_density_change(st, p, fluid_bound, stream));
+  // compute_densities + compute_DFSPH_factor + the first
+  // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
+  CHECK(launch_fused(3, st, p, fluid_bound, s));
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->inv_dt, fluid_bound, stream));
   CHECK(jacobi(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
   CHECK(dimcv_compute_non_pressure_forces(st, p, particle_bound, fluid_bound, stream));
   // dimcv() (dimcv_sph.py:812-819)
-  // init() + compute_d_vorticity in one pass, then the smoothing pass, which
-  // also produces vorticity_vis (compute_vorticity_vis below in the reference)
+  // init() + compute_d_vorticity in one pass; then the KVN (and Bernoulli
+  // sample update), which reads only grad_v; then the smoothing pass, which
+  // also produces vorticity_vis (compute_vorticity_vis below in the
+  // reference) and the kernel density with its kden_sum partials.
   CHECK(launch_fused(1, st, p, fluid_bound, s));
-  CHECK(launch_fused(2, st, p, fluid_bound, s));
-  if (o->m > 0) {
-    CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
-    CHECK(dimcv_compute_kernel_density(st, p, fluid_bound, stream));
-    CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
-  } else {
-    CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
-    CHECK(dimcv_record_sample_idx(st, p, particle_bound, stream));
-    CHECK(dimcv_compute_kernel_density(st, p, fluid_bound, stream));
-  }
+  if (o->m > 0) CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
+  else CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
+  CHECK(launch_fused(4, st, p, fluid_bound, s, ws.partials));
+  CHECK(reduce_partials_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_KDEN_SUM, stream, nullptr));
+  if (o->m > 0) CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
+  else CHECK(dimcv_record_sample_idx(st, p, particle_bound, stream));
   if (o->time_far) {
     if (!g_step_ev[0]) {
       DIMCV_HIP(hipEventCreate(&g_step_ev[0]));
