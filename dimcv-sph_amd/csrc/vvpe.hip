@@ -177,6 +177,9 @@ __device__ __forceinline__ void far_exact_sample(v3 xi, float4 a, float4 c, floa
 
 // Sum over sources [j0, j1).  The next pair's sample data are loaded before
 // the current pair is evaluated (software pipelining of the scalar loads).
+// Both paths only produce the two reciprocals (0 for a rejected sample);
+// the accumulation after them is shared, so the accumulators stay in place
+// (no register shuffling between the paths).
 template <bool KAHAN, bool PERIODIC>
 __device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, int j0, int j1, v3 xi, float T, v3 L,
                                               float* A, float* C, int& rej) {
@@ -194,14 +197,24 @@ __device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, in
       far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
       const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
       const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+      float inv0, inv1;
+      int nrej = 0;
       if (__builtin_expect(__any(fminf(r20, r21) <= T_hi), 0)) {
-        far_exact_sample<KAHAN, PERIODIC>(xi, a0, c0, T, L, A, C, rej);
-        far_exact_sample<KAHAN, PERIODIC>(xi, a1, c1, T, L, A, C, rej);
+        // reference order r^2 and the exact acceptance test, per sample
+        const float e0 = __fadd_rn(__fadd_rn(__fmul_rn(dx0, dx0), __fmul_rn(dy0, dy0)), __fmul_rn(dz0, dz0));
+        const float e1 = __fadd_rn(__fadd_rn(__fmul_rn(dx1, dx1), __fmul_rn(dy1, dy1)), __fmul_rn(dz1, dz1));
+        const bool ok0 = e0 > T, ok1 = e1 > T;
+        inv0 = ok0 ? __builtin_amdgcn_rcpf(e0) : 0.0f;
+        inv1 = ok1 ? __builtin_amdgcn_rcpf(e1) : 0.0f;
+        nrej = (ok0 ? 0 : 1) + (ok1 ? 0 : 1);
       } else {
         const float q = __builtin_amdgcn_rcpf(r20 * r21);
-        far_accumulate<KAHAN>(dx0, dy0, r21 * q, c0, A, C);
-        far_accumulate<KAHAN>(dx1, dy1, r20 * q, c1, A, C);
+        inv0 = r21 * q;
+        inv1 = r20 * q;
       }
+      far_accumulate<KAHAN>(dx0, dy0, inv0, c0, A, C);
+      far_accumulate<KAHAN>(dx1, dy1, inv1, c1, A, C);
+      rej += nrej;
       a0 = na0; c0 = nc0; a1 = na1; c1 = nc1;
     }
   }
