@@ -100,10 +100,9 @@ __device__ __forceinline__ float a_kernel(const dimcv_params& p, float s, float&
   const int64_t ns = p.n_max;                                          \
   const v3 xi = ld3(st.x, i)
 
-#define FOR_FLUID_NBR(k, j)                                  \
-  for (int k = 0; k < nn; ++k)                               \
-    if (const uint32_t e_ = nl[(int64_t)k * ns]; nb_fluid(e_)) \
-      if (const int64_t j = nb_slot(e_); true)
+#define FOR_FLUID_NBR(k, j) \
+  FOR_NBR(k, e_##k, j)         \
+  if (nb_fluid(e_##k))
 
 // rho_i = rho0 (mV_i W(0) + sum_j mV_j W(|r_ij|))            (dimcv_sph.py:126-148)
 // in: g_in = rho_bar.  out: x_bar.

@@ -205,6 +205,39 @@ __device__ __forceinline__ float transit_weight(const dimcv_params& p, float xpo
 __device__ __forceinline__ int nb_slot(uint32_t e) { return (int)(e & 0x7fffffffu); }
 __device__ __forceinline__ bool nb_fluid(uint32_t e) { return (e >> 31) != 0u; }
 
+// Register queue of the next NBR_Q list entries, loaded NBR_Q entries ahead
+// of use: a group of four neighbours then waits only for its neighbours'
+// data, not first for its own list entries (entries past nn read as 0 and
+// are never used).
+#define NBR_Q 16
+struct NbrQueue {
+  uint32_t q[NBR_Q];
+  __device__ __forceinline__ NbrQueue(const uint32_t* __restrict__ nl, int64_t ns, int nn) {
+#pragma unroll
+    for (int u = 0; u < NBR_Q; ++u) q[u] = u < nn ? nl[(int64_t)u * ns] : 0u;
+  }
+  __device__ __forceinline__ void advance4(const uint32_t* __restrict__ nl, int64_t ns, int nn, int k) {
+#pragma unroll
+    for (int u = 0; u < NBR_Q - 4; ++u) q[u] = q[u + 4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k + NBR_Q + u;
+      q[NBR_Q - 4 + u] = kk < nn ? nl[(int64_t)kk * ns] : 0u;
+    }
+  }
+};
+
+// With nl = st.nbr + f, ns = n_max and nn = the list length in scope:
+// for k in [0, nn): e = list entry k, j = its slot, in list order (groups
+// of four over the queue; `continue` moves to the next neighbour).
+#define FOR_NBR(k, e, j)                                                        \
+  if (NbrQueue Q_##k(nl, ns, nn); true)                                         \
+    for (int k##0 = 0; k##0 < nn; Q_##k.advance4(nl, ns, nn, k##0), k##0 += 4)  \
+      _Pragma("unroll") for (int k = k##0; k < k##0 + 4; ++k)                   \
+        if (k < nn)                                                              \
+          if (const uint32_t e = Q_##k.q[k - k##0]; true)                        \
+            if (const int64_t j = nb_slot(e); true)
+
 // Relative position x_i - x_j; with a periodic box the minimum image.
 __device__ __forceinline__ v3 rel(const dimcv_params& p, v3 xi, v3 xj) {
   v3 r = vsub(xi, xj);

@@ -18,10 +18,6 @@ namespace dimcv {
   const uint32_t* __restrict__ nl = st.nbr + f;                            \
   const int64_t ns = p.n_max
 
-#define FOR_NBR(k, e, j)                       \
-  _Pragma("unroll 4") for (int k = 0; k < nn; ++k) \
-    if (const uint32_t e = nl[(int64_t)k * ns]; true) \
-      if (const int64_t j = nb_slot(e); true)
 
 // Zero-fill of slots that a "field.fill(0) + fluid writes" kernel leaves at
 // zero: every slot >= particle_num or not fluid.
@@ -561,10 +557,8 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
       const uint32_t* __restrict__ nl = st.nbr + f;
       const int64_t ns = p.n_max;
       float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
-      for (int k = 0; k < nn; ++k) {
-        uint32_t e = nl[(int64_t)k * ns];
+      FOR_NBR(k, e, j) {
         if (!nb_fluid(e)) continue;
-        int64_t j = nb_slot(e);
         kden += st.kinematic_vort_num[j] * epan(p, rel(p, xi, ld3(st.x, j)));
       }
       st.kernel_density[i] = kden;

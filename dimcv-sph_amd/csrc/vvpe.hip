@@ -33,10 +33,8 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   const int64_t ns = p.n_max;
   v3 vv = mk(0.0f, 0.0f, 0.0f);
   float kns = st.kernel_density[i];
-  for (int k = 0; k < nn; ++k) {
-    uint32_t e = nl[(int64_t)k * ns];
+  FOR_NBR(k, e, j) {
     if (!nb_fluid(e)) continue;
-    int64_t j = nb_slot(e);
     v3 r2 = rel(p, xi, ld3(st.x, j));
     v3 r = mk(r2.x, r2.y, 0.0f);
     float rn = vnorm(r);
