@@ -43,7 +43,11 @@ for _p in (PKG, REPO):
 
 METRIC = "particle-steps/s at 1M, MC-VVPE vs direct Biot-Savart at fixed L2 vel err"
 FP32_PEAK_TFLOPS = 157.3
-FP32_ISSUE_TFLOPS = 78.6
+# Issue ceiling of the far-field kernel as built: 16 vector instructions per
+# (target, sample) pair (gfx950 ISA of the pair loop), at most two vector
+# instructions per SIMD quad-cycle (dual issue), 1024 SIMDs at 2.4 GHz.
+VALU_PER_PAIR = 16
+FP32_ISSUE_TFLOPS = round(1024 * 2.4e9 / 4 * 2 * 64 / VALU_PER_PAIR * 24 / 1e12, 1)
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_MC = 24
 FLOPS_PER_PAIR_DIRECT = 22
@@ -60,7 +64,7 @@ def run_gpu(args):
 
     t0 = time.time()
     ps, solver, info = scenes.build_ring_system(128, 128, 64, n_rings=8, seed=2, mc_samples=4096,
-                                                far_field="fast")
+                                                far_field=args.far_field)
     torch.cuda.synchronize()
     n = ps.particle_num[None]
     log(f"1M scene ready in {time.time() - t0:.1f}s: {n} particles, grid {list(ps.grid_num)}")
@@ -328,6 +332,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the timed steps, run a 256 MiB elementwise copy (PMC scale reference)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--far-field", default="fast", choices=["fast", "kahan"],
+                    help="far-field evaluation of the timed 1M step")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -365,7 +371,7 @@ def main():
         "data": "synthetic: seeded jittered lattice + Lamb-Oseen vortex rings",
         "config": {"workload": "1M-particle vortex-ring lattice, MC-VVPE M=4096",
                    "n_particles": res["n_fluid"], "lattice": [128, 128, 64], "rings": 8,
-                   "mc_samples": res["m"], "dt": 2e-4, "far_field": "fast",
+                   "mc_samples": res["m"], "dt": 2e-4, "far_field": args.far_field,
                    "mean_k_v": res["kv"], "mean_k_p": res["kp"], "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": "k_vv_far_fast (MC-VVPE far field)",
                      "achieved": round(res["achieved_tflops"], 3), "peak": FP32_PEAK_TFLOPS,
@@ -374,10 +380,12 @@ def main():
                      "work_per_launch": f"{res['pairs']} (target, sample) pairs x {FLOPS_PER_PAIR_MC} flop",
                      "issue_ceiling": FP32_ISSUE_TFLOPS,
                      "frac_of_issue_ceiling": round(res["achieved_tflops"] / FP32_ISSUE_TFLOPS, 4),
-                     "note": "FP32 vector-ALU bound (no MFMA). peak = MI355X FP32 157.3 TFLOP/s, which "
-                             "counts packed FP32; this build compiles with packed FP32 disabled (-packed-fp32-ops, "
-                             "a correctness hazard of this GPU pool), so one FMA per lane per cycle, 78.6 "
-                             "TFLOP/s, is the issue ceiling of the kernel as built"},
+                     "note": "FP32 vector-ALU bound. peak = MI355X FP32 157.3 TFLOP/s. Built without packed "
+                             "FP32 (-packed-fp32-ops); the pair loop issues 16 vector instructions per pair and "
+                             "the SIMDs dual-issue part of them (SQ_ACTIVE_INST_VALU2), so the issue ceiling is 2 "
+                             "instructions per SIMD quad-cycle: issue_ceiling TFLOP/s at 24 flop per pair. A "
+                             "matrix-core form (S = W Q on v_mfma_f32_16x16x4_f32) measured slower: no VALU/MFMA "
+                             "co-execution (SQ_VALU_MFMA_COEXEC_CYCLES = 0) and 7 of 16 columns used"},
         "cpu_baseline": cpu,
         "mc_vs_direct": direct,
         "batching": batching,
