@@ -173,49 +173,62 @@ __device__ __forceinline__ void far_exact_sample(v3 xi, float4 a, float4 c, floa
   far_accumulate<KAHAN>(dx, dy, inv, c, A, C);
 }
 
-// Sum over sources [j0, j1).  The next pair's sample data are loaded before
-// the current pair is evaluated (software pipelining of the scalar loads).
-// Both paths only produce the two reciprocals (0 for a rejected sample);
+// One pair of samples.  Both paths only produce the two reciprocals (0 for a
+// rejected sample);
 // the accumulation after them is shared, so the accumulators stay in place
 // (no register shuffling between the paths).
+template <bool KAHAN, bool PERIODIC>
+__device__ __forceinline__ void far_pair(float4 a0, float4 c0, float4 a1, float4 c1, v3 xi, float T, float T_hi,
+                                         v3 L, float* A, float* C, int& rej) {
+  float dx0, dy0, dz0, dx1, dy1, dz1;
+  far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
+  far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
+  const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+  const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+  float inv0, inv1;
+  int nrej = 0;
+  if (__builtin_expect(__any(fminf(r20, r21) <= T_hi), 0)) {
+    // reference order r^2 and the exact acceptance test, per sample
+    const float e0 = __fadd_rn(__fadd_rn(__fmul_rn(dx0, dx0), __fmul_rn(dy0, dy0)), __fmul_rn(dz0, dz0));
+    const float e1 = __fadd_rn(__fadd_rn(__fmul_rn(dx1, dx1), __fmul_rn(dy1, dy1)), __fmul_rn(dz1, dz1));
+    const bool ok0 = e0 > T, ok1 = e1 > T;
+    inv0 = ok0 ? __builtin_amdgcn_rcpf(e0) : 0.0f;
+    inv1 = ok1 ? __builtin_amdgcn_rcpf(e1) : 0.0f;
+    nrej = (ok0 ? 0 : 1) + (ok1 ? 0 : 1);
+  } else {
+    const float q = __builtin_amdgcn_rcpf(r20 * r21);
+    inv0 = r21 * q;
+    inv1 = r20 * q;
+  }
+  far_accumulate<KAHAN>(dx0, dy0, inv0, c0, A, C);
+  far_accumulate<KAHAN>(dx1, dy1, inv1, c1, A, C);
+  rej += nrej;
+}
+
+// Sum over sources [j0, j1) in pairs (each pair decides its own path), four
+// samples per trip: the next four samples' data are loaded (scalar loads)
+// before the current four are evaluated.
 template <bool KAHAN, bool PERIODIC>
 __device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, int j0, int j1, v3 xi, float T, v3 L,
                                               float* A, float* C, int& rej) {
   const float T_hi = T * (1.0f + 9.5367431640625e-07f);
   int js = j0;
-  if (js + 1 < j1) {
-    float4 a0 = src[2 * js], c0 = src[2 * js + 1];
-    float4 a1 = src[2 * js + 2], c1 = src[2 * js + 3];
-    for (; js + 1 < j1; js += 2) {
-      const int nx = js + 3 < j1 ? js + 2 : js;  // in-range prefetch index
-      const float4 na0 = src[2 * nx], nc0 = src[2 * nx + 1];
-      const float4 na1 = src[2 * nx + 2], nc1 = src[2 * nx + 3];
-      float dx0, dy0, dz0, dx1, dy1, dz1;
-      far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
-      far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
-      const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
-      const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-      float inv0, inv1;
-      int nrej = 0;
-      if (__builtin_expect(__any(fminf(r20, r21) <= T_hi), 0)) {
-        // reference order r^2 and the exact acceptance test, per sample
-        const float e0 = __fadd_rn(__fadd_rn(__fmul_rn(dx0, dx0), __fmul_rn(dy0, dy0)), __fmul_rn(dz0, dz0));
-        const float e1 = __fadd_rn(__fadd_rn(__fmul_rn(dx1, dx1), __fmul_rn(dy1, dy1)), __fmul_rn(dz1, dz1));
-        const bool ok0 = e0 > T, ok1 = e1 > T;
-        inv0 = ok0 ? __builtin_amdgcn_rcpf(e0) : 0.0f;
-        inv1 = ok1 ? __builtin_amdgcn_rcpf(e1) : 0.0f;
-        nrej = (ok0 ? 0 : 1) + (ok1 ? 0 : 1);
-      } else {
-        const float q = __builtin_amdgcn_rcpf(r20 * r21);
-        inv0 = r21 * q;
-        inv1 = r20 * q;
-      }
-      far_accumulate<KAHAN>(dx0, dy0, inv0, c0, A, C);
-      far_accumulate<KAHAN>(dx1, dy1, inv1, c1, A, C);
-      rej += nrej;
+  if (js + 3 < j1) {
+    float4 a0 = src[2 * js], c0 = src[2 * js + 1], a1 = src[2 * js + 2], c1 = src[2 * js + 3];
+    float4 a2 = src[2 * js + 4], c2 = src[2 * js + 5], a3 = src[2 * js + 6], c3 = src[2 * js + 7];
+    for (; js + 3 < j1; js += 4) {
+      const int nx = js + 7 < j1 ? js + 4 : js;  // in-range prefetch index
+      const float4 na0 = src[2 * nx], nc0 = src[2 * nx + 1], na1 = src[2 * nx + 2], nc1 = src[2 * nx + 3];
+      const float4 na2 = src[2 * nx + 4], nc2 = src[2 * nx + 5], na3 = src[2 * nx + 6], nc3 = src[2 * nx + 7];
+      far_pair<KAHAN, PERIODIC>(a0, c0, a1, c1, xi, T, T_hi, L, A, C, rej);
+      far_pair<KAHAN, PERIODIC>(a2, c2, a3, c3, xi, T, T_hi, L, A, C, rej);
       a0 = na0; c0 = nc0; a1 = na1; c1 = nc1;
+      a2 = na2; c2 = nc2; a3 = na3; c3 = nc3;
     }
   }
+  for (; js + 1 < j1; js += 2)
+    far_pair<KAHAN, PERIODIC>(src[2 * js], src[2 * js + 1], src[2 * js + 2], src[2 * js + 3], xi, T, T_hi, L, A, C,
+                              rej);
   if (js < j1) far_exact_sample<KAHAN, PERIODIC>(xi, src[2 * js], src[2 * js + 1], T, L, A, C, rej);
 }
 
