@@ -364,9 +364,36 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
 // keeps cell 0 empty exactly as the per-cell rule does.  The distance test
 // sqrt_rn(r.r) < h is evaluated as r.r < nbr_r2_limit, which is equivalent
 // for every f32 r.r (sqrt is monotone and correctly rounded).
+// Candidates [beg, end) of one run, SCAN_U at a time: their positions and
+// material flags are loaded (indices clamped into the run) before any is
+// tested, so a lane waits for one memory latency per SCAN_U candidates
+// rather than per candidate.  f(j, is_fluid) is called in slot order.
+#define SCAN_U 8
+template <class F>
+__device__ __forceinline__ void scan_run(const float* __restrict__ x, const int32_t* __restrict__ material,
+                                         int beg, int end, int64_t i, v3 xi, float lim, F&& f) {
+  for (int j = beg; j < end; j += SCAN_U) {
+    v3 xj[SCAN_U];
+    int32_t mt[SCAN_U];
+#pragma unroll
+    for (int u = 0; u < SCAN_U; ++u) {
+      const int jj = min(j + u, end - 1);
+      xj[u] = ld3(x, jj);
+      mt[u] = material[jj];
+    }
+#pragma unroll
+    for (int u = 0; u < SCAN_U; ++u) {
+      const v3 r = vsub(xi, xj[u]);
+      const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+      if (j + u < end && r2 < lim && j + u != i) f(j + u, mt[u] == 1);
+    }
+  }
+}
+
 template <class F>
 __device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __restrict__ x,
-                                          const int32_t* __restrict__ gpn, int64_t i, v3 xi, F&& f) {
+                                          const int32_t* __restrict__ material, const int32_t* __restrict__ gpn,
+                                          int64_t i, v3 xi, F&& f) {
   int cc[3];
   cell_of(p, xi, cc);
   const int nx = p.grid_num[0], ny = p.grid_num[1], nz = p.grid_num[2];
@@ -381,11 +408,7 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __
       int fl = (g0 * ny + g1) * nz;
       int beg = gpn[max(fl + zlo - 1, 0)];
       int end = gpn[fl + zhi];
-      for (int j = beg; j < end; ++j) {
-        v3 r = vsub(xi, ld3(x, j));
-        float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-        if (r2 < lim && j != i) f(j);
-      }
+      scan_run(x, material, beg, end, i, xi, lim, f);
     }
   }
 }
@@ -419,14 +442,14 @@ __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params 
     const v3 xi = ld3(st.x, i);
     const int kmax = p.nbr_kmax;
     const int64_t stride = p.n_max;
-    auto emit = [&](int j) {
-      const uint32_t e = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
+    auto emit = [&](int j, bool fluid) {
+      const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
       if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
       else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
       ++cnt;
     };
-    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, emit);
-    else scan_runs(p, st.x, st.grid_particles_num, i, xi, emit);
+    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) { emit(j, st.material[j] == 1); });
+    else scan_runs(p, st.x, st.material, st.grid_particles_num, i, xi, emit);
     st.nbr_count[f] = cnt;
     atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
   }
@@ -560,16 +583,12 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
         const int lo = fl + zlo - 1;
         const int beg = lo >= 0 ? gpn[lo] : gpn[0];
         const int end = gpn[fl + zhi];
-        for (int j = beg; j < end; ++j) {
-          const v3 r = vsub(xi, ld3(st.x, j));
-          const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-          if (r2 < lim && j != i) {
-            const uint32_t e = (uint32_t)j | (st.material[j] == 1 ? 0x80000000u : 0u);
-            if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
-            else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
-            ++cnt;
-          }
-        }
+        scan_run(st.x, st.material, beg, end, i, xi, lim, [&](int j, bool fluid) {
+          const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
+          if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
+          else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
+          ++cnt;
+        });
       }
     }
     st.nbr_count[f] = cnt;
