@@ -238,6 +238,25 @@ struct NbrQueue {
           if (const uint32_t e = Q_##k.q[k - k##0]; true)                        \
             if (const int64_t j = nb_slot(e); true)
 
+// Neighbour loop in two phases per group of four list entries: gather(j)
+// returns the neighbour's fields (a struct), called for all four entries
+// before body(k, e, fields) runs for the live ones, so the group's loads
+// are in flight together (one memory latency per four neighbours).  Entries
+// past nn read slot 0 (a valid slot) and are never passed to body.
+template <class Gather, class Body>
+__device__ __forceinline__ void nbr_loop(const uint32_t* __restrict__ nl, int64_t ns, int nn, Gather&& gather,
+                                         Body&& body) {
+  NbrQueue Q(nl, ns, nn);
+  for (int k0 = 0; k0 < nn; Q.advance4(nl, ns, nn, k0), k0 += 4) {
+    decltype(gather((int64_t)0)) g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g[u] = gather((int64_t)nb_slot(Q.q[u]));
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < nn) body(k0 + u, Q.q[u], g[u]);
+  }
+}
+
 // Relative position x_i - x_j; with a periodic box the minimum image.
 __device__ __forceinline__ v3 rel(const dimcv_params& p, v3 xi, v3 xj) {
   v3 r = vsub(xi, xj);

@@ -67,24 +67,27 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
   float den = 0.0f;
   float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
   float dadv = 0.0f;
-  FOR_NBR(k, e, j) {
-    v3 r = rel(p, xi, ld3(st.x, j));
-    den += st.m_V[j] * cubic_W(p, vnorm(r));
-    v3 gw = cubic_dW(p, r);
-    if (CHANGE) dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), gw);
-    if (nb_fluid(e)) {
-      v3 g = smul(st.m[j], gw);
-      r3 += vdot(g, g);
-      r0 += g.x;
-      r1 += g.y;
-      r2 += g.z;
-    } else {
-      v3 g = smul(st.m_V[j] * p.density0, gw);
-      r0 += g.x;
-      r1 += g.y;
-      r2 += g.z;
-    }
-  }
+  struct G { v3 x, v; float m, mV; };
+  nbr_loop(nl, ns, nn,
+           [&](int64_t j) { return G{ld3(st.x, j), CHANGE ? ld3(st.v, j) : mk(0.0f, 0.0f, 0.0f), st.m[j], st.m_V[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             v3 r = rel(p, xi, g.x);
+             den += g.mV * cubic_W(p, vnorm(r));
+             v3 gw = cubic_dW(p, r);
+             if (CHANGE) dadv += g.mV * vdot(vsub(vi, g.v), gw);
+             if (nb_fluid(e)) {
+               v3 gg = smul(g.m, gw);
+               r3 += vdot(gg, gg);
+               r0 += gg.x;
+               r1 += gg.y;
+               r2 += gg.z;
+             } else {
+               v3 gg = smul(g.mV * p.density0, gw);
+               r0 += gg.x;
+               r1 += gg.y;
+               r2 += gg.z;
+             }
+           });
   dens += den;
   dens *= p.density0;
   st.density[i] = dens;
@@ -136,9 +139,9 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float dadv = 0.0f;
-  FOR_NBR(k, e, j) {
-    dadv += st.m_V[j] * vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-  }
+  struct G { v3 x, v; float mV; };
+  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+           [&](int, uint32_t, const G& g) { dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x))); });
   float d = fmaxf(dadv, 0.0f);
   if (st.nbr_count[f] < 20) d = 0.0f;
   st.density_adv[i] = d;
@@ -183,6 +186,8 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
   float b_i = st.density_adv[i];
   float k_i = b_i * st.dfsph_factor[i] * st.density[i];
   v3 dv = mk(0.0f, 0.0f, 0.0f);
+  // plain loop: most pairs stop at the |k_i + k_j| test, so positions and
+  // masses are loaded only for the pairs that need them
   FOR_NBR(k, e, j) {
     if (nb_fluid(e)) {
       float b_j = st.density_adv[j];
@@ -213,15 +218,17 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
   const float visc_coef = st.block_scene ? st.scenes[block_scene_of(st)].visc_coef : p.visc_coef;
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
-  FOR_NBR(k, e, j) {
-    if (nb_fluid(e)) {
-      v3 r = rel(p, xi, ld3(st.x, j));
-      float v_xy = vdot(vsub(vi, ld3(st.v, j)), r);
-      float rn = vnorm(r);
-      v3 fv = smul(visc_coef * st.m_V[j] * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
-      dv = vadd(dv, fv);
-    }
-  }
+  struct G { v3 x, v; float mV; };
+  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             if (nb_fluid(e)) {
+               v3 r = rel(p, xi, g.x);
+               float v_xy = vdot(vsub(vi, g.v), r);
+               float rn = vnorm(r);
+               v3 fv = smul(visc_coef * g.mV * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
+               dv = vadd(dv, fv);
+             }
+           });
   st3(st.acceleration, i, dv);
 }
 
@@ -262,11 +269,13 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float delta = 0.0f;
-  FOR_NBR(k, e, j) {
-    float dd = vdot(vsub(vi, ld3(st.v, j)), cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-    if (nb_fluid(e)) delta += st.m[j] * dd;
-    else delta += st.m_V[j] * p.density0 * dd;
-  }
+  struct G { v3 x, v; float m, mV; };
+  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
+             if (nb_fluid(e)) delta += g.m * dd;
+             else delta += g.mV * p.density0 * dd;
+           });
   float da = st.density[i] + p.dt * delta;
   st.density_adv[i] = fmaxf(da, p.density0);
 }
@@ -282,6 +291,7 @@ __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_par
   float k_i = b_i * st.dfsph_factor[i];
   v3 vi = ld3(st.v, i);
   const float w = p.has_transit ? transit_weight(p, xi.x) : 1.0f;
+  // plain loop: most pairs stop at the |k_i + k_j| test (see k_div_iter)
   FOR_NBR(k, e, j) {
     if (nb_fluid(e)) {
       float b_j = st.density_adv[j] - p.density0;
@@ -378,24 +388,30 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
   v3 vsi = mk(0.0f, 0.0f, 0.0f);
   if (MODE & 4) vsi = vadd(vi, vmuls(ld3(st.acceleration, i), p.dt));
   float G[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  FOR_NBR(k, e, j) {
-    if (!nb_fluid(e)) continue;
-    v3 vj = ld3(st.v, j);
-    v3 kd = cubic_dW(p, rel(p, xi, ld3(st.x, j)));
-    v3 vji = vsub(vj, vi);
-    float mv = st.m_V[j];
-    if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
-    if (MODE & 2) {
-      v3 a = smul(mv, vji);
-      G[0] = G[0] + a.x * kd.x; G[1] = G[1] + a.x * kd.y; G[2] = G[2] + a.x * kd.z;
-      G[3] = G[3] + a.y * kd.x; G[4] = G[4] + a.y * kd.y; G[5] = G[5] + a.y * kd.z;
-      G[6] = G[6] + a.z * kd.x; G[7] = G[7] + a.z * kd.y; G[8] = G[8] + a.z * kd.z;
-    }
-    if (MODE & 4) {
-      v3 vsj = vadd(vj, vmuls(ld3(st.acceleration, j), p.dt));
-      curl = vadd(curl, smul(mv, vcross(vsub(vsj, vsi), kd)));
-    }
-  }
+  struct Gt { v3 x, v, a; float mV; };
+  nbr_loop(nl, ns, nn,
+           [&](int64_t j) {
+             return Gt{ld3(st.x, j), ld3(st.v, j), (MODE & 4) ? ld3(st.acceleration, j) : mk(0.0f, 0.0f, 0.0f),
+                       st.m_V[j]};
+           },
+           [&](int, uint32_t e, const Gt& g) {
+             if (!nb_fluid(e)) return;
+             v3 vj = g.v;
+             v3 kd = cubic_dW(p, rel(p, xi, g.x));
+             v3 vji = vsub(vj, vi);
+             float mv = g.mV;
+             if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
+             if (MODE & 2) {
+               v3 a = smul(mv, vji);
+               G[0] = G[0] + a.x * kd.x; G[1] = G[1] + a.x * kd.y; G[2] = G[2] + a.x * kd.z;
+               G[3] = G[3] + a.y * kd.x; G[4] = G[4] + a.y * kd.y; G[5] = G[5] + a.y * kd.z;
+               G[6] = G[6] + a.z * kd.x; G[7] = G[7] + a.z * kd.y; G[8] = G[8] + a.z * kd.z;
+             }
+             if (MODE & 4) {
+               v3 vsj = vadd(vj, vmuls(g.a, p.dt));
+               curl = vadd(curl, smul(mv, vcross(vsub(vsj, vsi), kd)));
+             }
+           });
   if (MODE & 1) st3(st.vorticity, i, vort);
   if (MODE & 2)
     for (int q = 0; q < 9; ++q) st.grad_v[9 * i + q] = G[q];
@@ -520,18 +536,23 @@ __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_p
     float w = st.m_V[i] * w0;
     v3 vort = smul(w, ld3(st.vorticity, i));
     float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
-    FOR_NBR(k, e, j) {
-      const v3 r = rel(p, xi, ld3(st.x, j));
-      float Wr = cubic_W(p, vnorm(r));
-      float mv = st.m_V[j];
-      if (dom && nb_fluid(e)) {
-        acc = vadd(acc, vmuls(smul(mv, ld3(st.d_vort, j)), Wr));
-        kden += st.kinematic_vort_num[j] * epan(p, r);
-      }
-      float wj = mv * Wr;
-      vort = vadd(vort, smul(wj, ld3(st.vorticity, j)));
-      w += wj;
-    }
+    struct G { v3 x, dv, vo; float mV, kvn; };
+    nbr_loop(nl, ns, nn,
+             [&](int64_t j) {
+               return G{ld3(st.x, j), ld3(st.d_vort, j), ld3(st.vorticity, j), st.m_V[j], st.kinematic_vort_num[j]};
+             },
+             [&](int, uint32_t e, const G& g) {
+               const v3 r = rel(p, xi, g.x);
+               float Wr = cubic_W(p, vnorm(r));
+               float mv = g.mV;
+               if (dom && nb_fluid(e)) {
+                 acc = vadd(acc, vmuls(smul(mv, g.dv), Wr));
+                 kden += g.kvn * epan(p, r);
+               }
+               float wj = mv * Wr;
+               vort = vadd(vort, smul(wj, g.vo));
+               w += wj;
+             });
     if (dom) {
       st3(st.d_vort_smoothed, i, acc);
       st.kernel_density[i] = kden;

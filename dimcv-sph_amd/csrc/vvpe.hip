@@ -33,15 +33,18 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   const int64_t ns = p.n_max;
   v3 vv = mk(0.0f, 0.0f, 0.0f);
   float kns = st.kernel_density[i];
-  FOR_NBR(k, e, j) {
-    if (!nb_fluid(e)) continue;
-    v3 r2 = rel(p, xi, ld3(st.x, j));
-    v3 r = mk(r2.x, r2.y, 0.0f);
-    float rn = vnorm(r);
-    float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
-    vv = vadd(vv, smul(A, vcross(r, ld3(st.d_vort_smoothed, j))));
-    kns += st.kernel_density[j];
-  }
+  struct G { v3 x, w; float kd; };
+  nbr_loop(nl, ns, nn,
+           [&](int64_t j) { return G{ld3(st.x, j), ld3(st.d_vort_smoothed, j), st.kernel_density[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             if (!nb_fluid(e)) return;
+             v3 r2 = rel(p, xi, g.x);
+             v3 r = mk(r2.x, r2.y, 0.0f);
+             float rn = vnorm(r);
+             float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
+             vv = vadd(vv, smul(A, vcross(r, g.w)));
+             kns += g.kd;
+           });
   vv = vmuls(vv, p.vp_factor);
   st3(near3, f, vv);
   scale_s[f] = (float)(kden_sum_of(st) - (double)kns);
