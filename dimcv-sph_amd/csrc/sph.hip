@@ -186,25 +186,29 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
   float b_i = st.density_adv[i];
   float k_i = b_i * st.dfsph_factor[i] * st.density[i];
   v3 dv = mk(0.0f, 0.0f, 0.0f);
-  // plain loop: most pairs stop at the |k_i + k_j| test, so positions and
-  // masses are loaded only for the pairs that need them
-  FOR_NBR(k, e, j) {
-    if (nb_fluid(e)) {
-      float b_j = st.density_adv[j];
-      float k_j = b_j * st.dfsph_factor[j] * st.density[j];
-      float k_sum = k_i + k_j;
-      if (fabsf(k_sum) > p.m_eps) {
-        v3 g = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-        dv = vsub(dv, smul(dt * k_sum, g));
-      }
-    } else {
-      if (fabsf(k_i) > p.m_eps) {
-        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-        v3 vc = smul(-dt * 1.0f * k_i, g);
-        dv = vadd(dv, vc);
-      }
-    }
-  }
+  // most pairs stop at the |k_i + k_j| test: only its three scalars are
+  // gathered up front; positions and masses are loaded for the pairs that
+  // pass it
+  struct G { float b, fac, rho; };
+  nbr_loop(nl, ns, nn,
+           [&](int64_t j) { return G{st.density_adv[j], st.dfsph_factor[j], st.density[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             const int64_t j = nb_slot(e);
+             if (nb_fluid(e)) {
+               float k_j = g.b * g.fac * g.rho;
+               float k_sum = k_i + k_j;
+               if (fabsf(k_sum) > p.m_eps) {
+                 v3 gg = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
+                 dv = vsub(dv, smul(dt * k_sum, gg));
+               }
+             } else {
+               if (fabsf(k_i) > p.m_eps) {
+                 v3 gg = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
+                 v3 vc = smul(-dt * 1.0f * k_i, gg);
+                 dv = vadd(dv, vc);
+               }
+             }
+           });
   v3 vi = ld3(st.v, i);
   if (!p.has_transit) vi = vadd(vi, dv);
   else vi = vadd(vi, smul(transit_weight(p, xi.x), dv));
@@ -291,26 +295,29 @@ __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_par
   float k_i = b_i * st.dfsph_factor[i];
   v3 vi = ld3(st.v, i);
   const float w = p.has_transit ? transit_weight(p, xi.x) : 1.0f;
-  // plain loop: most pairs stop at the |k_i + k_j| test (see k_div_iter)
-  FOR_NBR(k, e, j) {
-    if (nb_fluid(e)) {
-      float b_j = st.density_adv[j] - p.density0;
-      float k_j = b_j * st.dfsph_factor[j];
-      float k_sum = k_i + k_j;
-      if (fabsf(k_sum) > p.m_eps) {
-        v3 g = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-        if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, g));
-        else vi = vsub(vi, smul(w * dt * k_sum, g));
-      }
-    } else {
-      if (fabsf(k_i) > p.m_eps) {
-        v3 g = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-        v3 vc = smul(-dt * 1.0f * k_i, g);
-        if (!p.has_transit) vi = vadd(vi, vc);
-        else vi = vadd(vi, smul(w, vc));
-      }
-    }
-  }
+  // most pairs stop at the |k_i + k_j| test (see k_div_iter)
+  struct G { float b, fac; };
+  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{st.density_adv[j], st.dfsph_factor[j]}; },
+           [&](int, uint32_t e, const G& g) {
+             const int64_t j = nb_slot(e);
+             if (nb_fluid(e)) {
+               float b_j = g.b - p.density0;
+               float k_j = b_j * g.fac;
+               float k_sum = k_i + k_j;
+               if (fabsf(k_sum) > p.m_eps) {
+                 v3 gg = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
+                 if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, gg));
+                 else vi = vsub(vi, smul(w * dt * k_sum, gg));
+               }
+             } else {
+               if (fabsf(k_i) > p.m_eps) {
+                 v3 gg = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
+                 v3 vc = smul(-dt * 1.0f * k_i, gg);
+                 if (!p.has_transit) vi = vadd(vi, vc);
+                 else vi = vadd(vi, smul(w, vc));
+               }
+             }
+           });
   st3(st.v, i, vi);
 }
 
