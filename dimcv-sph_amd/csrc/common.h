@@ -217,12 +217,17 @@ struct NbrQueue {
     for (int u = 0; u < NBR_Q; ++u) q[u] = u < nn ? nl[(int64_t)u * ns] : 0u;
   }
   __device__ __forceinline__ void advance4(const uint32_t* __restrict__ nl, int64_t ns, int nn, int k) {
+    advance<4>(nl, ns, nn, k);
+  }
+  // entries [k, k + S) have been used: shift and load entries k + NBR_Q ..
+  template <int S>
+  __device__ __forceinline__ void advance(const uint32_t* __restrict__ nl, int64_t ns, int nn, int k) {
 #pragma unroll
-    for (int u = 0; u < NBR_Q - 4; ++u) q[u] = q[u + 4];
+    for (int u = 0; u < NBR_Q - S; ++u) q[u] = q[u + S];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < S; ++u) {
       const int kk = k + NBR_Q + u;
-      q[NBR_Q - 4 + u] = kk < nn ? nl[(int64_t)kk * ns] : 0u;
+      q[NBR_Q - S + u] = kk < nn ? nl[(int64_t)kk * ns] : 0u;
     }
   }
 };
@@ -238,21 +243,24 @@ struct NbrQueue {
           if (const uint32_t e = Q_##k.q[k - k##0]; true)                        \
             if (const int64_t j = nb_slot(e); true)
 
-// Neighbour loop in two phases per group of four list entries: gather(j)
-// returns the neighbour's fields (a struct), called for all four entries
+// Neighbour loop in two phases per group of GS list entries: gather(j)
+// returns the neighbour's fields (a struct), called for all GS entries
 // before body(k, e, fields) runs for the live ones, so the group's loads
-// are in flight together (one memory latency per four neighbours).  Entries
-// past nn read slot 0 (a valid slot) and are never passed to body.
-template <class Gather, class Body>
+// are in flight together (one memory latency per GS neighbours).  Entries
+// past nn read slot 0 (a valid slot) and are never passed to body.  GS = 8
+// suits passes with few gathered fields; passes with many keep 4 (register
+// pressure).
+template <int GS = 4, class Gather, class Body>
 __device__ __forceinline__ void nbr_loop(const uint32_t* __restrict__ nl, int64_t ns, int nn, Gather&& gather,
                                          Body&& body) {
+  static_assert(GS <= NBR_Q, "group larger than the list queue");
   NbrQueue Q(nl, ns, nn);
-  for (int k0 = 0; k0 < nn; Q.advance4(nl, ns, nn, k0), k0 += 4) {
-    decltype(gather((int64_t)0)) g[4];
+  for (int k0 = 0; k0 < nn; Q.template advance<GS>(nl, ns, nn, k0), k0 += GS) {
+    decltype(gather((int64_t)0)) g[GS];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) g[u] = gather((int64_t)nb_slot(Q.q[u]));
+    for (int u = 0; u < GS; ++u) g[u] = gather((int64_t)nb_slot(Q.q[u]));
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < GS; ++u)
       if (k0 + u < nn) body(k0 + u, Q.q[u], g[u]);
   }
 }

@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float dadv = 0.0f;
   struct G { v3 x, v; float mV; };
-  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+  nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
            [&](int, uint32_t, const G& g) { dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x))); });
   float d = fmaxf(dadv, 0.0f);
   if (st.nbr_count[f] < 20) d = 0.0f;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   float delta = 0.0f;
   struct G { v3 x, v; float m, mV; };
-  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
+  nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
            [&](int, uint32_t e, const G& g) {
              float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
              if (nb_fluid(e)) delta += g.m * dd;
@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_p
     v3 vort = smul(w, ld3(st.vorticity, i));
     float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
     struct G { v3 x, dv, vo; float mV, kvn; };
-    nbr_loop(nl, ns, nn,
+    nbr_loop<8>(nl, ns, nn,
              [&](int64_t j) {
                return G{ld3(st.x, j), ld3(st.d_vort, j), ld3(st.vorticity, j), st.m_V[j], st.kinematic_vort_num[j]};
              },

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   v3 vv = mk(0.0f, 0.0f, 0.0f);
   float kns = st.kernel_density[i];
   struct G { v3 x, w; float kd; };
-  nbr_loop(nl, ns, nn,
+  nbr_loop<8>(nl, ns, nn,
            [&](int64_t j) { return G{ld3(st.x, j), ld3(st.d_vort_smoothed, j), st.kernel_density[j]}; },
            [&](int, uint32_t e, const G& g) {
              if (!nb_fluid(e)) return;
