@@ -69,7 +69,9 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
   float dadv = 0.0f;
   struct G { v3 x, v; float m, mV; };
   nbr_loop(nl, ns, nn,
-           [&](int64_t j) { return G{ld3(st.x, j), CHANGE ? ld3(st.v, j) : mk(0.0f, 0.0f, 0.0f), st.m[j], st.m_V[j]}; },
+           [&](int64_t j) {
+             return G{ld3(st.x, j), CHANGE ? ld3(st.v, j) : mk(0.0f, 0.0f, 0.0f), st.m[j], st.m_V[j]};
+           },
            [&](int, uint32_t e, const G& g) {
              v3 r = rel(p, xi, g.x);
              den += g.mV * cubic_W(p, vnorm(r));
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
   float dadv = 0.0f;
   struct G { v3 x, v; float mV; };
   nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
-           [&](int, uint32_t, const G& g) { dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x))); });
+              [&](int, uint32_t, const G& g) { dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x))); });
   float d = fmaxf(dadv, 0.0f);
   if (st.nbr_count[f] < 20) d = 0.0f;
   st.density_adv[i] = d;
