@@ -274,7 +274,8 @@ class RingRollout:
 
     def initial_velocity(self, gamma) -> torch.Tensor:
         g = torch.as_tensor(np.asarray(gamma, np.float64), device=self.U.device)
-        return torch.einsum("k,knd->nd", g, self.U).float()
+        # sum_k g_k U_k as a broadcast sum (a GEMM call for this shape is slow)
+        return (g[:, None, None] * self.U).sum(0).float()
 
     # ------------------------------------------------------------ state --
     def _tensors(self):
@@ -337,7 +338,7 @@ class RingRollout:
                 xb, vb = step_vjp(self.ps, self.sv, T, xb, vb)
             self.last_tapes = tapes + self.last_tapes
         self.v0_bar = vb
-        return torch.einsum("nd,knd->k", vb.double(), self.U)
+        return (vb.double()[None] * self.U).sum(dim=(1, 2))
 
     def structure(self, gamma):
         """Run the rollout from gamma recording every step's tape (the frozen

@@ -24,7 +24,8 @@
 namespace dimcv {
 
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
-                       const int32_t* done, hipStream_t s);
+                       const int32_t* done, hipStream_t s, double* partials = nullptr,
+                       float offset = 0.0f);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials = nullptr);
 int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,

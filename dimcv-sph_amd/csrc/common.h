@@ -398,6 +398,10 @@ int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw);
 // host helpers implemented in util.hip
 int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream,
                         const int32_t* done = nullptr);
+// reduce_partials_f64 in one launch (the last group finishes the tree),
+// optionally followed by the Jacobi convergence test on the sum.
+int reduce_tail_f64(double* partials, int64_t n, double* out, int32_t* counters, const int32_t* done, int check,
+                    int32_t fluid_count, double eta, int32_t max_iter, hipStream_t s);
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, uint32_t* total,
                        void* stream);
 

@@ -134,19 +134,42 @@ __global__ void __launch_bounds__(256) k_dfsph_factor(dimcv_state st, dimcv_para
   st.dfsph_factor[i] = factor;
 }
 
-// dimcv_sph.py:337-374
+// Residual term of compute_density_error (dimcv_sph.py:407-419) of slot i
+// whose density_adv is da, and the block's canonical FP64 partial of it
+// (as k_residual forms them); every thread of the block calls it.
+__device__ __forceinline__ double residual_term(const dimcv_params& p, const dimcv_state& st, int64_t i, float da,
+                                                float offset) {
+  const float t = !p.has_transit ? da - offset : transit_weight(p, st.x[3 * i]) * (da - offset);
+  return (double)t;
+}
+__device__ __forceinline__ void residual_partial(double v, double* __restrict__ partials, double* lds) {
+  const double sum = block_tree_sum256(v, lds);
+  if (threadIdx.x == 0) partials[xcd_block()] = sum;
+}
+
+// dimcv_sph.py:337-374.  With `partials`, also the block partial of the
+// divergence residual (offset 0) of the values just written.
 __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done) {
-  if (jacobi_gated(st, done)) return;
-  FLUID_ENTRY();
-  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
-  float dadv = 0.0f;
-  struct G { v3 x, v; float mV; };
-  nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
-              [&](int, uint32_t, const G& g) { dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x))); });
-  float d = fmaxf(dadv, 0.0f);
-  if (st.nbr_count[f] < 20) d = 0.0f;
-  st.density_adv[i] = d;
+                                                  const int32_t* __restrict__ done,
+                                                  double* __restrict__ partials) {
+  __shared__ double lds[256];
+  if (jacobi_gated(st, done)) return;  // uniform per block
+  double res = 0.0;
+  [&] {
+    FLUID_ENTRY();
+    v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+    float dadv = 0.0f;
+    struct G { v3 x, v; float mV; };
+    nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+                [&](int, uint32_t, const G& g) {
+                  dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
+                });
+    float d = fmaxf(dadv, 0.0f);
+    if (st.nbr_count[f] < 20) d = 0.0f;
+    st.density_adv[i] = d;
+    res = residual_term(p, st, i, d, 0.0f);
+  }();
+  if (partials) residual_partial(res, partials, lds);
 }
 
 // dimcv_sph.py:395-399
@@ -269,21 +292,31 @@ __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
 }
 
 // dimcv_sph.py:470-493
+// With `partials`, also the block partial of the pressure residual
+// (offset: the caller's density0) of the values just written.
 __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done) {
-  if (jacobi_gated(st, done)) return;
-  FLUID_ENTRY();
-  v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
-  float delta = 0.0f;
-  struct G { v3 x, v; float m, mV; };
-  nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
-           [&](int, uint32_t e, const G& g) {
-             float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
-             if (nb_fluid(e)) delta += g.m * dd;
-             else delta += g.mV * p.density0 * dd;
-           });
-  float da = st.density[i] + p.dt * delta;
-  st.density_adv[i] = fmaxf(da, p.density0);
+                                                  const int32_t* __restrict__ done,
+                                                  double* __restrict__ partials, float offset) {
+  __shared__ double lds[256];
+  if (jacobi_gated(st, done)) return;  // uniform per block
+  double res = 0.0;
+  [&] {
+    FLUID_ENTRY();
+    v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
+    float delta = 0.0f;
+    struct G { v3 x, v; float m, mV; };
+    nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
+                [&](int, uint32_t e, const G& g) {
+                  float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
+                  if (nb_fluid(e)) delta += g.m * dd;
+                  else delta += g.mV * p.density0 * dd;
+                });
+    float da = st.density[i] + p.dt * delta;
+    const float d = fmaxf(da, p.density0);
+    st.density_adv[i] = d;
+    res = residual_term(p, st, i, d, offset);
+  }();
+  if (partials) residual_partial(res, partials, lds);
 }
 
 // dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour)
@@ -612,15 +645,17 @@ int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t
 
 // Jacobi pieces gated by a device flag (iterations after convergence are
 // skipped on the device), used by the native substep.
+// which 1 and 3 also write the residual's block partials when `partials`
+// is given (offset: that of the pressure residual).
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
-                       const int32_t* done, hipStream_t s) {
+                       const int32_t* done, hipStream_t s, double* partials, float offset) {
   if (fluid_bound <= 0) return DIMCV_OK;
   dim3 g(nblocks(fluid_bound)), b(256);
   switch (which) {
     case 0: hipLaunchKernelGGL(k_div_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
-    case 1: hipLaunchKernelGGL(k_density_change, g, b, 0, s, *st, *p, fluid_bound, done); break;
+    case 1: hipLaunchKernelGGL(k_density_change, g, b, 0, s, *st, *p, fluid_bound, done, partials); break;
     case 2: hipLaunchKernelGGL(k_pressure_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
-    default: hipLaunchKernelGGL(k_density_adv, g, b, 0, s, *st, *p, fluid_bound, done); break;
+    default: hipLaunchKernelGGL(k_density_adv, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset); break;
   }
   DIMCV_LAUNCHED("jacobi pass");
   return DIMCV_OK;
@@ -655,16 +690,6 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
   }
   DIMCV_LAUNCHED("fused pass");
   return DIMCV_OK;
-}
-
-int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, int32_t fluid_bound,
-                    double* partials, const int32_t* done, hipStream_t s) {
-  unsigned nb = nblocks(fluid_bound);
-  if (nb > 0) {
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, *st, *p, offset, fluid_bound, partials, done);
-    DIMCV_LAUNCHED("k_residual");
-  }
-  return reduce_partials_f64(partials, nb, st->dscal + DIMCV_D_RESIDUAL, s, done);
 }
 
 // Per-block FP64 partials of the residual (which = 0, with offset) or of
@@ -715,7 +740,7 @@ int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int
 }
 
 int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound, nullptr, nullptr);
   return DIMCV_OK;
 }
 
@@ -769,7 +794,7 @@ int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_
 }
 
 int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound, nullptr, nullptr, 0.0f);
   return DIMCV_OK;
 }
 

@@ -13,9 +13,8 @@
 namespace dimcv {
 
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
-                       const int32_t* done, hipStream_t s);
-int launch_residual(const dimcv_state* st, const dimcv_params* p, float offset, int32_t fluid_bound,
-                    double* partials, const int32_t* done, hipStream_t s);
+                       const int32_t* done, hipStream_t s, double* partials = nullptr,
+                       float offset = 0.0f);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials = nullptr);
@@ -29,13 +28,6 @@ __global__ void k_jacobi_reset(int32_t* counters) {
   counters[DIMCV_C_JACOBI_ITERS] = 0;
 }
 
-__global__ void k_jacobi_check(int32_t* counters, const double* sum, int32_t fluid_count, double eta,
-                               int32_t max_iter) {
-  if (counters[DIMCV_C_JACOBI_DONE]) return;
-  int32_t it = ++counters[DIMCV_C_JACOBI_ITERS];
-  double avg = *sum / (double)fluid_count;
-  if (avg <= eta || it >= max_iter) counters[DIMCV_C_JACOBI_DONE] = 1;
-}
 
 // which = 0: divergence (iteration, density change, residual offset 0)
 // which = 1: pressure (iteration, density_adv, residual offset rho0)
@@ -50,12 +42,14 @@ static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv
   int32_t host[2] = {0, 0};
   for (;;) {
     for (int c = 0; c < chunk; ++c) {
+      // iteration pass; density pass with the residual's block partials;
+      // the FP64 reduction and the convergence test in one launch
       int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s);
-      if (!rc) rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s);
-      if (!rc) rc = launch_residual(st, p, offset, fluid_bound, ws.partials, done, s);
+      if (!rc) rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s, ws.partials, offset);
+      if (!rc)
+        rc = reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_RESIDUAL, st->counters, done, 1,
+                             o->fluid_count, eta, max_iter, s);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(1), 0, s, st->counters, st->dscal + DIMCV_D_RESIDUAL,
-                         o->fluid_count, eta, max_iter);
     }
     DIMCV_LAUNCHED("jacobi chunk");
     DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, 2 * sizeof(int32_t),
@@ -115,7 +109,8 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
   if (o->m > 0) CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
   else CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
   CHECK(launch_fused(4, st, p, fluid_bound, s, ws.partials));
-  CHECK(reduce_partials_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_KDEN_SUM, stream, nullptr));
+  CHECK(reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_KDEN_SUM, st->counters, nullptr, 0,
+                        0, 0.0, 0, s));
   if (o->m > 0) CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
   else CHECK(dimcv_record_sample_idx(st, p, particle_bound, stream));
   if (o->time_far) {

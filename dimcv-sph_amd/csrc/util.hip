@@ -119,6 +119,65 @@ __global__ void k_reduce_groups(const double* __restrict__ a, int64_t n, double*
   if (threadIdx.x == 0) b[blockIdx.x] = s;
 }
 
+// reduce_partials_f64 in one launch, optionally followed by the Jacobi
+// convergence check.  Block g sums group g (level 1 of the tree) into
+// b[g] = a[n + g]; the last block to finish (a ticket in counters, device-
+// scope fences around it) runs the remaining levels exactly as
+// k_reduce_levels does, writes *out, clears the ticket and, with check,
+// applies the Jacobi loop's test (dimcv_sph.py:376-393, 597-614): one more
+// iteration; done when the average residual is <= eta or max_iter is hit.
+__global__ void k_reduce_tail(double* a, int64_t n, double* out, int32_t* counters, const int32_t* done,
+                              int check, int32_t fluid_count, double eta, int32_t max_iter) {
+  __shared__ double lds[256];
+  __shared__ int last;
+  if (done && *done) return;
+  const int t = threadIdx.x;
+  const int64_t G = gridDim.x;
+  const int64_t i = (int64_t)blockIdx.x * 256 + t;
+  double r = block_tree_sum256(i < n ? a[i] : 0.0, lds);
+  if (G > 1) {
+    double* b = a + n;
+    if (t == 0) {
+      b[blockIdx.x] = r;
+      __threadfence();
+      last = atomicAdd(&counters[DIMCV_C_REDUCE_TICKET], 1) == (int)G - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    int64_t m = G;
+    while (m > 1) {
+      const int64_t groups = (m + 255) / 256;
+      for (int64_t g = 0; g < groups; ++g) {
+        const int64_t k = g * 256 + t;
+        const double v = k < m ? __hip_atomic_load(&b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        const double s = block_tree_sum256(v, lds);
+        if (t == 0) b[g] = s;
+        __syncthreads();
+      }
+      m = groups;
+    }
+    r = __hip_atomic_load(&b[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) counters[DIMCV_C_REDUCE_TICKET] = 0;
+  }
+  if (t != 0) return;
+  *out = n > 0 ? r : 0.0;
+  if (check) {
+    const int32_t it = ++counters[DIMCV_C_JACOBI_ITERS];
+    const double avg = r / (double)fluid_count;
+    if (avg <= eta || it >= max_iter) counters[DIMCV_C_JACOBI_DONE] = 1;
+  }
+}
+
+int reduce_tail_f64(double* partials, int64_t n, double* out, int32_t* counters, const int32_t* done, int check,
+                    int32_t fluid_count, double eta, int32_t max_iter, hipStream_t s) {
+  const unsigned g = n > 0 ? (unsigned)((n + 255) / 256) : 1;
+  hipLaunchKernelGGL(k_reduce_tail, dim3(g), dim3(256), 0, s, partials, n, out, counters, done, check, fluid_count,
+                     eta, max_iter);
+  DIMCV_LAUNCHED("k_reduce_tail");
+  return DIMCV_OK;
+}
+
 int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream, const int32_t* done) {
   hipStream_t s = (hipStream_t)stream;
   double* a = const_cast<double*>(partials);

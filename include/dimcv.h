@@ -56,6 +56,7 @@ typedef enum dimcv_counter {
   DIMCV_C_SAMPLE_SLOTS = 8,   /* is_sample count (compaction scratch) */
   DIMCV_C_JACOBI_DONE = 9,    /* native substep: current solve has converged */
   DIMCV_C_JACOBI_ITERS = 10,  /* native substep: iterations of the current solve */
+  DIMCV_C_REDUCE_TICKET = 11, /* native substep: groups done in the running FP64 reduction (0 between calls) */
   DIMCV_C_COUNT = 16
 } dimcv_counter;
 
