@@ -51,7 +51,7 @@ inline unsigned nblocks(int64_t n, int per = DIMCV_BLOCK) {
 struct Workspace {
   uint32_t* keys[2];
   uint32_t* vals[2];
-  uint32_t* radix_hist;   // 256 * sort blocks
+  uint32_t* radix_hist;   // 512 * sort blocks (digits of up to 9 bits)
   uint32_t* scan_tmp;     // block sums for the u32 scan
   uint32_t* compact_cnt;  // per-block counts for compaction
   double* partials;       // per-block FP64 partial sums

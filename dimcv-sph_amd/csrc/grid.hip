@@ -151,38 +151,44 @@ __global__ void k_set_particle_num(int32_t* counters, int32_t n_in) {
 
 // ---------------------------------------------------------------------------
 // stable LSD radix sort, 8-bit digits, 4096 keys per workgroup
+// Digits of BITS bits (8 or 9: 9 when that saves a pass over the key bits).
+template <int BITS>
 __global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n,
                                                     int shift, uint32_t* __restrict__ hist, int nblk) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr int R = 1 << BITS;
+  constexpr uint32_t MASK = R - 1;
+  __shared__ uint32_t h[R];
+  for (int d = threadIdx.x; d < R; d += 256) h[d] = 0;
   __syncthreads();
   int64_t base = (int64_t)blockIdx.x * DIMCV_SORT_TILE;
 #pragma unroll 4
   for (int k = 0; k < DIMCV_SORT_TILE / 256; ++k) {
     int64_t i = base + k * 256 + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & MASK], 1u);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < R; d += 256) hist[(int64_t)d * nblk + blockIdx.x] = h[d];
 }
 
 // Each workgroup walks its tile in 16 strips of 256 keys (index order).  In a
-// strip, every wave ranks its lanes among same-digit lanes with 8 ballots;
+// strip, every wave ranks its lanes among same-digit lanes with BITS ballots;
 // wave counts per digit go to LDS so a key's output position is
 //   base(digit, block) + keys of that digit in earlier strips
 //   + same-digit keys in lower waves + same-digit lanes below it,
 // which is exactly its stable rank.
+template <int BITS>
 __global__ void __launch_bounds__(256) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int nblk) {
-  __shared__ uint32_t cnt[256];
-  __shared__ uint32_t wc[4][256];
+  constexpr int R = 1 << BITS;
+  constexpr uint32_t MASK = R - 1;
+  __shared__ uint32_t cnt[R];
+  __shared__ uint32_t wc[4][R];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  cnt[t] = offs[(int64_t)t * nblk + blockIdx.x];
-  wc[0][t] = 0;
-  wc[1][t] = 0;
-  wc[2][t] = 0;
-  wc[3][t] = 0;
+  for (int d = t; d < R; d += 256) {
+    cnt[d] = offs[(int64_t)d * nblk + blockIdx.x];
+    wc[0][d] = wc[1][d] = wc[2][d] = wc[3][d] = 0;
+  }
   __syncthreads();
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int64_t base = (int64_t)blockIdx.x * DIMCV_SORT_TILE;
@@ -191,10 +197,10 @@ __global__ void __launch_bounds__(256) k_radix_scatter(
     bool valid = i < n;
     uint32_t key = valid ? kin[i] : 0u;
     uint32_t val = valid ? vin[i] : 0u;
-    uint32_t d = (key >> shift) & 255u;
+    uint32_t d = (key >> shift) & MASK;
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
       bool bit = (d >> b) & 1u;
       uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
@@ -209,11 +215,10 @@ __global__ void __launch_bounds__(256) k_radix_scatter(
       vout[pos] = val;
     }
     __syncthreads();
-    cnt[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-    wc[0][t] = 0;
-    wc[1][t] = 0;
-    wc[2][t] = 0;
-    wc[3][t] = 0;
+    for (int dd = t; dd < R; dd += 256) {
+      cnt[dd] += wc[0][dd] + wc[1][dd] + wc[2][dd] + wc[3][dd];
+      wc[0][dd] = wc[1][dd] = wc[2][dd] = wc[3][dd] = 0;
+    }
     __syncthreads();
   }
 }
@@ -473,19 +478,31 @@ __global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound,
 // n_cells), the cell table and the gather into the *_buffer fields.  Keys
 // and slots are in ws.keys[0] / ws.vals[0].
 int sort_cells_gather(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int64_t n_in, hipStream_t s) {
-  int nbits = 32 - __builtin_clz((unsigned)p->n_cells);
-  int passes = (nbits + 7) / 8;
+  // key bits in use (the sentinel n_cells included); 9-bit digits where
+  // they need fewer passes than 8-bit ones (18 bits: 2 passes, not 3)
+  const int nbits = 32 - __builtin_clz((unsigned)p->n_cells);
+  const int bits = (nbits + 8) / 9 < (nbits + 7) / 8 ? 9 : 8;
+  const int passes = (nbits + bits - 1) / bits;
   int cur = 0;
   if (n_in > 0) {
     int nblk = (int)((n_in + DIMCV_SORT_TILE - 1) / DIMCV_SORT_TILE);
     for (int ps = 0; ps < passes; ++ps) {
-      int shift = 8 * ps;
-      hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift,
-                         ws.radix_hist, nblk);
-      int rc = exclusive_scan_u32(ws.radix_hist, ws.radix_hist, (int64_t)256 * nblk, ws.scan_tmp, nullptr, s);
+      int shift = bits * ps;
+      if (bits == 9)
+        hipLaunchKernelGGL(k_radix_hist<9>, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift, ws.radix_hist,
+                           nblk);
+      else
+        hipLaunchKernelGGL(k_radix_hist<8>, dim3(nblk), dim3(256), 0, s, ws.keys[cur], n_in, shift, ws.radix_hist,
+                           nblk);
+      int rc = exclusive_scan_u32(ws.radix_hist, ws.radix_hist, (int64_t)(1 << bits) * nblk, ws.scan_tmp, nullptr,
+                                  s);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
-                         ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
+      if (bits == 9)
+        hipLaunchKernelGGL(k_radix_scatter<9>, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
+                           ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
+      else
+        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(nblk), dim3(256), 0, s, ws.keys[cur], ws.vals[cur],
+                           ws.keys[cur ^ 1], ws.vals[cur ^ 1], n_in, shift, ws.radix_hist, nblk);
       DIMCV_LAUNCHED("k_radix_scatter");
       cur ^= 1;
     }

@@ -41,8 +41,8 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.keys[1] = (uint32_t*)take(4 * n);
   w.vals[0] = (uint32_t*)take(4 * n);
   w.vals[1] = (uint32_t*)take(4 * n);
-  w.radix_hist = (uint32_t*)take(4 * 256 * sort_blocks);
-  w.scan_tmp = (uint32_t*)take(4 * (256 * sort_blocks / 1024 + 1024));
+  w.radix_hist = (uint32_t*)take(4 * 512 * sort_blocks);  // up to 9-bit digits
+  w.scan_tmp = (uint32_t*)take(4 * (512 * sort_blocks / 1024 + 1024));
   w.compact_cnt = (uint32_t*)take(4 * (nb256 + 1024));
   w.partials = (double*)take(8 * (nb256 + nb256 / 256 + 512));
   w.cdf = (double*)take(8 * n);
