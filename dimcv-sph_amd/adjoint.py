@@ -176,7 +176,7 @@ def step_vjp(ps, sv, T: StepTape, xb_id: torch.Tensor, vb_id: torch.Tensor):
     adj(N.ADJ_GATE, f0=T.v_vort, f1=T.v_div, b2=gate)
     ub = (gate[:, None] * vb).contiguous()
     S_b, wts, ws_b, kden_b = z1(), z3(), z3(), z1()
-    adj(N.ADJ_FAR_TARGETS, f0=T.ws, f1=T.kden, g_in=ub, x_bar=xb, b0=S_b, b2=wts, dscalar=T.K)
+    adj(N.ADJ_FAR_TARGETS, f0=T.ws, f1=T.kden, g_in=ub, x_bar=xb, b0=S_b, b2=wts, dscalar=T.K, mode=T.M)
     adj(N.ADJ_FAR_SOURCES, f0=T.ws, f1=T.kden, f2=wts, x_bar=xb, b0=ws_b, b1=kden_b, mode=T.M)
     adj(N.ADJ_NEAR, f0=T.ws, f1=S_b, g_in=ub, x_bar=xb, b0=ws_b, b1=kden_b)
     kden_b += S_b.double().sum().float()                  # S_i = kden_sum - ...

@@ -19,6 +19,8 @@
 // the product Epanechnikov kernel (dimcv_sph.py:721-736) differentiates
 // factor by factor; A(s) = (1 - E)/(s (s + eps E)), E = exp(-s/eps), is the
 // kernel of vort2vel (dimcv_sph.py:652-657, 698-706).
+#include <algorithm>
+
 #include "common.h"
 
 namespace dimcv {
@@ -406,12 +408,29 @@ __global__ void k_adj_near(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
   atomic_add3(a.x_bar, i, xb);
 }
 
-// Far field of vort2vel, target side (dimcv_sph.py:693-716):
-//   vs_i = fac / n_i * S_i * sum_{k accepted} A(|r|) (r_xy x w_k) / (kden_k + 1e-8).
-// S_i is recomputed as in the forward.  in: f0 = w, f1 = kden, g_in = u_bar,
-// dscalar = kden_sum.  out: x_bar, b0 = S_bar, b2 = the per-target weight
-// fac / n_i * S_i * u_bar (3 floats, read by the source pass).
-__global__ void k_adj_far_targets(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
+// Far field of vort2vel (dimcv_sph.py:693-716) for target i over the
+// accepted samples k (reference-order r^2 > T):
+//   v_vort_i += (vp_factor / added_i) S_i sum_k A(|r|) c_k (r_xy x w_k),
+// r = x_i - x_k, c_k = 1 / (kden_k + 1e-8), A(s) = (1 - E) / (s (s + eps E)),
+// E = exp(-s / eps).  Every accepted pair has s > h >> eps, where E is 0 in
+// f32 (eps = 1e-6: s / eps > 4e4), so A = 1 / r^2 and dA/ds / s = -2 A^2.
+// The samples are read from a packed table (scalar loads, one entry per
+// sample for the whole wave): (x, c) and (w, slot).
+__global__ void k_adj_pack_samples(dimcv_state st, dimcv_adj_args a, int32_t cap, float4* __restrict__ tab) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
+  if (q >= M) return;
+  const int64_t sk = st.sample_idx[q];
+  tab[2 * q] = make_float4(st.x[3 * sk], st.x[3 * sk + 1], st.x[3 * sk + 2], 1.0f / (a.f1[sk] + 1e-8f));
+  tab[2 * q + 1] = make_float4(a.f0[3 * sk], a.f0[3 * sk + 1], a.f0[3 * sk + 2], __int_as_float((int)sk));
+}
+
+// Target side.  in: f0 = w, f1 = kden, g_in = u_bar (v_vort adjoint),
+// dscalar = kden_sum.  out: x_bar, b0 += S_bar, b2 = the target's weights
+// (vp_factor / added) S u_bar (0 for targets without samples).  One loop
+// counts the accepted samples, the second sums with u_bar and scales once.
+__global__ void k_adj_far_targets(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound, int32_t cap,
+                                  const float4* __restrict__ tab) {
   ADJ_ENTRY();
   st3(a.b2, i, mk(0.0f, 0.0f, 0.0f));
   if (!in_vortex_domain(p, xi)) return;
@@ -419,80 +438,97 @@ __global__ void k_adj_far_targets(dimcv_state st, dimcv_params p, dimcv_adj_args
   float kns = a.f1[i];
   FOR_FLUID_NBR(k, j) { kns += a.f1[j]; }
   const float S = (float)(a.dscalar - (double)kns);
-  const int M = st.counters[DIMCV_C_NUM_SAMPLES];
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
   const float T = p.far_r2_max_rejected;
+  auto r2_of = [&](const float4& s4, v3& r) {
+    r = mk(xi.x - s4.x, xi.y - s4.y, xi.z - s4.z);
+    return __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
+  };
   int added = 0;
+#pragma unroll 4
   for (int q = 0; q < M; ++q) {
-    const v3 r = vsub(xi, ld3(st.x, st.sample_idx[q]));
-    const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
-    added += r2 > T ? 1 : 0;
+    v3 r;
+    added += r2_of(tab[2 * q], r) > T ? 1 : 0;
   }
   if (added == 0) return;
   const float coef = p.vp_factor / (float)added;
-  const v3 w = smul(coef * S, ub);
-  st3(a.b2, i, w);
-  v3 sum = mk(0.0f, 0.0f, 0.0f), xb = mk(0.0f, 0.0f, 0.0f);
+  const float sc = coef * S;
+  st3(a.b2, i, smul(sc, ub));
+  v3 sum = mk(0.0f, 0.0f, 0.0f), xbu = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll 2
   for (int q = 0; q < M; ++q) {
-    const int64_t sk = st.sample_idx[q];
-    const v3 r = vsub(xi, ld3(st.x, sk));
-    const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
+    const float4 s4 = tab[2 * q], w4 = tab[2 * q + 1];
+    v3 r;
+    const float r2 = r2_of(s4, r);
     if (!(r2 > T)) continue;
-    const float s = sqrtf(r2);
-    float dA;
-    const float A = a_kernel(p, s, dA);
-    const float c = 1.0f / (a.f1[sk] + 1e-8f);
-    const v3 rxy = mk(r.x, r.y, 0.0f);
-    const v3 wk = ld3(a.f0, sk);
-    const v3 t = vcross(rxy, wk);
+    const float A = __builtin_amdgcn_rcpf(r2);
+    const float c = s4.w;
+    const v3 t = mk(r.y * w4.z, -(r.x * w4.z), r.x * w4.y - r.y * w4.x);  // r_xy x w_k
     sum = vadd(sum, smul(A * c, t));
-    v3 rb = smul(A, vcross(wk, w));
-    rb.z = 0.0f;
-    rb = vadd(rb, smul(vdot(t, w) * dA / s, r));
-    acc3(xb, smul(c, rb));
+    // d/dx_i of A (r_xy x w_k).u: A (w_k x u)_xy + (t.u) (dA/ds / s) r
+    const float kk = -2.0f * A * A * vdot(t, ub);
+    const float cx = w4.y * ub.z - w4.z * ub.y, cy = w4.z * ub.x - w4.x * ub.z;
+    xbu = vadd(xbu, smul(c, mk(A * cx + kk * r.x, A * cy + kk * r.y, kk * r.z)));
   }
   a.b0[i] += coef * vdot(ub, sum);
-  atomic_add3(a.x_bar, i, xb);
+  atomic_add3(a.x_bar, i, smul(sc, xbu));
 }
 
-// Far field, source side: for sample k (slot s_k) the sums over targets of
-// the terms belonging to the source: w_bar_{s_k}, kden_bar_{s_k} (through
-// c = 1 / (kden + 1e-8)) and x_bar_{s_k}.  Thread per sample, targets split
-// over blockIdx.y (partial sums added atomically).  in: f0 = w, f1 = kden,
-// f2 = per-target weights (b2 of the target pass), mode = sample bound.
-// out: x_bar, b0 = w_bar, b1 = kden_bar.
-__global__ void k_adj_far_sources(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
+// Targets packed for the source side: (x_i, -) and (weights_i, -) per fluid
+// entry; entries without weight are all zero (skipped).
+__global__ void k_adj_pack_targets(dimcv_state st, dimcv_adj_args a, int32_t bound, float4* __restrict__ tab) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nf = min((int64_t)bound, (int64_t)st.counters[DIMCV_C_FLUID_LIST]);
+  if (f >= nf) return;
+  const int64_t i = st.fluid_list[f];
+  float4 x4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w4 = x4;
+  if (i >= 0) {
+    const v3 w = ld3(a.f2, i);
+    if (w.x != 0.0f || w.y != 0.0f || w.z != 0.0f) {
+      x4 = make_float4(st.x[3 * i], st.x[3 * i + 1], st.x[3 * i + 2], 0.0f);
+      w4 = make_float4(w.x, w.y, w.z, 1.0f);
+    }
+  }
+  tab[2 * f] = x4;
+  tab[2 * f + 1] = w4;
+}
+
+// Source side: for sample k the sums over targets of w_bar_k, kden_bar_k
+// (through c_k) and x_bar_k.  One lane per sample, targets read from their
+// packed table (scalar loads) in blockIdx.y chunks; one set of atomics per
+// lane.  in: f0 = w, f1 = kden, f2 = per-target weights (b2 of the target
+// pass), mode = sample bound.  out: x_bar, b0 = w_bar, b1 = kden_bar.
+__global__ void k_adj_far_sources(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound,
+                                  const float4* __restrict__ stab, const float4* __restrict__ ttab, int32_t cap) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int M = st.counters[DIMCV_C_NUM_SAMPLES];
-  if (q >= M) return;
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
   const int64_t nf = min((int64_t)bound, (int64_t)st.counters[DIMCV_C_FLUID_LIST]);
   const int64_t chunk = (nf + gridDim.y - 1) / gridDim.y;
   const int64_t f0 = (int64_t)blockIdx.y * chunk, f1 = min(f0 + chunk, nf);
-  const int64_t sk = st.sample_idx[q];
-  const v3 xs = ld3(st.x, sk);
-  const v3 wk = ld3(a.f0, sk);
-  const float c = 1.0f / (a.f1[sk] + 1e-8f);
+  if (q >= M) return;
+  const float4 s4 = stab[2 * q], k4 = stab[2 * q + 1];
+  const v3 xs = mk(s4.x, s4.y, s4.z), wk = mk(k4.x, k4.y, k4.z);
+  const float c = s4.w;
+  const int64_t sk = __float_as_int(k4.w);
   const float T = p.far_r2_max_rejected;
   v3 wbar = mk(0.0f, 0.0f, 0.0f), xb = mk(0.0f, 0.0f, 0.0f);
   float cbar = 0.0f;
+#pragma unroll 2
   for (int64_t f = f0; f < f1; ++f) {
-    const int64_t i = st.fluid_list[f];
-    if (i < 0) continue;
-    const v3 w = ld3(a.f2, i);
-    if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f) continue;
-    const v3 r = vsub(ld3(st.x, i), xs);
+    const float4 x4 = ttab[2 * f], w4 = ttab[2 * f + 1];
+    if (w4.w == 0.0f) continue;  // uniform: target without weight
+    const v3 r = mk(x4.x - xs.x, x4.y - xs.y, x4.z - xs.z);
     const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
     if (!(r2 > T)) continue;
-    const float s = sqrtf(r2);
-    float dA;
-    const float A = a_kernel(p, s, dA);
-    const v3 rxy = mk(r.x, r.y, 0.0f);
-    const v3 t = vcross(rxy, wk);
-    wbar = vadd(wbar, smul(A * c, vcross(w, rxy)));
-    cbar += A * vdot(t, w);
-    v3 rb = smul(A, vcross(wk, w));
-    rb.z = 0.0f;
-    rb = vadd(rb, smul(vdot(t, w) * dA / s, r));
-    xb = vsub(xb, smul(c, rb));
+    const float A = __builtin_amdgcn_rcpf(r2);
+    const v3 t = mk(r.y * wk.z, -(r.x * wk.z), r.x * wk.y - r.y * wk.x);  // r_xy x w_k
+    const float tw = t.x * w4.x + t.y * w4.y + t.z * w4.z;
+    // w x r_xy
+    wbar = vadd(wbar, smul(A * c, mk(-(w4.z * r.y), w4.z * r.x, w4.x * r.y - w4.y * r.x)));
+    cbar += A * tw;
+    const float kk = -2.0f * A * A * tw;
+    const float cx = wk.y * w4.z - wk.z * w4.y, cy = wk.z * w4.x - wk.x * w4.z;
+    xb = vsub(xb, smul(c, mk(A * cx + kk * r.x, A * cy + kk * r.y, kk * r.z)));
   }
   atomic_add3(a.b0, sk, wbar);
   atomicAdd(a.b1 + sk, -c * c * cbar);
@@ -530,12 +566,29 @@ extern "C" int dimcv_adj_pass(const dimcv_state* st, const dimcv_params* p, int3
     case DIMCV_ADJ_GATE: hipLaunchKernelGGL(k_adj_gate, g, b, 0, s, *st, *p, *a, fluid_bound); break;
     case DIMCV_ADJ_NEAR: hipLaunchKernelGGL(k_adj_near, g, b, 0, s, *st, *p, *a, fluid_bound); break;
     case DIMCV_ADJ_FAR_TARGETS:
-      hipLaunchKernelGGL(k_adj_far_targets, g, b, 0, s, *st, *p, *a, fluid_bound);
-      break;
     case DIMCV_ADJ_FAR_SOURCES: {
-      // a->mode bounds the sample count; targets split 64 ways
+      // a->mode bounds the sample count.  Sample table: the far-field
+      // partials region; target table: the sample region (n_max entries).
       if (a->mode <= 0) break;
-      hipLaunchKernelGGL(k_adj_far_sources, dim3(nblocks(a->mode), 64), b, 0, s, *st, *p, *a, fluid_bound);
+      Workspace ws;
+      int rc = bind_workspace(st, p, &ws, 0);
+      if (rc) return rc;
+      if ((int64_t)a->mode > ws.far_partial_cap || (int64_t)fluid_bound > ws.max_samples) {
+        set_error("dimcv_adj_pass: far-field tables exceed the workspace (%d samples, %d targets)", a->mode,
+                  fluid_bound);
+        return DIMCV_ERR_WORKSPACE;
+      }
+      float4* stab = (float4*)ws.far_partials;
+      hipLaunchKernelGGL(k_adj_pack_samples, dim3(nblocks(a->mode)), b, 0, s, *st, *a, a->mode, stab);
+      if (pass == DIMCV_ADJ_FAR_TARGETS) {
+        hipLaunchKernelGGL(k_adj_far_targets, g, b, 0, s, *st, *p, *a, fluid_bound, a->mode, (const float4*)stab);
+      } else {
+        hipLaunchKernelGGL(k_adj_pack_targets, g, b, 0, s, *st, *a, fluid_bound, ws.samples);
+        // targets split into chunks of ~256 over blockIdx.y
+        const unsigned ych = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (fluid_bound + 255) / 256));
+        hipLaunchKernelGGL(k_adj_far_sources, dim3(nblocks(a->mode), ych), b, 0, s, *st, *p, *a, fluid_bound,
+                           (const float4*)stab, (const float4*)ws.samples, a->mode);
+      }
       break;
     }
     default:

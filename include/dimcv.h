@@ -490,13 +490,16 @@ void dimcv_batch_graph_destroy(dimcv_batch_graph* g);
  *  KVN                  :295-312           grad_v (9)                    kvn_bar     b1 = grad_v_bar (9)
  *  KDEN                 :721-751           kvn                           kden_bar    x_bar, b1 = kvn_bar
  *  GATE                 :717-719           v_vort / v                    -           b2 = gate (1 / 0)
- *  FAR_TARGETS          :693-716           w / kden; dscalar = kden_sum  u_bar       x_bar, b0 = S_bar, b2 = weights
+ *  FAR_TARGETS          :693-716           w / kden; kden_sum; mode = M  u_bar       x_bar, b0 = S_bar, b2 = weights
  *  FAR_SOURCES          :693-716           w / kden / weights; mode = M  -           x_bar, b0 = w_bar, b1 = kden_bar
  *  NEAR                 :649-658, 687-692  w / S_bar                     u_bar       x_bar, b0 = w_bar, b1 = kden_bar
  *
  * P_bar is the adjoint of the sum a Jacobi iteration subtracts from v (minus
  * the adjoint of the iteration's output velocity); u_bar is the gated
- * adjoint of the recovered velocity. */
+ * adjoint of the recovered velocity.  The far-field passes take the sample
+ * bound M in `mode` (kden_sum in `dscalar`) and pack samples and targets into
+ * the workspace's far-field regions (DIMCV_ERR_WORKSPACE when they do not
+ * fit). */
 typedef enum dimcv_adj_pass_id {
   DIMCV_ADJ_DENSITY = 0,
   DIMCV_ADJ_FACTOR = 1,
