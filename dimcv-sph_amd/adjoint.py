@@ -49,6 +49,7 @@ class StepTape:
         st.nbr = self.nbr.data_ptr()
         st.sample_idx = self.sample_idx.data_ptr()
         st.counters = self.counters.data_ptr()
+        st.grid_particles_num = self.gpn0.data_ptr()
         return st
 
 
@@ -71,6 +72,7 @@ def record_step(ps, sv, u_cdf) -> StepTape:
     T.fluid_list = ps.fluid_list.t.clone()
     T.nbr_count = ps.nbr_count.t.clone()
     T.nbr = ps.nbr.clone()
+    T.gpn0 = ps.grid_particles_num.t[:1].clone()        # cell 0's particles: slots [0, gpn0)
     sv.compute_densities()
     sv.compute_DFSPH_factor()
     T.rho, T.alpha0 = _clone(ps.density, n), _clone(ps.dfsph_factor, n)

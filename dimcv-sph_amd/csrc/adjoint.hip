@@ -7,11 +7,13 @@
 // and the |v_vort| < |v| update) keep the values they had in the forward.
 // Within that structure each pass is smooth and its VJP is exact.
 //
-// One thread per fluid-list entry i walks i's neighbour list (the forward's
-// visiting order); terms of the pair (i, j) that belong to j are added with
-// float atomics.  Every pass accumulates (+=) into its adjoint outputs.
-// Pair terms depend on x through r = x_i - x_j: r_bar goes to x_i and its
-// negative to x_j.
+// One thread per fluid-list entry i walks i's neighbour list.  Pair terms
+// depend on x through r = x_i - x_j: r_bar goes to x_i and its negative to
+// x_j.  The neighbour passes gather: i's thread evaluates each pair from both
+// ends (its own term and the one the pair sends to i), so only particles no
+// list contains scatter.  Every pass accumulates (+=) into its adjoint
+// outputs with float atomics (one per particle and component in the
+// neighbour passes).
 //
 // Kernel derivatives: grad W(r) = phi(|r|) r/|r| with phi = dW/ds
 // (sph_base.py:46-68), so the Hessian-vector product is
@@ -106,43 +108,85 @@ __device__ __forceinline__ float a_kernel(const dimcv_params& p, float s, float&
   FOR_NBR(k, e_##k, j)         \
   if (nb_fluid(e_##k))
 
+// Neighbour-pass VJPs in gather form.  The thread of particle i adds, for
+// each fluid neighbour j in i's list, both i's own term of the pair (i, j)
+// and the term that the pair (j, i) of j's list sends to i, so no pair term
+// is scattered.  The list test (|x_i - x_j| < h, j != i) is symmetric: j's
+// list holds i exactly when i's list holds j, except for the particles of
+// cell 0 (slots [0, gpn[0]) in cell order), which no list contains.  Such a
+// particle (vis = false) receives nothing and adds its terms for j to j's
+// outputs with atomics.  Pair identities used, with r = x_i - x_j and
+// g = grad W(r): grad W(-r) = -g, H(-r) = H(r), W, A and E are even, grad E
+// is odd.  Each thread adds its sums into the outputs with one atomic add
+// per component (a cell-0 particle's terms may reach any particle).
+#define ADJ_GATHER_ENTRY() \
+  ADJ_ENTRY();             \
+  const bool vis = i >= st.grid_particles_num[0]
+
+__device__ __forceinline__ v3 zero3() { return mk(0.0f, 0.0f, 0.0f); }
+
 // rho_i = rho0 (mV_i W(0) + sum_j mV_j W(|r_ij|))            (dimcv_sph.py:126-148)
-// in: g_in = rho_bar.  out: x_bar.
+// in: g_in = rho_bar.  out: x_bar += rho0 sum_j (mV_j rho_bar_i + mV_i rho_bar_j) gW_ij.
 __global__ void k_adj_density(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const float rb = a.g_in[i];
-  if (rb == 0.0f) return;
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 rbar = smul(p.density0 * st.m_V[j] * rb, cubic_dW(p, r));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  ADJ_GATHER_ENTRY();
+  const float ci = p.density0 * a.g_in[i];
+  const float cr = vis ? p.density0 * st.m_V[i] : 0.0f;
+  struct G {
+    v3 x;
+    float mv, rb;
+  };
+  v3 xb = zero3();
+  nbr_loop<8>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), st.m_V[j], a.g_in[j]}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 g = cubic_dW(p, vsub(xi, q.x));
+        acc3(xb, smul(ci * q.mv + cr * q.rb, g));
+        if (!vis) atomic_add3(a.x_bar, nb_slot(e), smul(-(ci * q.mv), g));
+      });
   atomic_add3(a.x_bar, i, xb);
 }
 
 // alpha_i = 1 / (sum_j |m_j gW_ij|^2 + |sum_j m_j gW_ij|^2), 0 below 1e-6
 // (dimcv_sph.py:86-124).  in: f0 = alpha (forward), g_in = alpha_bar.
-// out: x_bar.
-__global__ void k_adj_factor(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
+// out: x_bar.  With s_i = -alpha_bar_i alpha_i^2 and G_i = sum_j m_j gW_ij,
+// the pair (i, j) gives gW_ij the adjoint 2 m_j s_i (m_j gW_ij + G_i).  A
+// first pass tabulates G in slot order.
+__device__ __forceinline__ float factor_s(const dimcv_adj_args& a, int64_t i) {
   const float al = a.f0[i];
-  const float ab = a.g_in[i];
-  if (al == 0.0f || ab == 0.0f) return;
-  const float sbar = -ab * al * al;
-  v3 G = mk(0.0f, 0.0f, 0.0f);
+  return al == 0.0f ? 0.0f : -a.g_in[i] * al * al;
+}
+
+__global__ void k_adj_factor_sum(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound,
+                                 float* __restrict__ Gt) {
+  ADJ_ENTRY();
+  v3 G = zero3();
   FOR_FLUID_NBR(k, j) { acc3(G, smul(st.m[j], cubic_dW(p, vsub(xi, ld3(st.x, j))))); }
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const float mj = st.m[j];
-    const v3 gbar = vadd(smul(2.0f * mj * mj * sbar, g), smul(2.0f * mj * sbar, G));
-    const v3 rbar = hess_w(p, r, gbar);
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  st3(Gt, i, G);
+}
+
+__global__ void k_adj_factor(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound,
+                             const float* __restrict__ Gt) {
+  ADJ_GATHER_ENTRY();
+  const float si = factor_s(a, i), mi = st.m[i];
+  const v3 Gi = ld3(Gt, i);
+  struct G {
+    v3 x, G;
+    float m, s;
+  };
+  v3 xb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(Gt, j), st.m[j], factor_s(a, j)}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        // own: 2 m_j s_i (m_j g + G_i); pair (j, i), gW_ji = -g: 2 m_i s_j (G_j - m_i g)
+        const v3 own = smul(2.0f * q.m * si, vadd(smul(q.m, g), Gi));
+        const v3 rec = smul(2.0f * mi * (vis ? q.s : 0.0f), vsub(q.G, smul(mi, g)));
+        acc3(xb, hess_w(p, r, vsub(own, rec)));
+        if (!vis) atomic_add3(a.x_bar, nb_slot(e), vneg(hess_w(p, r, own)));
+      });
   atomic_add3(a.x_bar, i, xb);
 }
 
@@ -150,34 +194,42 @@ __global__ void k_adj_factor(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
 //   in: f0 = v, f1 = b (forward, gives the gate), g_in = b_bar.
 // mode 1: da_i = max(rho_i + dt sum_j m_j (v_i - v_j).gW_ij, rho0) (:470-493)
 //   in: f0 = v, f1 = da (forward), g_in = da_bar.  out also b0 = rho_bar.
-// out: x_bar, b1 = v_bar.
+// out: x_bar, b1 = v_bar.  With u_i the gated upstream (b_bar_i, or
+// dt da_bar_i) and c = mV (mode 0) or m (mode 1), the pair weight is
+// w = c_j u_i + c_i u_j: v_bar_i += w gW_ij, x_bar_i += H_ij w (v_i - v_j).
+__device__ __forceinline__ float div_u(const dimcv_params& p, const dimcv_adj_args& a, int64_t i) {
+  if (a.mode == 0) return a.f1[i] > 0.0f ? a.g_in[i] : 0.0f;
+  return a.f1[i] > p.density0 ? p.dt * a.g_in[i] : 0.0f;
+}
+
 __global__ void k_adj_div(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const float gb = a.g_in[i];
-  bool on;
-  float ub;
-  if (a.mode == 0) {
-    on = a.f1[i] > 0.0f;
-    ub = gb;
-  } else {
-    on = a.f1[i] > p.density0;
-    ub = p.dt * gb;
-    if (on && a.b0) atomicAdd(a.b0 + i, gb);
-  }
-  if (!on || ub == 0.0f) return;
+  ADJ_GATHER_ENTRY();
+  const float* __restrict__ cm = a.mode == 0 ? st.m_V : st.m;
+  if (a.mode == 1 && a.b0 && a.f1[i] > p.density0) atomicAdd(a.b0 + i, a.g_in[i]);
+  const float ui = div_u(p, a, i), ci = vis ? cm[i] : 0.0f;
   const v3 vi = ld3(a.f0, i);
-  v3 xb = mk(0.0f, 0.0f, 0.0f), vb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const float w = (a.mode == 0 ? st.m_V[j] : st.m[j]) * ub;
-    const v3 d = vsub(vi, ld3(a.f0, j));
-    acc3(vb, smul(w, g));
-    atomic_add3(a.b1, j, smul(-w, g));
-    const v3 rbar = hess_w(p, r, smul(w, d));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  struct G {
+    v3 x, v;
+    float c, u;
+  };
+  v3 xb = zero3(), vb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), cm[j], div_u(p, a, j)}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const v3 d = vsub(vi, q.v);
+        const float w = q.c * ui + ci * q.u;
+        acc3(vb, smul(w, g));
+        acc3(xb, hess_w(p, r, smul(w, d)));
+        if (!vis && ui != 0.0f) {
+          const float wo = q.c * ui;
+          const int64_t j = nb_slot(e);
+          atomic_add3(a.b1, j, smul(-wo, g));
+          atomic_add3(a.x_bar, j, vneg(hess_w(p, r, smul(wo, d))));
+        }
+      });
   atomic_add3(a.b1, i, vb);
   atomic_add3(a.x_bar, i, xb);
 }
@@ -187,137 +239,206 @@ __global__ void k_adj_div(dimcv_state st, dimcv_params p, dimcv_adj_args a, int3
 //   mode 0: k = b * alpha * rho   (f0 = b, f1 = alpha, f2 = rho)
 //   mode 1: k = (da - rho0) * alpha (f0 = da, f1 = alpha)
 // in: g_in = P_bar (the adjoint of the subtracted sum, i.e. -v_bar).
-// out: x_bar, b0 = kappa_bar.
+// out: x_bar, b0 = kappa_bar.  Per kept pair:
+//   kappa_bar_i += dt gW_ij.(m_j P_bar_i - m_i P_bar_j),
+//   x_bar_i += H_ij dt (k_i + k_j) (m_j P_bar_i - m_i P_bar_j).
 __device__ __forceinline__ float kappa_of(const dimcv_params& p, const dimcv_adj_args& a, int64_t i) {
   return a.mode == 0 ? a.f0[i] * a.f1[i] * a.f2[i] : (a.f0[i] - p.density0) * a.f1[i];
 }
 
 __global__ void k_adj_jacobi(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const v3 pb = ld3(a.g_in, i);
-  if (pb.x == 0.0f && pb.y == 0.0f && pb.z == 0.0f) return;
-  const float ki = kappa_of(p, a, i);
+  ADJ_GATHER_ENTRY();
+  const v3 pbi = ld3(a.g_in, i);
+  const float ki = kappa_of(p, a, i), mi = vis ? st.m[i] : 0.0f;
+  struct G {
+    v3 x, pb;
+    float m, k;
+  };
   float kb = 0.0f;
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const float ksum = ki + kappa_of(p, a, j);
-    if (!(fabsf(ksum) > p.m_eps)) continue;
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const float mj = st.m[j];
-    const float s = p.dt * mj * vdot(g, pb);
-    kb += s;
-    atomicAdd(a.b0 + j, s);
-    const v3 rbar = hess_w(p, r, smul(p.dt * ksum * mj, pb));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  v3 xb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.g_in, j), st.m[j], kappa_of(p, a, j)}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const float ksum = ki + q.k;
+        if (!(fabsf(ksum) > p.m_eps)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const float so = p.dt * q.m * vdot(g, pbi);
+        kb += so - p.dt * mi * vdot(g, q.pb);
+        acc3(xb, hess_w(p, r, smul(p.dt * ksum, vsub(smul(q.m, pbi), smul(mi, q.pb)))));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          atomicAdd(a.b0 + j, so);
+          atomic_add3(a.x_bar, j, vneg(hess_w(p, r, smul(p.dt * ksum * q.m, pbi))));
+        }
+      });
   atomicAdd(a.b0 + i, kb);
   atomic_add3(a.x_bar, i, xb);
 }
 
 // a_i = g + sum_j c mV_j ((v_i - v_j).r) / (|r|^2 + 0.01 h^2) gW_ij  (dimcv_sph.py:150-208)
-// in: f0 = v, g_in = a_bar.  out: x_bar, b1 = v_bar.
+// in: f0 = v, g_in = a_bar.  out: x_bar, b1 = v_bar.  With
+// Q = c (mV_j a_bar_i - mV_i a_bar_j), D = |r|^2 + 0.01 h^2, d = v_i - v_j and
+// t = Q.gW_ij / D: v_bar_i += t r, x_bar_i += t (d - 2 (d.r) r / D) + H_ij (d.r / D) Q.
 __global__ void k_adj_nonpressure(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const v3 ab = ld3(a.g_in, i);
-  if (ab.x == 0.0f && ab.y == 0.0f && ab.z == 0.0f) return;
+  ADJ_GATHER_ENTRY();
+  const v3 abi = ld3(a.g_in, i);
   const v3 vi = ld3(a.f0, i);
-  v3 xb = mk(0.0f, 0.0f, 0.0f), vb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const v3 d = vsub(vi, ld3(a.f0, j));
-    const float c = p.visc_coef * st.m_V[j];
-    const float D = vdot(r, r) + p.visc_h2;
-    const float dr = vdot(d, r);
-    const float beta = c * dr / D;
-    const float bb = vdot(ab, g);
-    const v3 dv = smul(bb * c / D, r);
-    acc3(vb, dv);
-    atomic_add3(a.b1, j, vneg(dv));
-    const v3 rbar = vadd(smul(bb * c / D, vsub(d, smul(2.0f * dr / D, r))), hess_w(p, r, smul(beta, ab)));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  const float ci = vis ? p.visc_coef * st.m_V[i] : 0.0f;
+  struct G {
+    v3 x, v, ab;
+    float mv;
+  };
+  v3 xb = zero3(), vb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const v3 d = vsub(vi, q.v);
+        const float cj = p.visc_coef * q.mv;
+        const float D = vdot(r, r) + p.visc_h2;
+        const float dr = vdot(d, r);
+        const v3 dd = vsub(d, smul(2.0f * dr / D, r));
+        const v3 Q = vsub(smul(cj, abi), smul(ci, q.ab));
+        const float t = vdot(Q, g) / D;
+        acc3(vb, smul(t, r));
+        acc3(xb, vadd(smul(t, dd), hess_w(p, r, smul(dr / D, Q))));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          const float to = cj * vdot(abi, g) / D;
+          atomic_add3(a.b1, j, smul(-to, r));
+          atomic_add3(a.x_bar, j, vneg(vadd(smul(to, dd), hess_w(p, r, smul(cj * dr / D, abi)))));
+        }
+      });
   atomic_add3(a.b1, i, vb);
   atomic_add3(a.x_bar, i, xb);
 }
 
 // C_i = sum_j mV_j (u_j - u_i) x gW_ij   (vorticity, dimcv_sph.py:228-248; the
 // curl of v* in compute_d_vorticity, :551-583)
-// in: f0 = u, g_in = C_bar.  out: x_bar, b1 = u_bar.
+// in: f0 = u, g_in = C_bar.  out: x_bar, b1 = u_bar.  With
+// Q = mV_j C_bar_i + mV_i C_bar_j: u_bar_i -= gW_ij x Q, x_bar_i += H_ij (Q x (u_j - u_i)).
 __global__ void k_adj_curl(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const v3 cb = ld3(a.g_in, i);
-  if (cb.x == 0.0f && cb.y == 0.0f && cb.z == 0.0f) return;
+  ADJ_GATHER_ENTRY();
+  const v3 cbi = ld3(a.g_in, i);
   const v3 ui = ld3(a.f0, i);
-  v3 xb = mk(0.0f, 0.0f, 0.0f), ubi = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const float mv = st.m_V[j];
-    const v3 d = vsub(ld3(a.f0, j), ui);
-    const v3 du = smul(mv, vcross(g, cb));
-    atomic_add3(a.b1, j, du);
-    ubi = vsub(ubi, du);
-    const v3 rbar = hess_w(p, r, smul(mv, vcross(cb, d)));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
-  atomic_add3(a.b1, i, ubi);
+  const float mvi = vis ? st.m_V[i] : 0.0f;
+  struct G {
+    v3 x, u, cb;
+    float mv;
+  };
+  v3 xb = zero3(), ub = zero3();
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const v3 d = vsub(q.u, ui);
+        const v3 Q = vadd(smul(q.mv, cbi), smul(mvi, q.cb));
+        ub = vsub(ub, vcross(g, Q));
+        acc3(xb, hess_w(p, r, vcross(Q, d)));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          atomic_add3(a.b1, j, smul(q.mv, vcross(g, cbi)));
+          atomic_add3(a.x_bar, j, vneg(hess_w(p, r, smul(q.mv, vcross(cbi, d)))));
+        }
+      });
+  atomic_add3(a.b1, i, ub);
   atomic_add3(a.x_bar, i, xb);
 }
 
+// M g and M^T d for a row-major 3x3 M.
+__device__ __forceinline__ v3 mat_vec(const float* M, v3 g) {
+  return mk(M[0] * g.x + M[1] * g.y + M[2] * g.z, M[3] * g.x + M[4] * g.y + M[5] * g.z,
+            M[6] * g.x + M[7] * g.y + M[8] * g.z);
+}
+__device__ __forceinline__ v3 mat_tvec(const float* M, v3 d) {
+  return mk(M[0] * d.x + M[3] * d.y + M[6] * d.z, M[1] * d.x + M[4] * d.y + M[7] * d.z,
+            M[2] * d.x + M[5] * d.y + M[8] * d.z);
+}
+
 // G_i[r][c] = sum_j mV_j (v_j - v_i)_r gW_ij_c   (dimcv_sph.py:250-262)
-// in: f0 = v, g_in = G_bar (9 per particle).  out: x_bar, b1 = v_bar.
+// in: f0 = v, g_in = G_bar (9 per particle).  out: x_bar, b1 = v_bar.  With
+// M = mV_j G_bar_i + mV_i G_bar_j: v_bar_i -= M gW_ij, x_bar_i += H_ij (M^T (v_j - v_i)).
 __global__ void k_adj_grad_v(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  float Gb[9];
-  bool any = false;
-  for (int q = 0; q < 9; ++q) {
-    Gb[q] = a.g_in[9 * i + q];
-    any |= Gb[q] != 0.0f;
-  }
-  if (!any) return;
+  ADJ_GATHER_ENTRY();
+  float Gi[9];
+#pragma unroll
+  for (int u = 0; u < 9; ++u) Gi[u] = a.g_in[9 * i + u];
   const v3 vi = ld3(a.f0, i);
-  v3 xb = mk(0.0f, 0.0f, 0.0f), vbi = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const v3 g = cubic_dW(p, r);
-    const float mv = st.m_V[j];
-    const v3 d = vsub(ld3(a.f0, j), vi);
-    const float gc[3] = {g.x, g.y, g.z}, ar[3] = {mv * d.x, mv * d.y, mv * d.z};
-    float abar[3], gbar[3];
-    for (int rr = 0; rr < 3; ++rr) abar[rr] = Gb[3 * rr] * gc[0] + Gb[3 * rr + 1] * gc[1] + Gb[3 * rr + 2] * gc[2];
-    for (int c = 0; c < 3; ++c) gbar[c] = Gb[c] * ar[0] + Gb[3 + c] * ar[1] + Gb[6 + c] * ar[2];
-    const v3 dv = mk(mv * abar[0], mv * abar[1], mv * abar[2]);
-    atomic_add3(a.b1, j, dv);
-    vbi = vsub(vbi, dv);
-    const v3 rbar = hess_w(p, r, mk(gbar[0], gbar[1], gbar[2]));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
-  atomic_add3(a.b1, i, vbi);
+  const float mvi = vis ? st.m_V[i] : 0.0f;
+  struct G {
+    v3 x, v;
+    float mv;
+    float gb[9];
+  };
+  v3 xb = zero3(), vb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn,
+      [&](int64_t j) {
+        G q;
+        q.x = ld3(st.x, j);
+        q.v = ld3(a.f0, j);
+        q.mv = st.m_V[j];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) q.gb[u] = a.g_in[9 * j + u];
+        return q;
+      },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const v3 d = vsub(q.v, vi);
+        float M[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) M[u] = q.mv * Gi[u] + mvi * q.gb[u];
+        vb = vsub(vb, mat_vec(M, g));
+        acc3(xb, hess_w(p, r, mat_tvec(M, d)));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          atomic_add3(a.b1, j, smul(q.mv, mat_vec(Gi, g)));
+          atomic_add3(a.x_bar, j, vneg(hess_w(p, r, smul(q.mv, mat_tvec(Gi, d)))));
+        }
+      });
+  atomic_add3(a.b1, i, vb);
   atomic_add3(a.x_bar, i, xb);
 }
 
 // w_i = mV_i d_i W(0) + sum_j mV_j d_j W(|r_ij|)   (dimcv_sph.py:660-677)
-// in: f0 = d_vort, g_in = w_bar.  out: x_bar, b1 = d_vort_bar.
+// in: f0 = d_vort, g_in = w_bar.  out: x_bar, b1 = d_vort_bar:
+//   d_bar_i += mV_i (W(0) w_bar_i + sum_j W(|r_ij|) w_bar_j),
+//   x_bar_i += sum_j (mV_j d_j.w_bar_i + mV_i d_i.w_bar_j) gW_ij.
 __global__ void k_adj_smooth(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const v3 wb = ld3(a.g_in, i);
-  if (wb.x == 0.0f && wb.y == 0.0f && wb.z == 0.0f) return;
-  atomic_add3(a.b1, i, smul(st.m_V[i] * cubic_W(p, 0.0f), wb));
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    const float mv = st.m_V[j];
-    atomic_add3(a.b1, j, smul(mv * cubic_W(p, vnorm(r)), wb));
-    const v3 rbar = smul(mv * vdot(ld3(a.f0, j), wb), cubic_dW(p, r));
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  ADJ_GATHER_ENTRY();
+  const v3 wbi = ld3(a.g_in, i);
+  const v3 di = ld3(a.f0, i);
+  const float mvi = st.m_V[i];
+  struct G {
+    v3 x, d, wb;
+    float mv;
+  };
+  v3 xb = zero3(), ws = smul(cubic_W(p, 0.0f), wbi);
+  nbr_loop<4>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r = vsub(xi, q.x);
+        const v3 g = cubic_dW(p, r);
+        const float W = cubic_W(p, vnorm(r));
+        const float own = q.mv * vdot(q.d, wbi);
+        if (vis) acc3(ws, smul(W, q.wb));
+        acc3(xb, smul(own + (vis ? mvi * vdot(di, q.wb) : 0.0f), g));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          atomic_add3(a.b1, j, smul(q.mv * W, wbi));
+          atomic_add3(a.x_bar, j, smul(-own, g));
+        }
+      });
+  atomic_add3(a.b1, i, smul(mvi, ws));
   atomic_add3(a.x_bar, i, xb);
 }
 
@@ -349,22 +470,34 @@ __global__ void k_adj_kvn(dimcv_state st, dimcv_params p, dimcv_adj_args a, int3
 }
 
 // kden_i = kvn_i E(0) + sum_j kvn_j E(r_ij)   (dimcv_sph.py:721-751)
-// in: f0 = kvn, g_in = kden_bar.  out: x_bar, b1 = kvn_bar.
+// in: f0 = kvn, g_in = kden_bar.  out: x_bar, b1 = kvn_bar:
+//   kvn_bar_i += E(0) kden_bar_i + sum_j E(r_ij) kden_bar_j,
+//   x_bar_i += sum_j (kvn_j kden_bar_i + kvn_i kden_bar_j) grad E(r_ij).
 __global__ void k_adj_kden(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  const float kb = a.g_in[i];
-  if (kb == 0.0f) return;
-  atomicAdd(a.b1 + i, epan_f(p, mk(0.0f, 0.0f, 0.0f), nullptr) * kb);
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    const v3 r = vsub(xi, ld3(st.x, j));
-    v3 ge;
-    const float E = epan_f(p, r, &ge);
-    atomicAdd(a.b1 + j, E * kb);
-    const v3 rbar = smul(a.f0[j] * kb, ge);
-    acc3(xb, rbar);
-    atomic_add3(a.x_bar, j, vneg(rbar));
-  }
+  ADJ_GATHER_ENTRY();
+  const float kbi = a.g_in[i];
+  const float kvi = vis ? a.f0[i] : 0.0f;
+  struct G {
+    v3 x;
+    float kv, kb;
+  };
+  float bs = epan_f(p, zero3(), nullptr) * kbi;
+  v3 xb = zero3();
+  nbr_loop<8>(
+      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), a.f0[j], a.g_in[j]}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        v3 ge;
+        const float E = epan_f(p, vsub(xi, q.x), &ge);
+        if (vis) bs += E * q.kb;
+        acc3(xb, smul(q.kv * kbi + kvi * q.kb, ge));
+        if (!vis) {
+          const int64_t j = nb_slot(e);
+          atomicAdd(a.b1 + j, E * kbi);
+          atomic_add3(a.x_bar, j, smul(-(q.kv * kbi), ge));
+        }
+      });
+  atomicAdd(a.b1 + i, bs);
   atomic_add3(a.x_bar, i, xb);
 }
 
@@ -382,29 +515,57 @@ __global__ void k_adj_gate(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
 //   vv_i = fac sum_j A(|r_xy|) (r_xy x w_j),  S_i = kden_sum - kden_i - sum_j kden_j.
 // in: f0 = w (d_vort_smoothed), g_in = u_bar (gate * v_bar), f1 = S_bar (from
 // the far-field target pass).  out: x_bar, b0 = w_bar, b1 = kden_bar.
+// Targets outside the vortex domain carry no upstream.  With u_i = fac u_bar_i
+// the pair (i, j) sends w_bar_j += A (u_i x r_xy), kden_bar_j -= S_bar_i and
+// x_bar_i += R(w_j, u_i) (x_bar_j -= the same), where
+// R(w, u) = A (w x u)_xy + ((r_xy x w).u) (A'(s) / s) r_xy, s = |r_xy|.
+__device__ __forceinline__ v3 near_rbar(v3 r, float s, float A, float dA, v3 w, v3 u) {
+  v3 rb = smul(A, vcross(w, u));
+  if (s > 0.0f) rb = vadd(rb, smul(vdot(vcross(r, w), u) * dA / s, r));
+  rb.z = 0.0f;
+  return rb;
+}
+
 __global__ void k_adj_near(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound) {
-  ADJ_ENTRY();
-  if (!in_vortex_domain(p, xi)) return;
-  const v3 ub = smul(p.vp_factor, ld3(a.g_in, i));
-  const float sb = a.f1 ? a.f1[i] : 0.0f;
-  if (sb != 0.0f) atomicAdd(a.b1 + i, -sb);
-  v3 xb = mk(0.0f, 0.0f, 0.0f);
-  FOR_FLUID_NBR(k, j) {
-    if (sb != 0.0f) atomicAdd(a.b1 + j, -sb);
-    const v3 r3 = vsub(xi, ld3(st.x, j));
-    const v3 r = mk(r3.x, r3.y, 0.0f);
-    const float s = vnorm(r);
-    float dA;
-    const float A = a_kernel(p, s, dA);
-    const v3 wj = ld3(a.f0, j);
-    atomic_add3(a.b0, j, smul(A, vcross(ub, r)));
-    const float tu = vdot(vcross(r, wj), ub);
-    v3 rb = smul(A, vcross(wj, ub));
-    if (s > 0.0f) rb = vadd(rb, smul(tu * dA / s, r));
-    rb.z = 0.0f;
-    acc3(xb, rb);
-    atomic_add3(a.x_bar, j, vneg(rb));
-  }
+  ADJ_GATHER_ENTRY();
+  const bool dom = in_vortex_domain(p, xi);
+  const v3 ubi = dom ? smul(p.vp_factor, ld3(a.g_in, i)) : zero3();
+  const float sbi = (dom && a.f1) ? a.f1[i] : 0.0f;
+  const v3 wi = ld3(a.f0, i);
+  struct G {
+    v3 x, w, ub;
+    float sb;
+  };
+  float kb = -sbi;
+  v3 xb = zero3(), wb = zero3();
+  nbr_loop<4>(
+      nl, ns, nn,
+      [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), a.f1 ? a.f1[j] : 0.0f}; },
+      [&](int, uint32_t e, const G& q) {
+        if (!nb_fluid(e)) return;
+        const v3 r3 = vsub(xi, q.x);
+        const v3 r = mk(r3.x, r3.y, 0.0f);
+        const float s = vnorm(r);
+        float dA;
+        const float A = a_kernel(p, s, dA);
+        const v3 ro = near_rbar(r, s, A, dA, q.w, ubi);
+        v3 rb = ro;
+        if (vis && in_vortex_domain(p, q.x)) {  // pair (j, i): r_xy -> -r_xy
+          const v3 uj = smul(p.vp_factor, q.ub);
+          kb -= q.sb;
+          wb = vsub(wb, smul(A, vcross(uj, r)));
+          rb = vsub(rb, near_rbar(r, s, A, dA, wi, uj));
+        }
+        acc3(xb, rb);
+        if (!vis && dom) {
+          const int64_t j = nb_slot(e);
+          if (sbi != 0.0f) atomicAdd(a.b1 + j, -sbi);
+          atomic_add3(a.b0, j, smul(A, vcross(ubi, r)));
+          atomic_add3(a.x_bar, j, vneg(ro));
+        }
+      });
+  if (kb != 0.0f) atomicAdd(a.b1 + i, kb);
+  atomic_add3(a.b0, i, wb);
   atomic_add3(a.x_bar, i, xb);
 }
 
@@ -554,7 +715,15 @@ extern "C" int dimcv_adj_pass(const dimcv_state* st, const dimcv_params* p, int3
   const dim3 g(nblocks(fluid_bound)), b(256);
   switch (pass) {
     case DIMCV_ADJ_DENSITY: hipLaunchKernelGGL(k_adj_density, g, b, 0, s, *st, *p, *a, fluid_bound); break;
-    case DIMCV_ADJ_FACTOR: hipLaunchKernelGGL(k_adj_factor, g, b, 0, s, *st, *p, *a, fluid_bound); break;
+    case DIMCV_ADJ_FACTOR: {
+      // G_i = sum_j m_j gW_ij in slot order, in the workspace's near-field region
+      Workspace ws;
+      int rc = bind_workspace(st, p, &ws, 0);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_adj_factor_sum, g, b, 0, s, *st, *p, *a, fluid_bound, ws.near3);
+      hipLaunchKernelGGL(k_adj_factor, g, b, 0, s, *st, *p, *a, fluid_bound, (const float*)ws.near3);
+      break;
+    }
     case DIMCV_ADJ_DIV: hipLaunchKernelGGL(k_adj_div, g, b, 0, s, *st, *p, *a, fluid_bound); break;
     case DIMCV_ADJ_JACOBI: hipLaunchKernelGGL(k_adj_jacobi, g, b, 0, s, *st, *p, *a, fluid_bound); break;
     case DIMCV_ADJ_NONPRESSURE: hipLaunchKernelGGL(k_adj_nonpressure, g, b, 0, s, *st, *p, *a, fluid_bound); break;
