@@ -23,6 +23,14 @@
 // kernel of vort2vel (dimcv_sph.py:652-657, 698-706).
 #include <algorithm>
 
+// The adjoint reproduces no reference bits (its sums are reordered and
+// atomic anyway), so a * b + c may contract to one fused multiply-add in this
+// file, the shared vector helpers included; the forward passes, compiled in
+// their own files, keep every operation separate.  Tests that must repeat a
+// forward decision bit for bit (the far-field r^2 > T) use __fmul_rn /
+// __fadd_rn, which never contract.
+#pragma clang fp contract(fast)
+
 #include "common.h"
 
 namespace dimcv {
@@ -588,55 +596,128 @@ __global__ void k_adj_pack_samples(dimcv_state st, dimcv_adj_args a, int32_t cap
 
 // Target side.  in: f0 = w, f1 = kden, g_in = u_bar (v_vort adjoint),
 // dscalar = kden_sum.  out: x_bar, b0 += S_bar, b2 = the target's weights
-// (vp_factor / added) S u_bar (0 for targets without samples).  One loop
-// counts the accepted samples, the second sums with u_bar and scales once.
-__global__ void k_adj_far_targets(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound, int32_t cap,
-                                  const float4* __restrict__ tab) {
-  ADJ_ENTRY();
-  st3(a.b2, i, mk(0.0f, 0.0f, 0.0f));
-  if (!in_vortex_domain(p, xi)) return;
-  const v3 ub = ld3(a.g_in, i);
-  float kns = a.f1[i];
-  FOR_FLUID_NBR(k, j) { kns += a.f1[j]; }
-  const float S = (float)(a.dscalar - (double)kns);
-  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
-  const float T = p.far_r2_max_rejected;
-  auto r2_of = [&](const float4& s4, v3& r) {
-    r = mk(xi.x - s4.x, xi.y - s4.y, xi.z - s4.z);
-    return __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
-  };
-  int added = 0;
-#pragma unroll 4
-  for (int q = 0; q < M; ++q) {
-    v3 r;
-    added += r2_of(tab[2 * q], r) > T ? 1 : 0;
-  }
-  if (added == 0) return;
-  const float coef = p.vp_factor / (float)added;
-  const float sc = coef * S;
-  st3(a.b2, i, smul(sc, ub));
-  v3 sum = mk(0.0f, 0.0f, 0.0f), xbu = mk(0.0f, 0.0f, 0.0f);
-#pragma unroll 2
-  for (int q = 0; q < M; ++q) {
-    const float4 s4 = tab[2 * q], w4 = tab[2 * q + 1];
-    v3 r;
-    const float r2 = r2_of(s4, r);
-    if (!(r2 > T)) continue;
-    const float A = __builtin_amdgcn_rcpf(r2);
-    const float c = s4.w;
-    const v3 t = mk(r.y * w4.z, -(r.x * w4.z), r.x * w4.y - r.y * w4.x);  // r_xy x w_k
-    sum = vadd(sum, smul(A * c, t));
-    // d/dx_i of A (r_xy x w_k).u: A (w_k x u)_xy + (t.u) (dA/ds / s) r
-    const float kk = -2.0f * A * A * vdot(t, ub);
-    const float cx = w4.y * ub.z - w4.z * ub.y, cy = w4.z * ub.x - w4.x * ub.z;
-    xbu = vadd(xbu, smul(c, mk(A * cx + kk * r.x, A * cy + kk * r.y, kk * r.z)));
-  }
-  a.b0[i] += coef * vdot(ub, sum);
-  atomic_add3(a.x_bar, i, smul(sc, xbu));
+// (vp_factor / added) S u_bar (0 for targets without samples).
+// Per target the pass needs the accepted-sample count and two sums over the
+// accepted samples, unscaled: sum A c (r_xy x w) and the gradient sum
+// c d/dx_i [A (r_xy x w).u_bar]; all are linear, so the sample range can be
+// split over blockIdx.y (partials in split order, added by the finish pass).
+// d/dx_i of A (r_xy x w_k).u is A (w_k x u)_xy + (t.u) (dA/ds / s) r with
+// t = r_xy x w_k; the first term is summed as W = sum A c w_k and crossed
+// with u once per thread (far_t_close).
+struct FarT {
+  float n;
+  v3 sum, xbu, W;
+};
+
+// One sample (s4 = (x, c), w4 = (w, slot)) for target xi, u2 = -2 u_bar; a
+// rejected sample (r^2 <= T, the forward's test) contributes 0.
+__device__ __forceinline__ void far_t_pair(FarT& acc, v3 xi, v3 u2, float T, const float4& s4, const float4& w4) {
+  const v3 r = mk(xi.x - s4.x, xi.y - s4.y, xi.z - s4.z);
+  const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
+  const bool ok = r2 > T;
+  const float A = ok ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+  acc.n += ok ? 1.0f : 0.0f;
+  const float Ac = A * s4.w;
+  const v3 w = mk(w4.x, w4.y, w4.z);
+  const v3 t = mk(r.y * w.z, -(r.x * w.z), r.x * w.y - r.y * w.x);  // r_xy x w_k
+  acc.sum = vadd(acc.sum, smul(Ac, t));
+  acc.W = vadd(acc.W, smul(Ac, w));
+  acc.xbu = vadd(acc.xbu, smul(Ac * A * vdot(t, u2), r));  // c (dA/ds / s) (t.u) r, dA/ds / s = -2 A^2
 }
 
-// Targets packed for the source side: (x_i, -) and (weights_i, -) per fluid
-// entry; entries without weight are all zero (skipped).
+// Adds (W x u)_xy to the gradient sum (u = u_bar, not doubled).
+__device__ __forceinline__ void far_t_close(FarT& acc, v3 ub) {
+  acc.xbu.x += acc.W.y * ub.z - acc.W.z * ub.y;
+  acc.xbu.y += acc.W.z * ub.x - acc.W.x * ub.z;
+  acc.W = zero3();
+}
+
+// Samples [q0, q1) of the packed table, four per trip, the next four loaded
+// (scalar loads) while the current four are summed.
+__device__ __forceinline__ void far_t_range(FarT& acc, const float4* __restrict__ tab, int q0, int q1, v3 xi, v3 u2,
+                                            float T) {
+  int q = q0;
+  if (q + 3 < q1) {
+    float4 s0 = tab[2 * q], w0 = tab[2 * q + 1], s1 = tab[2 * q + 2], w1 = tab[2 * q + 3];
+    float4 s2 = tab[2 * q + 4], w2 = tab[2 * q + 5], s3 = tab[2 * q + 6], w3 = tab[2 * q + 7];
+    for (; q + 3 < q1; q += 4) {
+      const int nx = q + 7 < q1 ? q + 4 : q;  // in-range prefetch index
+      const float4 ns0 = tab[2 * nx], nw0 = tab[2 * nx + 1], ns1 = tab[2 * nx + 2], nw1 = tab[2 * nx + 3];
+      const float4 ns2 = tab[2 * nx + 4], nw2 = tab[2 * nx + 5], ns3 = tab[2 * nx + 6], nw3 = tab[2 * nx + 7];
+      far_t_pair(acc, xi, u2, T, s0, w0);
+      far_t_pair(acc, xi, u2, T, s1, w1);
+      far_t_pair(acc, xi, u2, T, s2, w2);
+      far_t_pair(acc, xi, u2, T, s3, w3);
+      s0 = ns0; w0 = nw0; s1 = ns1; w1 = nw1;
+      s2 = ns2; w2 = nw2; s3 = ns3; w3 = nw3;
+    }
+  }
+  for (; q < q1; ++q) far_t_pair(acc, xi, u2, T, tab[2 * q], tab[2 * q + 1]);
+}
+
+// Scaling of target i's sums: S_i = kden_sum - kden_i - sum_j kden_j (the
+// forward's order), coef = vp_factor / added.
+__device__ __forceinline__ void far_t_finish(const dimcv_params& p, const dimcv_adj_args& a, int64_t i,
+                                             const uint32_t* __restrict__ nl, int64_t ns, int nn, v3 ub,
+                                             const FarT& acc) {
+  if (acc.n == 0.0f) return;
+  float kns = a.f1[i];
+  nbr_loop<8>(
+      nl, ns, nn, [&](int64_t j) { return a.f1[j]; },
+      [&](int, uint32_t e, float kj) {
+        if (nb_fluid(e)) kns += kj;
+      });
+  const float S = (float)(a.dscalar - (double)kns);
+  const float coef = p.vp_factor / acc.n;
+  const float sc = coef * S;
+  st3(a.b2, i, smul(sc, ub));
+  a.b0[i] += coef * vdot(ub, acc.sum);
+  atomic_add3(a.x_bar, i, smul(sc, acc.xbu));
+}
+
+// Split y = blockIdx.y of `splits` sums samples [y chunk, (y + 1) chunk);
+// with one split it finishes the target itself, otherwise it stores its
+// partial (sum, n) (xbu, 0) at part[2 (y bound + f)].
+__global__ void __launch_bounds__(256) k_adj_far_targets(dimcv_state st, dimcv_params p, dimcv_adj_args a,
+                                                         int32_t bound, int32_t cap, const float4* __restrict__ tab,
+                                                         int32_t splits, float4* __restrict__ part) {
+  ADJ_ENTRY();
+  if (splits == 1) st3(a.b2, i, zero3());
+  if (!in_vortex_domain(p, xi)) return;
+  const v3 ub = ld3(a.g_in, i);
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
+  const int chunk = (M + splits - 1) / splits;
+  const int q0 = min(M, (int)blockIdx.y * chunk), q1 = min(M, q0 + chunk);
+  FarT acc{0.0f, zero3(), zero3(), zero3()};
+  far_t_range(acc, tab, q0, q1, xi, smul(-2.0f, ub), p.far_r2_max_rejected);
+  far_t_close(acc, ub);
+  if (splits == 1) {
+    far_t_finish(p, a, i, nl, ns, nn, ub, acc);
+  } else {
+    const int64_t o = 2 * ((int64_t)blockIdx.y * bound + f);
+    part[o] = make_float4(acc.sum.x, acc.sum.y, acc.sum.z, acc.n);
+    part[o + 1] = make_float4(acc.xbu.x, acc.xbu.y, acc.xbu.z, 0.0f);
+  }
+}
+
+__global__ void k_adj_far_targets_finish(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound,
+                                         int32_t splits, const float4* __restrict__ part) {
+  ADJ_ENTRY();
+  st3(a.b2, i, zero3());
+  if (!in_vortex_domain(p, xi)) return;
+  FarT acc{0.0f, zero3(), zero3(), zero3()};
+  for (int y = 0; y < splits; ++y) {
+    const int64_t o = 2 * ((int64_t)y * bound + f);
+    const float4 u = part[o], v = part[o + 1];
+    acc.n += u.w;
+    acc.sum = vadd(acc.sum, mk(u.x, u.y, u.z));
+    acc.xbu = vadd(acc.xbu, mk(v.x, v.y, v.z));
+  }
+  far_t_finish(p, a, i, nl, ns, nn, ld3(a.g_in, i), acc);
+}
+
+// Targets packed for the source side: (x_i, -) and (weights_i, flag) per
+// fluid entry; entries without weight are all zero.
 __global__ void k_adj_pack_targets(dimcv_state st, dimcv_adj_args a, int32_t bound, float4* __restrict__ tab) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nf = min((int64_t)bound, (int64_t)st.counters[DIMCV_C_FLUID_LIST]);
@@ -655,45 +736,121 @@ __global__ void k_adj_pack_targets(dimcv_state st, dimcv_adj_args a, int32_t bou
 }
 
 // Source side: for sample k the sums over targets of w_bar_k, kden_bar_k
-// (through c_k) and x_bar_k.  One lane per sample, targets read from their
-// packed table (scalar loads) in blockIdx.y chunks; one set of atomics per
-// lane.  in: f0 = w, f1 = kden, f2 = per-target weights (b2 of the target
-// pass), mode = sample bound.  out: x_bar, b0 = w_bar, b1 = kden_bar.
-__global__ void k_adj_far_sources(dimcv_state st, dimcv_params p, dimcv_adj_args a, int32_t bound,
-                                  const float4* __restrict__ stab, const float4* __restrict__ ttab, int32_t cap) {
+// (through c_k) and x_bar_k.  One lane per sample; targets are read from
+// their packed table (scalar loads, the next four loaded while four are
+// summed) in blockIdx.y chunks.  A target without weight contributes 0;
+// a group of four such targets is skipped.  Each (chunk, sample) stores its
+// partial (w_bar, c sum) (x_bar, 0) at part[2 (y cap + q)]; the finish pass
+// adds them in chunk order.  in: f0 = w, f1 = kden, f2 = per-target weights
+// (b2 of the target pass), mode = sample bound.  out: x_bar, b0 = w_bar,
+// b1 = kden_bar.
+// Per lane, unscaled by c_k: wbar = sum A (w x r_xy), cbar = sum A t.w,
+// K = sum A^2 (t.w) r and Wt = sum A w (w = the target's weights,
+// t = r_xy x w_k); far_s_close forms the lane's (w_bar, x_bar) from them.
+struct FarS {
+  v3 wbar, K, Wt;
+  float cbar;
+};
+
+__device__ __forceinline__ void far_s_pair(FarS& acc, v3 xs, v3 wk, float T, const float4& x4, const float4& w4) {
+  const v3 r = mk(x4.x - xs.x, x4.y - xs.y, x4.z - xs.z);
+  const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
+  const float A = r2 > T ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+  const v3 w = mk(w4.x, w4.y, w4.z);
+  const v3 t = mk(r.y * wk.z, -(r.x * wk.z), r.x * wk.y - r.y * wk.x);  // r_xy x w_k
+  const float tw = vdot(t, w);
+  acc.wbar = vadd(acc.wbar, smul(A, mk(-(w.z * r.y), w.z * r.x, w.x * r.y - w.y * r.x)));  // w x r_xy
+  acc.cbar += A * tw;
+  acc.K = vadd(acc.K, smul(A * A * tw, r));
+  acc.Wt = vadd(acc.Wt, smul(A, w));
+}
+
+// x_bar_k = -c sum [A (w_k x w)_xy + (dA/ds / s) (t.w) r], dA/ds / s = -2 A^2.
+__device__ __forceinline__ void far_s_close(const FarS& acc, v3 wk, float c, float4& u, float4& v) {
+  const float xx = wk.y * acc.Wt.z - wk.z * acc.Wt.y - 2.0f * acc.K.x;
+  const float xy = wk.z * acc.Wt.x - wk.x * acc.Wt.z - 2.0f * acc.K.y;
+  u = make_float4(c * acc.wbar.x, c * acc.wbar.y, c * acc.wbar.z, acc.cbar);
+  v = make_float4(-c * xx, -c * xy, 2.0f * c * acc.K.z, 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_adj_far_sources(dimcv_state st, dimcv_params p, dimcv_adj_args a,
+                                                         int32_t bound, const float4* __restrict__ stab,
+                                                         const float4* __restrict__ ttab, int32_t cap,
+                                                         float4* __restrict__ part) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
   const int64_t nf = min((int64_t)bound, (int64_t)st.counters[DIMCV_C_FLUID_LIST]);
   const int64_t chunk = (nf + gridDim.y - 1) / gridDim.y;
-  const int64_t f0 = (int64_t)blockIdx.y * chunk, f1 = min(f0 + chunk, nf);
+  const int64_t f0 = min(nf, (int64_t)blockIdx.y * chunk), f1 = min(f0 + chunk, nf);
   if (q >= M) return;
   const float4 s4 = stab[2 * q], k4 = stab[2 * q + 1];
   const v3 xs = mk(s4.x, s4.y, s4.z), wk = mk(k4.x, k4.y, k4.z);
   const float c = s4.w;
-  const int64_t sk = __float_as_int(k4.w);
   const float T = p.far_r2_max_rejected;
-  v3 wbar = mk(0.0f, 0.0f, 0.0f), xb = mk(0.0f, 0.0f, 0.0f);
-  float cbar = 0.0f;
-#pragma unroll 2
-  for (int64_t f = f0; f < f1; ++f) {
-    const float4 x4 = ttab[2 * f], w4 = ttab[2 * f + 1];
-    if (w4.w == 0.0f) continue;  // uniform: target without weight
-    const v3 r = mk(x4.x - xs.x, x4.y - xs.y, x4.z - xs.z);
-    const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(r.x, r.x), __fmul_rn(r.y, r.y)), __fmul_rn(r.z, r.z));
-    if (!(r2 > T)) continue;
-    const float A = __builtin_amdgcn_rcpf(r2);
-    const v3 t = mk(r.y * wk.z, -(r.x * wk.z), r.x * wk.y - r.y * wk.x);  // r_xy x w_k
-    const float tw = t.x * w4.x + t.y * w4.y + t.z * w4.z;
-    // w x r_xy
-    wbar = vadd(wbar, smul(A * c, mk(-(w4.z * r.y), w4.z * r.x, w4.x * r.y - w4.y * r.x)));
-    cbar += A * tw;
-    const float kk = -2.0f * A * A * tw;
-    const float cx = wk.y * w4.z - wk.z * w4.y, cy = wk.z * w4.x - wk.x * w4.z;
-    xb = vsub(xb, smul(c, mk(A * cx + kk * r.x, A * cy + kk * r.y, kk * r.z)));
+  FarS acc{zero3(), zero3(), zero3(), 0.0f};
+  int64_t t = f0;
+  if (t + 3 < f1) {
+    float4 x0 = ttab[2 * t], w0 = ttab[2 * t + 1], x1 = ttab[2 * t + 2], w1 = ttab[2 * t + 3];
+    float4 x2 = ttab[2 * t + 4], w2 = ttab[2 * t + 5], x3 = ttab[2 * t + 6], w3 = ttab[2 * t + 7];
+    for (; t + 3 < f1; t += 4) {
+      const int64_t nx = t + 7 < f1 ? t + 4 : t;  // in-range prefetch index
+      const float4 nx0 = ttab[2 * nx], nw0 = ttab[2 * nx + 1], nx1 = ttab[2 * nx + 2], nw1 = ttab[2 * nx + 3];
+      const float4 nx2 = ttab[2 * nx + 4], nw2 = ttab[2 * nx + 5], nx3 = ttab[2 * nx + 6], nw3 = ttab[2 * nx + 7];
+      if (w0.w + w1.w + w2.w + w3.w != 0.0f) {  // uniform
+        far_s_pair(acc, xs, wk, T, x0, w0);
+        far_s_pair(acc, xs, wk, T, x1, w1);
+        far_s_pair(acc, xs, wk, T, x2, w2);
+        far_s_pair(acc, xs, wk, T, x3, w3);
+      }
+      x0 = nx0; w0 = nw0; x1 = nx1; w1 = nw1;
+      x2 = nx2; w2 = nw2; x3 = nx3; w3 = nw3;
+    }
   }
-  atomic_add3(a.b0, sk, wbar);
-  atomicAdd(a.b1 + sk, -c * c * cbar);
-  atomic_add3(a.x_bar, sk, xb);
+  for (; t < f1; ++t) far_s_pair(acc, xs, wk, T, ttab[2 * t], ttab[2 * t + 1]);
+  const int64_t o = 2 * ((int64_t)blockIdx.y * cap + q);
+  far_s_close(acc, wk, c, part[o], part[o + 1]);
+}
+
+// Sum of the chunk partials: block of 1024 threads per 64 samples, wave w
+// adding chunks w, w + 16, ...; the 16 wave sums are added in wave order.
+// Several samples may be the same particle, hence the atomics.
+__global__ void __launch_bounds__(1024) k_adj_far_sources_finish(dimcv_state st, dimcv_adj_args a, int32_t cap,
+                                                                 int32_t chunks, const float4* __restrict__ stab,
+                                                                 const float4* __restrict__ part) {
+  __shared__ float red[16][7][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
+  const int M = min(st.counters[DIMCV_C_NUM_SAMPLES], cap);
+  float s[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (q < M) {
+#pragma unroll 4
+    for (int y = w; y < chunks; y += 16) {
+      const int64_t o = 2 * ((int64_t)y * cap + q);
+      const float4 u = part[o], v = part[o + 1];
+      s[0] += u.x;
+      s[1] += u.y;
+      s[2] += u.z;
+      s[3] += u.w;
+      s[4] += v.x;
+      s[5] += v.y;
+      s[6] += v.z;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) red[w][k][lane] = s[k];
+  __syncthreads();
+  if (w != 0 || q >= M) return;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    float t = red[0][k][lane];
+    for (int u = 1; u < 16; ++u) t += red[u][k][lane];
+    s[k] = t;
+  }
+  const float c = stab[2 * q].w;
+  const int64_t sk = __float_as_int(stab[2 * q + 1].w);
+  atomic_add3(a.b0, sk, mk(s[0], s[1], s[2]));
+  atomicAdd(a.b1 + sk, -c * c * s[3]);
+  atomic_add3(a.x_bar, sk, mk(s[4], s[5], s[6]));
 }
 
 }  // namespace dimcv
@@ -736,27 +893,43 @@ extern "C" int dimcv_adj_pass(const dimcv_state* st, const dimcv_params* p, int3
     case DIMCV_ADJ_NEAR: hipLaunchKernelGGL(k_adj_near, g, b, 0, s, *st, *p, *a, fluid_bound); break;
     case DIMCV_ADJ_FAR_TARGETS:
     case DIMCV_ADJ_FAR_SOURCES: {
-      // a->mode bounds the sample count.  Sample table: the far-field
-      // partials region; target table: the sample region (n_max entries).
+      // a->mode bounds the sample count M.  The far-field partials region
+      // holds the sample table (M entries of 32 bytes) followed by the
+      // split partials (32 bytes each); the target table of the source side
+      // goes to the sample region (n_max entries).
       if (a->mode <= 0) break;
       Workspace ws;
       int rc = bind_workspace(st, p, &ws, 0);
       if (rc) return rc;
-      if ((int64_t)a->mode > ws.far_partial_cap || (int64_t)fluid_bound > ws.max_samples) {
+      const int64_t M = a->mode, room = ws.far_partial_cap - M;
+      if (room < M || (int64_t)fluid_bound > ws.max_samples) {
         set_error("dimcv_adj_pass: far-field tables exceed the workspace (%d samples, %d targets)", a->mode,
                   fluid_bound);
         return DIMCV_ERR_WORKSPACE;
       }
       float4* stab = (float4*)ws.far_partials;
+      float4* part = stab + 2 * M;
       hipLaunchKernelGGL(k_adj_pack_samples, dim3(nblocks(a->mode)), b, 0, s, *st, *a, a->mode, stab);
       if (pass == DIMCV_ADJ_FAR_TARGETS) {
-        hipLaunchKernelGGL(k_adj_far_targets, g, b, 0, s, *st, *p, *a, fluid_bound, a->mode, (const float4*)stab);
+        // enough splits for ~8192 waves, at least 64 samples each
+        const int64_t waves = (fluid_bound + 63) / 64;
+        const int64_t sp = std::max<int64_t>(
+            1, std::min<int64_t>({16, (8192 + waves - 1) / waves, M / 64, room / fluid_bound}));
+        hipLaunchKernelGGL(k_adj_far_targets, dim3(g.x, (unsigned)sp), b, 0, s, *st, *p, *a, fluid_bound, a->mode,
+                           (const float4*)stab, (int32_t)sp, part);
+        if (sp > 1)
+          hipLaunchKernelGGL(k_adj_far_targets_finish, g, b, 0, s, *st, *p, *a, fluid_bound, (int32_t)sp,
+                             (const float4*)part);
       } else {
         hipLaunchKernelGGL(k_adj_pack_targets, g, b, 0, s, *st, *a, fluid_bound, ws.samples);
-        // targets split into chunks of ~256 over blockIdx.y
-        const unsigned ych = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (fluid_bound + 255) / 256));
-        hipLaunchKernelGGL(k_adj_far_sources, dim3(nblocks(a->mode), ych), b, 0, s, *st, *p, *a, fluid_bound,
-                           (const float4*)stab, (const float4*)ws.samples, a->mode);
+        // target chunks over blockIdx.y: ~4096 waves, at least 64 targets each
+        const int64_t sw = (M + 63) / 64;
+        const int64_t ych = std::max<int64_t>(
+            1, std::min<int64_t>({1024, (4096 + sw - 1) / sw, fluid_bound / 64, room / M}));
+        hipLaunchKernelGGL(k_adj_far_sources, dim3(nblocks(a->mode), (unsigned)ych), b, 0, s, *st, *p, *a,
+                           fluid_bound, (const float4*)stab, (const float4*)ws.samples, a->mode, part);
+        hipLaunchKernelGGL(k_adj_far_sources_finish, dim3((unsigned)((M + 63) / 64)), dim3(1024), 0, s, *st, *a,
+                           a->mode, (int32_t)ych, (const float4*)stab, (const float4*)part);
       }
       break;
     }
