@@ -97,6 +97,7 @@ def run_gpu(args):
     res = {
         "n_fluid": n_fluid, "sec": sec, "far_ms": far_ms, "achieved_tflops": achieved,
         "pairs": pairs, "kv": float(np.mean(kv)), "kp": float(np.mean(kp)), "m": m,
+        "n_cells": int(ps.grid_num[0]) * int(ps.grid_num[1]) * int(ps.grid_num[2]),
     }
     if args.traffic_probe:
         # a plain copy of known size (64 Mi f32 read, 64 Mi f32 written): the
@@ -297,6 +298,25 @@ def run_adjoint(steps=32, checkpoint=8):
     return out
 
 
+def grid_sph_rate(res, ms_per_step):
+    """Algorithmic HBM bytes of the grid and SPH passes per step (each
+    particle's fields read and written once per pass):
+    B = N (833 + 95 k_v + 99 k_p) + 8 N_c (12 + 2 (k_v + k_p)), over the
+    step's time outside the far-field kernel."""
+    n, nc, kv, kp = res["n_fluid"], res["n_cells"], res["kv"], res["kp"]
+    b = n * (833 + 95 * kv + 99 * kp) + 8 * nc * (12 + 2 * (kv + kp))
+    ms = ms_per_step - res["far_ms"]
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes_per_step": int(round(b)), "ms_per_step": round(ms, 4),
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "passes": "the step outside the far-field kernel: grid build and sort, neighbour lists, density, "
+                      "DFSPH factor and both Jacobi solves, non-pressure forces, vorticity, grad v, d_vort, "
+                      "smoothing, KVN, AKDE, CDF sampling, near field, life time, predict, advect",
+            "note": "bytes counted once per particle per pass (k_v, k_p the mean iteration counts); the "
+                    "neighbour passes re-read neighbour fields through L2 and are gather-latency bound"}
+
+
 def run_cpu_baseline(info, steps=1):
     """The oracle on a bounded sample of the 1M workload: one step."""
     import copy
@@ -386,6 +406,7 @@ def main():
                              "instructions per SIMD quad-cycle: issue_ceiling TFLOP/s at 24 flop per pair. A "
                              "matrix-core form (S = W Q on v_mfma_f32_16x16x4_f32) measured slower: no VALU/MFMA "
                              "co-execution (SQ_VALU_MFMA_COEXEC_CYCLES = 0) and 7 of 16 columns used"},
+        "grid_sph_passes": grid_sph_rate(res, res["sec"] * 1e3 / args.steps),
         "cpu_baseline": cpu,
         "mc_vs_direct": direct,
         "batching": batching,

@@ -474,7 +474,10 @@ void dimcv_batch_graph_destroy(dimcv_batch_graph* g);
  * pass over the fluid list of `st` (the forward's neighbour lists and, for the
  * far field, its samples), reading forward values and the upstream adjoint
  * and ADDING into the adjoint outputs.  x_bar is the adjoint of the positions
- * x of `st`.  Open single-scene systems only.
+ * x of `st`.  Open single-scene systems only.  The neighbour passes also read
+ * grid_particles_num[0] of the grid build that made the lists (cell 0's
+ * particles, which no list contains); FACTOR uses the workspace's near-field
+ * region and the far-field passes its far-field and sample regions as scratch.
  *
  *  pass                 reference          f0 / f1 / f2                  g_in        outputs
  *  DENSITY              dimcv_sph.py:126   -                             rho_bar     x_bar
