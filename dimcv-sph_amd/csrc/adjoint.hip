@@ -108,7 +108,7 @@ __device__ __forceinline__ float a_kernel(const dimcv_params& p, float s, float&
   const int64_t i = st.fluid_list[f];                                  \
   if (i < 0) return;                                                   \
   const int nn = min(st.nbr_count[f], p.nbr_kmax);                     \
-  const uint32_t* __restrict__ nl = st.nbr + f;                        \
+  const NbrList nl(st.nbr, f);                        \
   const int64_t ns = p.n_max;                                          \
   const v3 xi = ld3(st.x, i)
 
@@ -145,7 +145,7 @@ __global__ void k_adj_density(dimcv_state st, dimcv_params p, dimcv_adj_args a, 
   };
   v3 xb = zero3();
   nbr_loop<8>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), st.m_V[j], a.g_in[j]}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld1s(st.m_V, j), ld1s(a.g_in, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 g = cubic_dW(p, vsub(xi, q.x));
@@ -184,7 +184,7 @@ __global__ void k_adj_factor(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
   };
   v3 xb = zero3();
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(Gt, j), st.m[j], factor_s(a, j)}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(Gt, j), ld1s(st.m, j), factor_s(a, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r = vsub(xi, q.x);
@@ -222,7 +222,7 @@ __global__ void k_adj_div(dimcv_state st, dimcv_params p, dimcv_adj_args a, int3
   };
   v3 xb = zero3(), vb = zero3();
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), cm[j], div_u(p, a, j)}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.f0, j), ld1s(cm, j), div_u(p, a, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r = vsub(xi, q.x);
@@ -265,7 +265,7 @@ __global__ void k_adj_jacobi(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
   float kb = 0.0f;
   v3 xb = zero3();
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.g_in, j), st.m[j], kappa_of(p, a, j)}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.g_in, j), ld1s(st.m, j), kappa_of(p, a, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const float ksum = ki + q.k;
@@ -300,7 +300,7 @@ __global__ void k_adj_nonpressure(dimcv_state st, dimcv_params p, dimcv_adj_args
   };
   v3 xb = zero3(), vb = zero3();
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.f0, j), ld3s(a.g_in, j), ld1s(st.m_V, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r = vsub(xi, q.x);
@@ -340,7 +340,7 @@ __global__ void k_adj_curl(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
   };
   v3 xb = zero3(), ub = zero3();
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.f0, j), ld3s(a.g_in, j), ld1s(st.m_V, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r = vsub(xi, q.x);
@@ -387,11 +387,11 @@ __global__ void k_adj_grad_v(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
   v3 xb = zero3(), vb = zero3();
   nbr_loop<4>(
       nl, ns, nn,
-      [&](int64_t j) {
+      [&](uint32_t j) {
         G q;
-        q.x = ld3(st.x, j);
-        q.v = ld3(a.f0, j);
-        q.mv = st.m_V[j];
+        q.x = ld3s(st.x, j);
+        q.v = ld3s(a.f0, j);
+        q.mv = ld1s(st.m_V, j);
 #pragma unroll
         for (int u = 0; u < 9; ++u) q.gb[u] = a.g_in[9 * j + u];
         return q;
@@ -431,7 +431,7 @@ __global__ void k_adj_smooth(dimcv_state st, dimcv_params p, dimcv_adj_args a, i
   };
   v3 xb = zero3(), ws = smul(cubic_W(p, 0.0f), wbi);
   nbr_loop<4>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), st.m_V[j]}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.f0, j), ld3s(a.g_in, j), ld1s(st.m_V, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r = vsub(xi, q.x);
@@ -492,7 +492,7 @@ __global__ void k_adj_kden(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
   float bs = epan_f(p, zero3(), nullptr) * kbi;
   v3 xb = zero3();
   nbr_loop<8>(
-      nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), a.f0[j], a.g_in[j]}; },
+      nl, ns, nn, [&](uint32_t j) { return G{ld3s(st.x, j), ld1s(a.f0, j), ld1s(a.g_in, j)}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         v3 ge;
@@ -548,7 +548,7 @@ __global__ void k_adj_near(dimcv_state st, dimcv_params p, dimcv_adj_args a, int
   v3 xb = zero3(), wb = zero3();
   nbr_loop<4>(
       nl, ns, nn,
-      [&](int64_t j) { return G{ld3(st.x, j), ld3(a.f0, j), ld3(a.g_in, j), a.f1 ? a.f1[j] : 0.0f}; },
+      [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(a.f0, j), ld3s(a.g_in, j), a.f1 ? ld1s(a.f1, j) : 0.0f}; },
       [&](int, uint32_t e, const G& q) {
         if (!nb_fluid(e)) return;
         const v3 r3 = vsub(xi, q.x);
@@ -658,12 +658,12 @@ __device__ __forceinline__ void far_t_range(FarT& acc, const float4* __restrict_
 // Scaling of target i's sums: S_i = kden_sum - kden_i - sum_j kden_j (the
 // forward's order), coef = vp_factor / added.
 __device__ __forceinline__ void far_t_finish(const dimcv_params& p, const dimcv_adj_args& a, int64_t i,
-                                             const uint32_t* __restrict__ nl, int64_t ns, int nn, v3 ub,
+                                             const NbrList& nl, int64_t ns, int nn, v3 ub,
                                              const FarT& acc) {
   if (acc.n == 0.0f) return;
   float kns = a.f1[i];
   nbr_loop<8>(
-      nl, ns, nn, [&](int64_t j) { return a.f1[j]; },
+      nl, ns, nn, [&](uint32_t j) { return ld1s(a.f1, j); },
       [&](int, uint32_t e, float kj) {
         if (nb_fluid(e)) kns += kj;
       });

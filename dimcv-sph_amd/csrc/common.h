@@ -63,6 +63,9 @@ struct Workspace {
   float* scale_s;         // vort2vel S_i per fluid entry
   void* far_partials;     // split far-field partial sums (32 B each)
   int64_t far_partial_cap;
+  float4* xm;             // per slot (x, y, z, m_V) of the current step (neighbour gathers)
+  float4* xw;             // per slot (x, y, z, w), w = m (fluid) or m_V rho0 (the Jacobi passes)
+  float* kap;             // per slot kappa of the current Jacobi iteration
   float4* samples;        // packed far-field sources: (x, y, z, -) (cx, cy, cz, -)
   int64_t max_samples;
   int64_t bytes;
@@ -89,6 +92,17 @@ __device__ __forceinline__ v3 vcross(v3 a, v3 b) {
 }
 __device__ __forceinline__ v3 ld3(const float* __restrict__ a, int64_t i) {
   return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+}
+// Neighbour-field reads by slot through a 32-bit byte offset from the
+// (wave-uniform) field base, so a gather is one scalar-base load with a
+// vector offset instead of 64-bit address arithmetic per lane.  Slots stay
+// below DIMCV_MAX_SLOTS, so 12 * slot fits in 32 bits.
+__device__ __forceinline__ v3 ld3s(const float* __restrict__ a, uint32_t j) {
+  const float* q = (const float*)((const char*)a + j * 12u);
+  return mk(q[0], q[1], q[2]);
+}
+__device__ __forceinline__ float ld1s(const float* __restrict__ a, uint32_t j) {
+  return *(const float*)((const char*)a + j * 4u);
 }
 __device__ __forceinline__ void st3(float* __restrict__ a, int64_t i, v3 q) {
   a[3 * i] = q.x;
@@ -205,6 +219,19 @@ __device__ __forceinline__ float transit_weight(const dimcv_params& p, float xpo
 __device__ __forceinline__ int nb_slot(uint32_t e) { return (int)(e & 0x7fffffffu); }
 __device__ __forceinline__ bool nb_fluid(uint32_t e) { return (e >> 31) != 0u; }
 
+// The neighbour list of fluid-list entry f: entry k at nbr[k * n_max + f].
+// Row k's base is wave-uniform (k is a loop counter), so a load is one
+// scalar-base load with the lane's 32-bit byte offset 4 f.
+struct NbrList {
+  const uint32_t* __restrict__ base;
+  uint32_t lane;
+  __device__ __forceinline__ NbrList(const uint32_t* __restrict__ nbr, int64_t f)
+      : base(nbr), lane((uint32_t)f * 4u) {}
+  __device__ __forceinline__ uint32_t at(int64_t k, int64_t ns) const {
+    return *(const uint32_t*)((const char*)(base + k * ns) + lane);
+  }
+};
+
 // Register queue of the next NBR_Q list entries, loaded NBR_Q entries ahead
 // of use: a group of four neighbours then waits only for its neighbours'
 // data, not first for its own list entries (entries past nn read as 0 and
@@ -212,27 +239,27 @@ __device__ __forceinline__ bool nb_fluid(uint32_t e) { return (e >> 31) != 0u; }
 #define NBR_Q 16
 struct NbrQueue {
   uint32_t q[NBR_Q];
-  __device__ __forceinline__ NbrQueue(const uint32_t* __restrict__ nl, int64_t ns, int nn) {
+  __device__ __forceinline__ NbrQueue(const NbrList& nl, int64_t ns, int nn) {
 #pragma unroll
-    for (int u = 0; u < NBR_Q; ++u) q[u] = u < nn ? nl[(int64_t)u * ns] : 0u;
+    for (int u = 0; u < NBR_Q; ++u) q[u] = u < nn ? nl.at(u, ns) : 0u;
   }
-  __device__ __forceinline__ void advance4(const uint32_t* __restrict__ nl, int64_t ns, int nn, int k) {
+  __device__ __forceinline__ void advance4(const NbrList& nl, int64_t ns, int nn, int k) {
     advance<4>(nl, ns, nn, k);
   }
   // entries [k, k + S) have been used: shift and load entries k + NBR_Q ..
   template <int S>
-  __device__ __forceinline__ void advance(const uint32_t* __restrict__ nl, int64_t ns, int nn, int k) {
+  __device__ __forceinline__ void advance(const NbrList& nl, int64_t ns, int nn, int k) {
 #pragma unroll
     for (int u = 0; u < NBR_Q - S; ++u) q[u] = q[u + S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
       const int kk = k + NBR_Q + u;
-      q[NBR_Q - S + u] = kk < nn ? nl[(int64_t)kk * ns] : 0u;
+      q[NBR_Q - S + u] = kk < nn ? nl.at(kk, ns) : 0u;
     }
   }
 };
 
-// With nl = st.nbr + f, ns = n_max and nn = the list length in scope:
+// With nl = NbrList(st.nbr, f), ns = n_max and nn = the list length in scope:
 // for k in [0, nn): e = list entry k, j = its slot, in list order (groups
 // of four over the queue; `continue` moves to the next neighbour).
 #define FOR_NBR(k, e, j)                                                        \
@@ -251,17 +278,43 @@ struct NbrQueue {
 // suits passes with few gathered fields; passes with many keep 4 (register
 // pressure).
 template <int GS = 4, class Gather, class Body>
-__device__ __forceinline__ void nbr_loop(const uint32_t* __restrict__ nl, int64_t ns, int nn, Gather&& gather,
-                                         Body&& body) {
+__device__ __forceinline__ void nbr_loop(const NbrList& nl, int64_t ns, int nn, Gather&& gather, Body&& body) {
   static_assert(GS <= NBR_Q, "group larger than the list queue");
   NbrQueue Q(nl, ns, nn);
   for (int k0 = 0; k0 < nn; Q.template advance<GS>(nl, ns, nn, k0), k0 += GS) {
-    decltype(gather((int64_t)0)) g[GS];
+    decltype(gather((uint32_t)0)) g[GS];
 #pragma unroll
-    for (int u = 0; u < GS; ++u) g[u] = gather((int64_t)nb_slot(Q.q[u]));
+    for (int u = 0; u < GS; ++u) g[u] = gather((uint32_t)nb_slot(Q.q[u]));
 #pragma unroll
     for (int u = 0; u < GS; ++u)
       if (k0 + u < nn) body(k0 + u, Q.q[u], g[u]);
+  }
+}
+
+// Two-phase neighbour loop for passes whose pairs mostly stop at a test on a
+// few gathered values: gather1(j) for the group's GS entries; keep(e, a)
+// selects the entries that need more; gather2(j) is issued for all of those
+// together; body(k, e, a, b) runs for the kept entries in list order.  A
+// group waits for two memory latencies however many entries pass the test.
+template <int GS, class Gather1, class Keep, class Gather2, class Body>
+__device__ __forceinline__ void nbr_loop2(const NbrList& nl, int64_t ns, int nn, Gather1&& gather1, Keep&& keep,
+                                          Gather2&& gather2, Body&& body) {
+  static_assert(GS <= NBR_Q, "group larger than the list queue");
+  NbrQueue Q(nl, ns, nn);
+  for (int k0 = 0; k0 < nn; Q.template advance<GS>(nl, ns, nn, k0), k0 += GS) {
+    decltype(gather1((uint32_t)0)) a[GS];
+#pragma unroll
+    for (int u = 0; u < GS; ++u) a[u] = gather1((uint32_t)nb_slot(Q.q[u]));
+    bool ok[GS];
+#pragma unroll
+    for (int u = 0; u < GS; ++u) ok[u] = k0 + u < nn && keep(Q.q[u], a[u]);
+    decltype(gather2((uint32_t)0, 0u)) b[GS];
+#pragma unroll
+    for (int u = 0; u < GS; ++u)
+      if (ok[u]) b[u] = gather2((uint32_t)nb_slot(Q.q[u]), Q.q[u]);
+#pragma unroll
+    for (int u = 0; u < GS; ++u)
+      if (ok[u]) body(k0 + u, Q.q[u], a[u], b[u]);
   }
 }
 

@@ -724,6 +724,10 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
 int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                          void* stream) {
   hipStream_t s = (hipStream_t)stream;
+  if (p->n_max > DIMCV_MAX_SLOTS) {
+    set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
+    return DIMCV_ERR_ARG;
+  }
   if (fluid_bound > 0)
     hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound);
   DIMCV_LAUNCHED("k_nbr_build");

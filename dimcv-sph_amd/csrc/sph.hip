@@ -15,8 +15,29 @@ namespace dimcv {
   const int64_t i = st.fluid_list[f];                                      \
   if (i < 0) return;                                                       \
   const int nn = min(st.nbr_count[f], p.nbr_kmax);                         \
-  const uint32_t* __restrict__ nl = st.nbr + f;                            \
+  const NbrList nl(st.nbr, f);                            \
   const int64_t ns = p.n_max
+
+// (x, m_V) of neighbour slot j: with XM from the step's packed 16-byte
+// records (one aligned load), otherwise from the separate fields.
+template <bool XM>
+__device__ __forceinline__ float4 nb_xmv(const dimcv_state& st, const float4* __restrict__ xm, uint32_t j) {
+  if constexpr (XM) return *(const float4*)((const char*)xm + j * 16u);
+  const v3 a = ld3s(st.x, j);
+  return make_float4(a.x, a.y, a.z, ld1s(st.m_V, j));
+}
+__device__ __forceinline__ v3 xyz(const float4& q) { return mk(q.x, q.y, q.z); }
+
+// (x, w) of neighbour entry e (slot j), w = m for fluid and m_V rho0 for
+// solids, the mass factor of the pressure passes: with XM from the packed
+// records.
+template <bool XM>
+__device__ __forceinline__ float4 nb_xw(const dimcv_state& st, const dimcv_params& p, const float4* __restrict__ xw,
+                                        uint32_t j, uint32_t e) {
+  if constexpr (XM) return *(const float4*)((const char*)xw + j * 16u);
+  const v3 a = ld3s(st.x, j);
+  return make_float4(a.x, a.y, a.z, nb_fluid(e) ? ld1s(st.m, j) : ld1s(st.m_V, j) * p.density0);
+}
 
 
 // Zero-fill of slots that a "field.fill(0) + fluid writes" kernel leaves at
@@ -58,8 +79,9 @@ __global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params 
 // accumulated exactly as in its own kernel.  CHANGE also accumulates the
 // first compute_density_change of the divergence solve (:337-374), which
 // reads only x and v and the same kernel gradients.
-template <bool CHANGE>
-__global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_params p, int32_t bound) {
+template <bool CHANGE, bool XM>
+__global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_params p, int32_t bound,
+                                                        const float4* __restrict__ xm) {
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i);
   const v3 vi = CHANGE ? ld3(st.v, i) : mk(0.0f, 0.0f, 0.0f);
@@ -67,16 +89,17 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
   float den = 0.0f;
   float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
   float dadv = 0.0f;
-  struct G { v3 x, v; float m, mV; };
+  struct G { float4 xmv; v3 v; float m; };
   nbr_loop(nl, ns, nn,
-           [&](int64_t j) {
-             return G{ld3(st.x, j), CHANGE ? ld3(st.v, j) : mk(0.0f, 0.0f, 0.0f), st.m[j], st.m_V[j]};
+           [&](uint32_t j) {
+             return G{nb_xmv<XM>(st, xm, j), CHANGE ? ld3s(st.v, j) : mk(0.0f, 0.0f, 0.0f), ld1s(st.m, j)};
            },
            [&](int, uint32_t e, const G& g) {
-             v3 r = rel(p, xi, g.x);
-             den += g.mV * cubic_W(p, vnorm(r));
+             const float mV = g.xmv.w;
+             v3 r = rel(p, xi, xyz(g.xmv));
+             den += mV * cubic_W(p, vnorm(r));
              v3 gw = cubic_dW(p, r);
-             if (CHANGE) dadv += g.mV * vdot(vsub(vi, g.v), gw);
+             if (CHANGE) dadv += mV * vdot(vsub(vi, g.v), gw);
              if (nb_fluid(e)) {
                v3 gg = smul(g.m, gw);
                r3 += vdot(gg, gg);
@@ -84,7 +107,7 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
                r1 += gg.y;
                r2 += gg.z;
              } else {
-               v3 gg = smul(g.mV * p.density0, gw);
+               v3 gg = smul(mV * p.density0, gw);
                r0 += gg.x;
                r1 += gg.y;
                r2 += gg.z;
@@ -149,9 +172,11 @@ __device__ __forceinline__ void residual_partial(double v, double* __restrict__ 
 
 // dimcv_sph.py:337-374.  With `partials`, also the block partial of the
 // divergence residual (offset 0) of the values just written.
+template <bool XM>
 __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, const float4* __restrict__ xm,
+                                                  float* __restrict__ kap) {
   __shared__ double lds[256];
   if (jacobi_gated(st, done)) return;  // uniform per block
   double res = 0.0;
@@ -159,17 +184,61 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
     FLUID_ENTRY();
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float dadv = 0.0f;
-    struct G { v3 x, v; float mV; };
-    nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+    struct G { float4 xmv; v3 v; };
+    nbr_loop<8>(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, xm, j), ld3s(st.v, j)}; },
                 [&](int, uint32_t, const G& g) {
-                  dadv += g.mV * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
+                  dadv += g.xmv.w * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xmv))));
                 });
     float d = fmaxf(dadv, 0.0f);
     if (st.nbr_count[f] < 20) d = 0.0f;
     st.density_adv[i] = d;
+    if (XM) kap[i] = d * st.dfsph_factor[i] * st.density[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, 0.0f);
   }();
   if (partials) residual_partial(res, partials, lds);
+}
+
+// (x, m_V) of every slot as one 16-byte record, for the neighbour gathers
+// of a step (positions and volumes do not change until advection).
+__global__ void k_pack_xm(dimcv_state st, dimcv_params p, int32_t bound, float4* __restrict__ xm,
+                          float4* __restrict__ xw) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  const float x = st.x[3 * s], y = st.x[3 * s + 1], z = st.x[3 * s + 2], mV = st.m_V[s];
+  xm[s] = make_float4(x, y, z, mV);
+  xw[s] = make_float4(x, y, z, st.material[s] == 1 ? st.m[s] : mV * p.density0);
+}
+
+int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, float4* xm, float4* xw,
+                   hipStream_t s) {
+  if (particle_bound <= 0) return DIMCV_OK;
+  hipLaunchKernelGGL(k_pack_xm, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound, xm, xw);
+  DIMCV_LAUNCHED("k_pack_xm");
+  return DIMCV_OK;
+}
+
+// multiply_time_step (dimcv_sph.py:395-399) of the DFSPH factor at the start
+// of a Jacobi solve, also writing kappa_i of its first iteration:
+// b alpha rho (which 0, divergence) or (da - rho0) alpha (which 1, pressure).
+__global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int which, int32_t bound,
+                              float* __restrict__ kap) {
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  const int64_t i = st.fluid_list[f];
+  if (i < 0) return;
+  float a = st.dfsph_factor[i];
+  a *= scale;
+  st.dfsph_factor[i] = a;
+  kap[i] = which == 0 ? st.density_adv[i] * a * st.density[i] : (st.density_adv[i] - p.density0) * a;
+}
+
+int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
+                       float* kap, hipStream_t s) {
+  if (fluid_bound <= 0) return DIMCV_OK;
+  hipLaunchKernelGGL(k_scale_kappa, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, scale, which,
+                     fluid_bound, kap);
+  DIMCV_LAUNCHED("k_scale_kappa");
+  return DIMCV_OK;
 }
 
 // dimcv_sph.py:395-399
@@ -201,39 +270,37 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
   if (threadIdx.x == 0) partials[xcd_block()] = s;
 }
 
-// dimcv_sph.py:421-468
+// dimcv_sph.py:421-468.  kappa_j = b_j alpha_j rho_j is gathered first (with
+// XM from the array the previous density change wrote); positions and mass
+// factors only for the pairs that pass |kappa_i + kappa_j| > eps.
+template <bool XM>
 __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done) {
+                                                  const int32_t* __restrict__ done, const float4* __restrict__ xw,
+                                                  const float* __restrict__ kap) {
   if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
-  float b_i = st.density_adv[i];
-  float k_i = b_i * st.dfsph_factor[i] * st.density[i];
+  const float k_i = XM ? kap[i] : st.density_adv[i] * st.dfsph_factor[i] * st.density[i];
   v3 dv = mk(0.0f, 0.0f, 0.0f);
-  // most pairs stop at the |k_i + k_j| test: only its three scalars are
-  // gathered up front; positions and masses are loaded for the pairs that
-  // pass it
-  struct G { float b, fac, rho; };
-  nbr_loop(nl, ns, nn,
-           [&](int64_t j) { return G{st.density_adv[j], st.dfsph_factor[j], st.density[j]}; },
-           [&](int, uint32_t e, const G& g) {
-             const int64_t j = nb_slot(e);
-             if (nb_fluid(e)) {
-               float k_j = g.b * g.fac * g.rho;
-               float k_sum = k_i + k_j;
-               if (fabsf(k_sum) > p.m_eps) {
-                 v3 gg = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-                 dv = vsub(dv, smul(dt * k_sum, gg));
-               }
-             } else {
-               if (fabsf(k_i) > p.m_eps) {
-                 v3 gg = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-                 v3 vc = smul(-dt * 1.0f * k_i, gg);
-                 dv = vadd(dv, vc);
-               }
-             }
-           });
+  nbr_loop2<8>(
+      nl, ns, nn,
+      [&](uint32_t j) {
+        if constexpr (XM) return ld1s(kap, j);
+        return ld1s(st.density_adv, j) * ld1s(st.dfsph_factor, j) * ld1s(st.density, j);
+      },
+      [&](uint32_t e, float k_j) { return fabsf(nb_fluid(e) ? k_i + k_j : k_i) > p.m_eps; },
+      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, xw, j, e); },
+      [&](int, uint32_t e, float k_j, const float4& q) {
+        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q))));
+        if (nb_fluid(e)) {
+          float k_sum = k_i + k_j;
+          dv = vsub(dv, smul(dt * k_sum, gg));
+        } else {
+          v3 vc = smul(-dt * 1.0f * k_i, gg);
+          dv = vadd(dv, vc);
+        }
+      });
   v3 vi = ld3(st.v, i);
   if (!p.has_transit) vi = vadd(vi, dv);
   else vi = vadd(vi, smul(transit_weight(p, xi.x), dv));
@@ -242,19 +309,21 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
 
 // dimcv_sph.py:150-208, fluid particles (surface tension sigma = 0 and the
 // boundary viscosity 0 contribute exact zeros and are not evaluated).
-__global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound) {
+template <bool XM>
+__global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound,
+                                                           const float4* __restrict__ xm) {
   FLUID_ENTRY();
   const float visc_coef = st.block_scene ? st.scenes[block_scene_of(st)].visc_coef : p.visc_coef;
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
-  struct G { v3 x, v; float mV; };
-  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m_V[j]}; },
+  struct G { float4 xmv; v3 v; };
+  nbr_loop(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, xm, j), ld3s(st.v, j)}; },
            [&](int, uint32_t e, const G& g) {
              if (nb_fluid(e)) {
-               v3 r = rel(p, xi, g.x);
+               v3 r = rel(p, xi, xyz(g.xmv));
                float v_xy = vdot(vsub(vi, g.v), r);
                float rn = vnorm(r);
-               v3 fv = smul(visc_coef * g.mV * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
+               v3 fv = smul(visc_coef * g.xmv.w * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
                dv = vadd(dv, fv);
              }
            });
@@ -294,9 +363,11 @@ __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
 // dimcv_sph.py:470-493
 // With `partials`, also the block partial of the pressure residual
 // (offset: the caller's density0) of the values just written.
+template <bool XM>
 __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done,
-                                                  double* __restrict__ partials, float offset) {
+                                                  double* __restrict__ partials, float offset,
+                                                  const float4* __restrict__ xw, float* __restrict__ kap) {
   __shared__ double lds[256];
   if (jacobi_gated(st, done)) return;  // uniform per block
   double res = 0.0;
@@ -304,55 +375,62 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
     FLUID_ENTRY();
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float delta = 0.0f;
-    struct G { v3 x, v; float m, mV; };
-    nbr_loop<8>(nl, ns, nn, [&](int64_t j) { return G{ld3(st.x, j), ld3(st.v, j), st.m[j], st.m_V[j]}; },
+    // w = m (fluid) or m_V rho0 (solid), as the reference weights the two
+    // sums; without the packed records w = m and solids read m_V below
+    struct G { float4 xw; v3 v; };
+    nbr_loop<8>(nl, ns, nn,
+                [&](uint32_t j) {
+                  if constexpr (XM) return G{nb_xw<true>(st, p, xw, j, 0u), ld3s(st.v, j)};
+                  const v3 a = ld3s(st.x, j);
+                  return G{make_float4(a.x, a.y, a.z, ld1s(st.m, j)), ld3s(st.v, j)};
+                },
                 [&](int, uint32_t e, const G& g) {
-                  float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, g.x)));
-                  if (nb_fluid(e)) delta += g.m * dd;
-                  else delta += g.mV * p.density0 * dd;
+                  float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xw))));
+                  if (XM || nb_fluid(e)) delta += g.xw.w * dd;
+                  else delta += ld1s(st.m_V, nb_slot(e)) * p.density0 * dd;
                 });
     float da = st.density[i] + p.dt * delta;
     const float d = fmaxf(da, p.density0);
     st.density_adv[i] = d;
+    if (XM) kap[i] = (d - p.density0) * st.dfsph_factor[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, offset);
   }();
   if (partials) residual_partial(res, partials, lds);
 }
 
-// dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour)
+// dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour);
+// kappa_j = (da_j - rho0) alpha_j first, as in k_div_iter.
+template <bool XM>
 __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done) {
+                                                       const int32_t* __restrict__ done,
+                                                       const float4* __restrict__ xw, const float* __restrict__ kap) {
   if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
-  float b_i = st.density_adv[i] - p.density0;
-  float k_i = b_i * st.dfsph_factor[i];
+  const float k_i = XM ? kap[i] : (st.density_adv[i] - p.density0) * st.dfsph_factor[i];
   v3 vi = ld3(st.v, i);
   const float w = p.has_transit ? transit_weight(p, xi.x) : 1.0f;
-  // most pairs stop at the |k_i + k_j| test (see k_div_iter)
-  struct G { float b, fac; };
-  nbr_loop(nl, ns, nn, [&](int64_t j) { return G{st.density_adv[j], st.dfsph_factor[j]}; },
-           [&](int, uint32_t e, const G& g) {
-             const int64_t j = nb_slot(e);
-             if (nb_fluid(e)) {
-               float b_j = g.b - p.density0;
-               float k_j = b_j * g.fac;
-               float k_sum = k_i + k_j;
-               if (fabsf(k_sum) > p.m_eps) {
-                 v3 gg = smul(st.m[j], cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-                 if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, gg));
-                 else vi = vsub(vi, smul(w * dt * k_sum, gg));
-               }
-             } else {
-               if (fabsf(k_i) > p.m_eps) {
-                 v3 gg = smul(st.m_V[j] * p.density0, cubic_dW(p, rel(p, xi, ld3(st.x, j))));
-                 v3 vc = smul(-dt * 1.0f * k_i, gg);
-                 if (!p.has_transit) vi = vadd(vi, vc);
-                 else vi = vadd(vi, smul(w, vc));
-               }
-             }
-           });
+  nbr_loop2<8>(
+      nl, ns, nn,
+      [&](uint32_t j) {
+        if constexpr (XM) return ld1s(kap, j);
+        return (ld1s(st.density_adv, j) - p.density0) * ld1s(st.dfsph_factor, j);
+      },
+      [&](uint32_t e, float k_j) { return fabsf(nb_fluid(e) ? k_i + k_j : k_i) > p.m_eps; },
+      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, xw, j, e); },
+      [&](int, uint32_t e, float k_j, const float4& q) {
+        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q))));
+        if (nb_fluid(e)) {
+          float k_sum = k_i + k_j;
+          if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, gg));
+          else vi = vsub(vi, smul(w * dt * k_sum, gg));
+        } else {
+          v3 vc = smul(-dt * 1.0f * k_i, gg);
+          if (!p.has_transit) vi = vadd(vi, vc);
+          else vi = vadd(vi, smul(w, vc));
+        }
+      });
   st3(st.v, i, vi);
 }
 
@@ -421,8 +499,9 @@ __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
 // 2 = grad_v, 3 = both (+ v*) in one neighbour pass.
 // MODE 7 also forms d_vort = vorticity - curl v* (:551-583) in the same loop:
 // v*_j = v_j + a_j dt is formed on the fly exactly as compute_v_star does.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound) {
+template <int MODE, bool XM>
+__global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound,
+                                                    const float4* __restrict__ xm) {
   FLUID_ENTRY();
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 vort = mk(0.0f, 0.0f, 0.0f);
@@ -430,18 +509,18 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
   v3 vsi = mk(0.0f, 0.0f, 0.0f);
   if (MODE & 4) vsi = vadd(vi, vmuls(ld3(st.acceleration, i), p.dt));
   float G[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  struct Gt { v3 x, v, a; float mV; };
+  struct Gt { float4 xmv; v3 v, a; };
   nbr_loop(nl, ns, nn,
-           [&](int64_t j) {
-             return Gt{ld3(st.x, j), ld3(st.v, j), (MODE & 4) ? ld3(st.acceleration, j) : mk(0.0f, 0.0f, 0.0f),
-                       st.m_V[j]};
+           [&](uint32_t j) {
+             return Gt{nb_xmv<XM>(st, xm, j), ld3s(st.v, j),
+                       (MODE & 4) ? ld3s(st.acceleration, j) : mk(0.0f, 0.0f, 0.0f)};
            },
            [&](int, uint32_t e, const Gt& g) {
              if (!nb_fluid(e)) return;
              v3 vj = g.v;
-             v3 kd = cubic_dW(p, rel(p, xi, g.x));
+             v3 kd = cubic_dW(p, rel(p, xi, xyz(g.xmv)));
              v3 vji = vsub(vj, vi);
-             float mv = g.mV;
+             float mv = g.xmv.w;
              if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
              if (MODE & 2) {
                v3 a = smul(mv, vji);
@@ -561,15 +640,17 @@ __device__ __forceinline__ float epan(const dimcv_params& p, v3 r) {
 // neighbour loop, with the block's canonical FP64 partial of kden_sum.  KVN
 // must be current (it depends only on grad_v, so computing it before the
 // smoothing changes no value).
+template <bool XM>
 __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_params p, int32_t bound,
-                                                         double* __restrict__ partials) {
+                                                         double* __restrict__ partials,
+                                                         const float4* __restrict__ xm) {
   __shared__ double lds[256];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   double contrib = 0.0;
   const int64_t i = (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) ? st.fluid_list[f] : -1;
   if (i >= 0) {
     const int nn = min(st.nbr_count[f], p.nbr_kmax);
-    const uint32_t* __restrict__ nl = st.nbr + f;
+    const NbrList nl(st.nbr, f);
     const int64_t ns = p.n_max;
     const v3 xi = ld3(st.x, i);
     const bool dom = in_vortex_domain(p, xi);
@@ -578,15 +659,16 @@ __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_p
     float w = st.m_V[i] * w0;
     v3 vort = smul(w, ld3(st.vorticity, i));
     float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
-    struct G { v3 x, dv, vo; float mV, kvn; };
+    struct G { float4 xmv; v3 dv, vo; float kvn; };
     nbr_loop<8>(nl, ns, nn,
-             [&](int64_t j) {
-               return G{ld3(st.x, j), ld3(st.d_vort, j), ld3(st.vorticity, j), st.m_V[j], st.kinematic_vort_num[j]};
+             [&](uint32_t j) {
+               return G{nb_xmv<XM>(st, xm, j), ld3s(st.d_vort, j), ld3s(st.vorticity, j),
+                        ld1s(st.kinematic_vort_num, j)};
              },
              [&](int, uint32_t e, const G& g) {
-               const v3 r = rel(p, xi, g.x);
+               const v3 r = rel(p, xi, xyz(g.xmv));
                float Wr = cubic_W(p, vnorm(r));
-               float mv = g.mV;
+               float mv = g.xmv.w;
                if (dom && nb_fluid(e)) {
                  acc = vadd(acc, vmuls(smul(mv, g.dv), Wr));
                  kden += g.kvn * epan(p, r);
@@ -617,7 +699,7 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
     if (i >= 0 && in_vortex_domain(p, ld3(st.x, i))) {
       const v3 xi = ld3(st.x, i);
       const int nn = min(st.nbr_count[f], p.nbr_kmax);
-      const uint32_t* __restrict__ nl = st.nbr + f;
+      const NbrList nl(st.nbr, f);
       const int64_t ns = p.n_max;
       float kden = st.kinematic_vort_num[i] * epan(p, mk(0.0f, 0.0f, 0.0f));
       FOR_NBR(k, e, j) {
@@ -648,25 +730,54 @@ int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t
 // which 1 and 3 also write the residual's block partials when `partials`
 // is given (offset: that of the pressure residual).
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
-                       const int32_t* done, hipStream_t s, double* partials, float offset) {
+                       const int32_t* done, hipStream_t s, double* partials, float offset, const float4* xm,
+                       const float4* xw, float* kap) {
   if (fluid_bound <= 0) return DIMCV_OK;
-  dim3 g(nblocks(fluid_bound)), b(256);
+  const dim3 g(nblocks(fluid_bound)), b(256);
+  const bool packed = xm && xw && kap;
   switch (which) {
-    case 0: hipLaunchKernelGGL(k_div_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
-    case 1: hipLaunchKernelGGL(k_density_change, g, b, 0, s, *st, *p, fluid_bound, done, partials); break;
-    case 2: hipLaunchKernelGGL(k_pressure_iter, g, b, 0, s, *st, *p, fluid_bound, done); break;
-    default: hipLaunchKernelGGL(k_density_adv, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset); break;
+    case 0:
+      if (packed) hipLaunchKernelGGL(k_div_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      else hipLaunchKernelGGL(k_div_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      break;
+    case 1:
+      if (packed) hipLaunchKernelGGL(k_density_change<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, xm, kap);
+      else hipLaunchKernelGGL(k_density_change<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, xm, kap);
+      break;
+    case 2:
+      if (packed) hipLaunchKernelGGL(k_pressure_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      else hipLaunchKernelGGL(k_pressure_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      break;
+    default:
+      if (packed)
+        hipLaunchKernelGGL(k_density_adv<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, xw, kap);
+      else
+        hipLaunchKernelGGL(k_density_adv<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, xw, kap);
+      break;
   }
   DIMCV_LAUNCHED("jacobi pass");
   return DIMCV_OK;
 }
 
-// which: 0 density + factor; 1 v*, vorticity, grad_v, d_vort (zeroing the
-// non-fluid slots first); 2 smoothing + vorticity_vis; 3 density + factor +
-// the first density change; 4 smoothing + vorticity_vis + kernel density,
-// with kden_sum block partials into `partials` (KVN computed before).
+// compute_non_pressure_forces (dimcv_sph.py:150-208): non-fluid slots, then
+// the fluid pass (gathering the packed (x, m_V) records when xm is given).
+int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
+                       hipStream_t s, const float4* xm) {
+  if (particle_bound > 0)
+    hipLaunchKernelGGL(k_nonpressure_other, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p,
+                       particle_bound);
+  DIMCV_LAUNCHED("k_nonpressure_other");
+  if (fluid_bound > 0) {
+    const dim3 g(nblocks(fluid_bound)), b(256);
+    if (xm) hipLaunchKernelGGL(k_nonpressure_fluid<true>, g, b, 0, s, *st, *p, fluid_bound, xm);
+    else hipLaunchKernelGGL(k_nonpressure_fluid<false>, g, b, 0, s, *st, *p, fluid_bound, xm);
+  }
+  DIMCV_LAUNCHED("k_nonpressure_fluid");
+  return DIMCV_OK;
+}
+
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
-                 double* partials) {
+                 double* partials, const float4* xm) {
   if (which == 1) {  // v*, vorticity, grad_v, d_vort: zero the non-fluid slots first
     int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
     if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
@@ -682,10 +793,22 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
   if (fluid_bound <= 0) return DIMCV_OK;
   dim3 g(nblocks(fluid_bound)), b(256);
   switch (which) {
-    case 0: hipLaunchKernelGGL(k_density_factor<false>, g, b, 0, s, *st, *p, fluid_bound); break;
-    case 1: hipLaunchKernelGGL(k_vort_gradv<7>, g, b, 0, s, *st, *p, fluid_bound); break;
-    case 3: hipLaunchKernelGGL(k_density_factor<true>, g, b, 0, s, *st, *p, fluid_bound); break;
-    case 4: hipLaunchKernelGGL(k_smooth_vis_kden, g, b, 0, s, *st, *p, fluid_bound, partials); break;
+    case 0:
+      if (xm) hipLaunchKernelGGL((k_density_factor<false, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      else hipLaunchKernelGGL((k_density_factor<false, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      break;
+    case 1:
+      if (xm) hipLaunchKernelGGL((k_vort_gradv<7, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      else hipLaunchKernelGGL((k_vort_gradv<7, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      break;
+    case 3:
+      if (xm) hipLaunchKernelGGL((k_density_factor<true, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      else hipLaunchKernelGGL((k_density_factor<true, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      break;
+    case 4:
+      if (xm) hipLaunchKernelGGL(k_smooth_vis_kden<true>, g, b, 0, s, *st, *p, fluid_bound, partials, xm);
+      else hipLaunchKernelGGL(k_smooth_vis_kden<false>, g, b, 0, s, *st, *p, fluid_bound, partials, xm);
+      break;
     default: hipLaunchKernelGGL(k_smooth_vis, g, b, 0, s, *st, *p, fluid_bound); break;
   }
   DIMCV_LAUNCHED("fused pass");
@@ -710,6 +833,11 @@ int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* 
 }  // namespace dimcv
 
 using namespace dimcv;
+
+// single-mode vorticity passes of the per-pass entry points
+constexpr auto k_vort_gradv_1 = k_vort_gradv<1, false>;
+constexpr auto k_vort_gradv_2 = k_vort_gradv<2, false>;
+constexpr auto k_vort_gradv_3 = k_vort_gradv<3, false>;
 
 #define LAUNCH_FLUID(kern, ...)                                                                 \
   do {                                                                                          \
@@ -740,7 +868,7 @@ int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int
 }
 
 int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_change, *st, *p, fluid_bound, nullptr, nullptr);
+  LAUNCH_FLUID(k_density_change<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr, nullptr);
   return DIMCV_OK;
 }
 
@@ -772,15 +900,13 @@ int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, fl
 }
 
 int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_div_iter, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_div_iter<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr);
   return DIMCV_OK;
 }
 
 int dimcv_compute_non_pressure_forces(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
                                       int32_t fluid_bound, void* stream) {
-  LAUNCH_SLOTS(k_nonpressure_other, particle_bound, *st, *p, particle_bound);
-  LAUNCH_FLUID(k_nonpressure_fluid, *st, *p, fluid_bound);
-  return DIMCV_OK;
+  return dimcv::launch_nonpressure(st, p, particle_bound, fluid_bound, (hipStream_t)stream, nullptr);
 }
 
 int dimcv_predict_velocity(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {
@@ -794,12 +920,12 @@ int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_
 }
 
 int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_adv, *st, *p, fluid_bound, nullptr, nullptr, 0.0f);
+  LAUNCH_FLUID(k_density_adv<false>, *st, *p, fluid_bound, nullptr, nullptr, 0.0f, nullptr, nullptr);
   return DIMCV_OK;
 }
 
 int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_pressure_iter, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_pressure_iter<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr);
   return DIMCV_OK;
 }
 
@@ -831,21 +957,21 @@ int dimcv_compute_v_star(const dimcv_state* st, const dimcv_params* p, int32_t f
 int dimcv_compute_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->vorticity, 3);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv<1>, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_vort_gradv_1, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
 int dimcv_compute_grad_v(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->grad_v, 9);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv<2>, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_vort_gradv_2, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 
 int dimcv_dimcv_init(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv<3>, *st, *p, fluid_bound);
+  LAUNCH_FLUID(k_vort_gradv_3, *st, *p, fluid_bound, nullptr);
   return DIMCV_OK;
 }
 

@@ -51,6 +51,9 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.block_pre = (double*)take(8 * (scan_blocks + 16));
   w.near3 = (float*)take(4 * 3 * n);
   w.scale_s = (float*)take(4 * n);
+  w.xm = (float4*)take(16 * n);
+  w.xw = (float4*)take(16 * n);
+  w.kap = (float*)take(4 * n);
   // split far-field partials: splits * targets of far_split_count for any
   // target count up to n (16 n when n < 65536, below 2^21 otherwise)
   w.far_partial_cap = n * 16 < (int64_t)(1 << 21) ? n * 16 : (int64_t)(1 << 21);
@@ -66,6 +69,10 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
 int bind_workspace(const dimcv_state* st, const dimcv_params* p, Workspace* ws, int64_t max_samples) {
   if (!st || !p) {
     set_error("null state or params");
+    return DIMCV_ERR_ARG;
+  }
+  if (p->n_max > DIMCV_MAX_SLOTS) {
+    set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
     return DIMCV_ERR_ARG;
   }
   // The sample table is sized by whatever the caller's workspace has room for.

@@ -29,13 +29,13 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
   v3 xi = ld3(st.x, i);
   if (!in_vortex_domain(p, xi)) return;
   const int nn = min(st.nbr_count[f], p.nbr_kmax);
-  const uint32_t* __restrict__ nl = st.nbr + f;
+  const NbrList nl(st.nbr, f);
   const int64_t ns = p.n_max;
   v3 vv = mk(0.0f, 0.0f, 0.0f);
   float kns = st.kernel_density[i];
   struct G { v3 x, w; float kd; };
   nbr_loop<8>(nl, ns, nn,
-           [&](int64_t j) { return G{ld3(st.x, j), ld3(st.d_vort_smoothed, j), st.kernel_density[j]}; },
+           [&](uint32_t j) { return G{ld3s(st.x, j), ld3s(st.d_vort_smoothed, j), ld1s(st.kernel_density, j)}; },
            [&](int, uint32_t e, const G& g) {
              if (!nb_fluid(e)) return;
              v3 r2 = rel(p, xi, g.x);

@@ -33,6 +33,12 @@ extern "C" {
 
 #define DIMCV_ABI_VERSION 1
 
+/* Largest particle capacity n_max: the neighbour passes address a lane's
+ * list entries and its neighbours' fields by 32-bit byte offsets
+ * (4 * entry and 12 * slot).  Entry points that take a workspace or build
+ * neighbour lists return DIMCV_ERR_ARG above it. */
+#define DIMCV_MAX_SLOTS (1 << 28)
+
 typedef enum dimcv_status {
   DIMCV_OK = 0,
   DIMCV_ERR_ARG = -1,        /* invalid argument / shape */

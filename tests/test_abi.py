@@ -41,9 +41,10 @@ def test_library_version_and_workspace_query():
     big = L.dimcv_workspace_bytes(1_048_576, 131_072, 1_048_576)
     assert 0 < small < big
     assert big % 256 == 0
-    # keys/values 16 B, CDF 16 B, near field 16 B, sample table 32 B per
-    # particle, plus at most 2^21 split partials of 32 B
-    assert 80 * 1_048_576 <= big <= 80 * 1_048_576 + 32 * 2 ** 21 + (1 << 20)
+    # keys/values 16 B, CDF 16 B, near field 16 B, (x, m_V) and (x, w)
+    # records 32 B, kappa 4 B, sample table 32 B per particle, plus at most
+    # 2^21 split partials of 32 B
+    assert 116 * 1_048_576 <= big <= 116 * 1_048_576 + 32 * 2 ** 21 + (1 << 20)
 
 
 def _probe_layout():
