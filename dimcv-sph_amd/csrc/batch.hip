@@ -25,10 +25,9 @@ namespace dimcv {
 
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        const int32_t* done, hipStream_t s, double* partials = nullptr,
-                       float offset = 0.0f, const float4* xm = nullptr, const float4* xw = nullptr,
-                       float* kap = nullptr);
+                       float offset = 0.0f, const Packed* pk = nullptr);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
-                 double* partials = nullptr, const float4* xm = nullptr);
+                 double* partials = nullptr, const Packed* pk = nullptr);
 int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,
                           double* partials, const int32_t* done, hipStream_t s);
 int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,

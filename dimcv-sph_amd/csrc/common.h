@@ -66,6 +66,7 @@ struct Workspace {
   float4* xm;             // per slot (x, y, z, m_V) of the current step (neighbour gathers)
   float4* xw;             // per slot (x, y, z, w), w = m (fluid) or m_V rho0 (the Jacobi passes)
   float* kap;             // per slot kappa of the current Jacobi iteration
+  int32_t* tiny;          // step flag: some coordinate is nonzero and below 2^-76 in magnitude
   float4* samples;        // packed far-field sources: (x, y, z, -) (cx, cy, cz, -)
   int64_t max_samples;
   int64_t bytes;
@@ -127,13 +128,40 @@ __device__ __forceinline__ float cubic_W(const dimcv_params& p, float r_norm) {
   return res;
 }
 
-// Kernel gradient, sph_base.py:46-68.
-__device__ __forceinline__ v3 cubic_dW(const dimcv_params& p, v3 r) {
+// a / b for the three components of a, rounded exactly as the IEEE division
+// hipcc emits for gfx950 (v_div_scale, v_rcp, one refinement of the
+// reciprocal, the quotient and two residual corrections, v_div_fmas,
+// v_div_fixup), with the reciprocal refinement done once for the shared b.
+// Bitwise equal to a.x / b etc. whenever the scale and fix-up steps change
+// nothing: b a normal float in [2^-100, 2^100] and each component 0 or of
+// magnitude in [2^-100, b]; the fix-up then only sets the sign, sign(a).
+__device__ __forceinline__ float div_corrected(float x, float b, float r) {
+  float q = x * r;
+  const float e2 = fmaf(-b, q, x);
+  q = fmaf(e2, r, q);
+  const float e3 = fmaf(-b, q, x);
+  q = fmaf(e3, r, q);
+  return __int_as_float((__float_as_int(q) & 0x7fffffff) | (__float_as_int(x) & (int)0x80000000));
+}
+__device__ __forceinline__ v3 vdivs_shared(v3 a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, r, 1.0f);
+  r = fmaf(e, r, r);
+  return mk(div_corrected(a.x, b, r), div_corrected(a.y, b, r), div_corrected(a.z, b, r));
+}
+
+// Kernel gradient, sph_base.py:46-68.  With shared_div the three divisions
+// r / |r| share their reciprocal (vdivs_shared); the caller passes it only
+// when every coordinate of the step is 0 or at least 2^-76 in magnitude, so
+// every nonzero component of r (a difference of two such floats, or one
+// minus a box length) is at least 2^-99 and the quotients are bitwise the
+// IEEE ones (|r| > 1e-5 and |r| <= h here).
+__device__ __forceinline__ v3 cubic_dW(const dimcv_params& p, v3 r, bool shared_div = false) {
   float rn = vnorm(r);
   float q = rn * p.inv_support_radius;
   v3 res = mk(0.0f, 0.0f, 0.0f);
   if (rn > 1e-5f && q <= 1.0f) {
-    v3 gq = vmuls(vdivs(r, rn), p.inv_support_radius);
+    v3 gq = vmuls(shared_div ? vdivs_shared(r, rn) : vdivs(r, rn), p.inv_support_radius);
     if (q <= 0.5f) {
       res = smul(p.kernel_dk * q * (3.0f * q - 2.0f), gq);
     } else {
@@ -290,6 +318,16 @@ __device__ __forceinline__ void nbr_loop(const NbrList& nl, int64_t ns, int nn, 
       if (k0 + u < nn) body(k0 + u, Q.q[u], g[u]);
   }
 }
+
+// The step's packed per-slot records (built by the substep; absent in the
+// per-pass entry points): (x, m_V), (x, w) with w = m or m_V rho0, kappa,
+// and the tiny-coordinate flag that gates the shared-reciprocal divisions.
+struct Packed {
+  const float4* xm;
+  const float4* xw;
+  float* kap;
+  const int32_t* tiny;
+};
 
 // Two-phase neighbour loop for passes whose pairs mostly stop at a test on a
 // few gathered values: gather1(j) for the group's GS entries; keep(e, a)

@@ -374,18 +374,38 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
 // tested, so a lane waits for one memory latency per SCAN_U candidates
 // rather than per candidate.  f(j, is_fluid) is called in slot order.
 #define SCAN_U 8
-template <class F>
-__device__ __forceinline__ void scan_run(const float* __restrict__ x, const int32_t* __restrict__ material,
-                                         int beg, int end, int64_t i, v3 xi, float lim, F&& f) {
+// Candidate positions and material flags: from the separate fields, or from
+// 16-byte (x, y, z, material bits) records (one aligned load per candidate).
+struct CandFields {
+  const float* __restrict__ x;
+  const int32_t* __restrict__ material;
+  __device__ __forceinline__ void get(int j, v3& xj, int32_t& mt) const {
+    xj = ld3s(x, (uint32_t)j);
+    mt = material[j];
+  }
+};
+struct CandRecords {
+  const float4* __restrict__ rec;
+  __device__ __forceinline__ void get(int j, v3& xj, int32_t& mt) const {
+    const float4 q = *(const float4*)((const char*)rec + (uint32_t)j * 16u);
+    xj = mk(q.x, q.y, q.z);
+    mt = __float_as_int(q.w);
+  }
+};
+
+__global__ void k_pack_candidates(dimcv_state st, int32_t bound, float4* __restrict__ rec) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  rec[s] = make_float4(st.x[3 * s], st.x[3 * s + 1], st.x[3 * s + 2], __int_as_float(st.material[s]));
+}
+
+template <class Cand, class F>
+__device__ __forceinline__ void scan_run(const Cand& cand, int beg, int end, int64_t i, v3 xi, float lim, F&& f) {
   for (int j = beg; j < end; j += SCAN_U) {
     v3 xj[SCAN_U];
     int32_t mt[SCAN_U];
 #pragma unroll
-    for (int u = 0; u < SCAN_U; ++u) {
-      const int jj = min(j + u, end - 1);
-      xj[u] = ld3(x, jj);
-      mt[u] = material[jj];
-    }
+    for (int u = 0; u < SCAN_U; ++u) cand.get(min(j + u, end - 1), xj[u], mt[u]);
 #pragma unroll
     for (int u = 0; u < SCAN_U; ++u) {
       const v3 r = vsub(xi, xj[u]);
@@ -395,9 +415,8 @@ __device__ __forceinline__ void scan_run(const float* __restrict__ x, const int3
   }
 }
 
-template <class F>
-__device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __restrict__ x,
-                                          const int32_t* __restrict__ material, const int32_t* __restrict__ gpn,
+template <class Cand, class F>
+__device__ __forceinline__ void scan_runs(const dimcv_params& p, const Cand& cand, const int32_t* __restrict__ gpn,
                                           int64_t i, v3 xi, F&& f) {
   int cc[3];
   cell_of(p, xi, cc);
@@ -413,7 +432,7 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __
       int fl = (g0 * ny + g1) * nz;
       int beg = gpn[max(fl + zlo - 1, 0)];
       int end = gpn[fl + zhi];
-      scan_run(x, material, beg, end, i, xi, lim, f);
+      scan_run(cand, beg, end, i, xi, lim, f);
     }
   }
 }
@@ -437,7 +456,8 @@ __device__ __forceinline__ void nbr_flush(dimcv_state& st, const dimcv_params& p
     if (k < c) st.nbr[(int64_t)k * stride + f] = stage[k][lane];
 }
 
-__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
+template <class Cand>
+__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound, Cand cand) {
   __shared__ uint32_t stage[4][NBR_STAGE][64];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -454,7 +474,7 @@ __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params 
       ++cnt;
     };
     if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) { emit(j, st.material[j] == 1); });
-    else scan_runs(p, st.x, st.material, st.grid_particles_num, i, xi, emit);
+    else scan_runs(p, cand, st.grid_particles_num, i, xi, emit);
     st.nbr_count[f] = cnt;
     atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
   }
@@ -600,7 +620,7 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
         const int lo = fl + zlo - 1;
         const int beg = lo >= 0 ? gpn[lo] : gpn[0];
         const int end = gpn[fl + zhi];
-        scan_run(st.x, st.material, beg, end, i, xi, lim, [&](int j, bool fluid) {
+        scan_run(CandFields{st.x, st.material}, beg, end, i, xi, lim, [&](int j, bool fluid) {
           const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
           if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
           else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
@@ -728,8 +748,22 @@ int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t f
     set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
     return DIMCV_ERR_ARG;
   }
-  if (fluid_bound > 0)
-    hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound);
+  if (fluid_bound > 0) {
+    // with a workspace, candidates are read from packed (x, material) records
+    Workspace ws;
+    const bool packed = !p->periodic && st->workspace && p->n_max <= DIMCV_MAX_SLOTS &&
+                        st->workspace_bytes >= workspace_layout(p->n_max, p->n_cells, 1, nullptr, nullptr) &&
+                        bind_workspace(st, p, &ws, 0) == DIMCV_OK;
+    const int32_t pb = (int32_t)std::min<int64_t>(p->n_max, INT32_MAX);
+    if (packed) {
+      hipLaunchKernelGGL(k_pack_candidates, dim3(nblocks(pb)), dim3(256), 0, s, *st, pb, ws.xw);
+      hipLaunchKernelGGL(k_nbr_build<CandRecords>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p,
+                         fluid_bound, CandRecords{ws.xw});
+    } else {
+      hipLaunchKernelGGL(k_nbr_build<CandFields>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
+                         CandFields{st->x, st->material});
+    }
+  }
   DIMCV_LAUNCHED("k_nbr_build");
   int32_t mx = 0;
   DIMCV_HIP(hipMemcpyAsync(&mx, st->counters + DIMCV_C_NBR_MAX, sizeof(int32_t), hipMemcpyDeviceToHost, s));

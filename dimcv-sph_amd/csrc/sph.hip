@@ -81,8 +81,9 @@ __global__ void __launch_bounds__(256) k_densities(dimcv_state st, dimcv_params 
 // reads only x and v and the same kernel gradients.
 template <bool CHANGE, bool XM>
 __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_params p, int32_t bound,
-                                                        const float4* __restrict__ xm) {
+                                                        Packed pk) {
   FLUID_ENTRY();
+  const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   v3 xi = ld3(st.x, i);
   const v3 vi = CHANGE ? ld3(st.v, i) : mk(0.0f, 0.0f, 0.0f);
   float dens = st.m_V[i] * cubic_W(p, 0.0f);
@@ -92,13 +93,13 @@ __global__ void __launch_bounds__(256) k_density_factor(dimcv_state st, dimcv_pa
   struct G { float4 xmv; v3 v; float m; };
   nbr_loop(nl, ns, nn,
            [&](uint32_t j) {
-             return G{nb_xmv<XM>(st, xm, j), CHANGE ? ld3s(st.v, j) : mk(0.0f, 0.0f, 0.0f), ld1s(st.m, j)};
+             return G{nb_xmv<XM>(st, pk.xm, j), CHANGE ? ld3s(st.v, j) : mk(0.0f, 0.0f, 0.0f), ld1s(st.m, j)};
            },
            [&](int, uint32_t e, const G& g) {
              const float mV = g.xmv.w;
              v3 r = rel(p, xi, xyz(g.xmv));
              den += mV * cubic_W(p, vnorm(r));
-             v3 gw = cubic_dW(p, r);
+             v3 gw = cubic_dW(p, r, sd);
              if (CHANGE) dadv += mV * vdot(vsub(vi, g.v), gw);
              if (nb_fluid(e)) {
                v3 gg = smul(g.m, gw);
@@ -175,24 +176,24 @@ __device__ __forceinline__ void residual_partial(double v, double* __restrict__ 
 template <bool XM>
 __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done,
-                                                  double* __restrict__ partials, const float4* __restrict__ xm,
-                                                  float* __restrict__ kap) {
+                                                  double* __restrict__ partials, Packed pk) {
   __shared__ double lds[256];
   if (jacobi_gated(st, done)) return;  // uniform per block
   double res = 0.0;
   [&] {
     FLUID_ENTRY();
+    const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float dadv = 0.0f;
     struct G { float4 xmv; v3 v; };
-    nbr_loop<8>(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, xm, j), ld3s(st.v, j)}; },
+    nbr_loop<8>(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, pk.xm, j), ld3s(st.v, j)}; },
                 [&](int, uint32_t, const G& g) {
-                  dadv += g.xmv.w * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xmv))));
+                  dadv += g.xmv.w * vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xmv)), sd));
                 });
     float d = fmaxf(dadv, 0.0f);
     if (st.nbr_count[f] < 20) d = 0.0f;
     st.density_adv[i] = d;
-    if (XM) kap[i] = d * st.dfsph_factor[i] * st.density[i];  // the next iteration's kappa_i
+    if (XM) pk.kap[i] = d * st.dfsph_factor[i] * st.density[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, 0.0f);
   }();
   if (partials) residual_partial(res, partials, lds);
@@ -200,19 +201,26 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
 
 // (x, m_V) of every slot as one 16-byte record, for the neighbour gathers
 // of a step (positions and volumes do not change until advection).
+// Also raises *tiny when a coordinate is nonzero and below 2^-76 in
+// magnitude (the condition under which cubic_dW must not share reciprocals).
+__device__ __forceinline__ bool tiny_coord(float c) { return c != 0.0f && fabsf(c) < 0x1p-76f; }
+
 __global__ void k_pack_xm(dimcv_state st, dimcv_params p, int32_t bound, float4* __restrict__ xm,
-                          float4* __restrict__ xw) {
+                          float4* __restrict__ xw, int32_t* __restrict__ tiny) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
   const float x = st.x[3 * s], y = st.x[3 * s + 1], z = st.x[3 * s + 2], mV = st.m_V[s];
   xm[s] = make_float4(x, y, z, mV);
   xw[s] = make_float4(x, y, z, st.material[s] == 1 ? st.m[s] : mV * p.density0);
+  if (tiny_coord(x) || tiny_coord(y) || tiny_coord(z)) atomicOr(tiny, 1);
 }
 
-int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, float4* xm, float4* xw,
+int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s) {
+  DIMCV_HIP(hipMemsetAsync((void*)pk.tiny, 0, sizeof(int32_t), s));
   if (particle_bound <= 0) return DIMCV_OK;
-  hipLaunchKernelGGL(k_pack_xm, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound, xm, xw);
+  hipLaunchKernelGGL(k_pack_xm, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound,
+                     (float4*)pk.xm, (float4*)pk.xw, (int32_t*)pk.tiny);
   DIMCV_LAUNCHED("k_pack_xm");
   return DIMCV_OK;
 }
@@ -275,24 +283,24 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
 // factors only for the pairs that pass |kappa_i + kappa_j| > eps.
 template <bool XM>
 __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done, const float4* __restrict__ xw,
-                                                  const float* __restrict__ kap) {
+                                                  const int32_t* __restrict__ done, Packed pk) {
   if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
+  const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
-  const float k_i = XM ? kap[i] : st.density_adv[i] * st.dfsph_factor[i] * st.density[i];
+  const float k_i = XM ? pk.kap[i] : st.density_adv[i] * st.dfsph_factor[i] * st.density[i];
   v3 dv = mk(0.0f, 0.0f, 0.0f);
   nbr_loop2<8>(
       nl, ns, nn,
       [&](uint32_t j) {
-        if constexpr (XM) return ld1s(kap, j);
+        if constexpr (XM) return ld1s(pk.kap, j);
         return ld1s(st.density_adv, j) * ld1s(st.dfsph_factor, j) * ld1s(st.density, j);
       },
       [&](uint32_t e, float k_j) { return fabsf(nb_fluid(e) ? k_i + k_j : k_i) > p.m_eps; },
-      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, xw, j, e); },
+      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, pk.xw, j, e); },
       [&](int, uint32_t e, float k_j, const float4& q) {
-        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q))));
+        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q)), sd));
         if (nb_fluid(e)) {
           float k_sum = k_i + k_j;
           dv = vsub(dv, smul(dt * k_sum, gg));
@@ -311,19 +319,20 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
 // boundary viscosity 0 contribute exact zeros and are not evaluated).
 template <bool XM>
 __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound,
-                                                           const float4* __restrict__ xm) {
+                                                           Packed pk) {
   FLUID_ENTRY();
+  const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float visc_coef = st.block_scene ? st.scenes[block_scene_of(st)].visc_coef : p.visc_coef;
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 dv = mk(p.gravity[0], p.gravity[1], p.gravity[2]);
   struct G { float4 xmv; v3 v; };
-  nbr_loop(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, xm, j), ld3s(st.v, j)}; },
+  nbr_loop(nl, ns, nn, [&](uint32_t j) { return G{nb_xmv<XM>(st, pk.xm, j), ld3s(st.v, j)}; },
            [&](int, uint32_t e, const G& g) {
              if (nb_fluid(e)) {
                v3 r = rel(p, xi, xyz(g.xmv));
                float v_xy = vdot(vsub(vi, g.v), r);
                float rn = vnorm(r);
-               v3 fv = smul(visc_coef * g.xmv.w * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r));
+               v3 fv = smul(visc_coef * g.xmv.w * v_xy / (rn * rn + p.visc_h2), cubic_dW(p, r, sd));
                dv = vadd(dv, fv);
              }
            });
@@ -367,12 +376,13 @@ template <bool XM>
 __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
                                                   const int32_t* __restrict__ done,
                                                   double* __restrict__ partials, float offset,
-                                                  const float4* __restrict__ xw, float* __restrict__ kap) {
+                                                  Packed pk) {
   __shared__ double lds[256];
   if (jacobi_gated(st, done)) return;  // uniform per block
   double res = 0.0;
   [&] {
     FLUID_ENTRY();
+    const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float delta = 0.0f;
     // w = m (fluid) or m_V rho0 (solid), as the reference weights the two
@@ -380,19 +390,19 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
     struct G { float4 xw; v3 v; };
     nbr_loop<8>(nl, ns, nn,
                 [&](uint32_t j) {
-                  if constexpr (XM) return G{nb_xw<true>(st, p, xw, j, 0u), ld3s(st.v, j)};
+                  if constexpr (XM) return G{nb_xw<true>(st, p, pk.xw, j, 0u), ld3s(st.v, j)};
                   const v3 a = ld3s(st.x, j);
                   return G{make_float4(a.x, a.y, a.z, ld1s(st.m, j)), ld3s(st.v, j)};
                 },
                 [&](int, uint32_t e, const G& g) {
-                  float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xw))));
+                  float dd = vdot(vsub(vi, g.v), cubic_dW(p, rel(p, xi, xyz(g.xw)), sd));
                   if (XM || nb_fluid(e)) delta += g.xw.w * dd;
                   else delta += ld1s(st.m_V, nb_slot(e)) * p.density0 * dd;
                 });
     float da = st.density[i] + p.dt * delta;
     const float d = fmaxf(da, p.density0);
     st.density_adv[i] = d;
-    if (XM) kap[i] = (d - p.density0) * st.dfsph_factor[i];  // the next iteration's kappa_i
+    if (XM) pk.kap[i] = (d - p.density0) * st.dfsph_factor[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, offset);
   }();
   if (partials) residual_partial(res, partials, lds);
@@ -403,24 +413,25 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
 template <bool XM>
 __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
                                                        const int32_t* __restrict__ done,
-                                                       const float4* __restrict__ xw, const float* __restrict__ kap) {
+                                                       Packed pk) {
   if (jacobi_gated(st, done)) return;
   FLUID_ENTRY();
+  const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
-  const float k_i = XM ? kap[i] : (st.density_adv[i] - p.density0) * st.dfsph_factor[i];
+  const float k_i = XM ? pk.kap[i] : (st.density_adv[i] - p.density0) * st.dfsph_factor[i];
   v3 vi = ld3(st.v, i);
   const float w = p.has_transit ? transit_weight(p, xi.x) : 1.0f;
   nbr_loop2<8>(
       nl, ns, nn,
       [&](uint32_t j) {
-        if constexpr (XM) return ld1s(kap, j);
+        if constexpr (XM) return ld1s(pk.kap, j);
         return (ld1s(st.density_adv, j) - p.density0) * ld1s(st.dfsph_factor, j);
       },
       [&](uint32_t e, float k_j) { return fabsf(nb_fluid(e) ? k_i + k_j : k_i) > p.m_eps; },
-      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, xw, j, e); },
+      [&](uint32_t j, uint32_t e) { return nb_xw<XM>(st, p, pk.xw, j, e); },
       [&](int, uint32_t e, float k_j, const float4& q) {
-        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q))));
+        const v3 gg = smul(q.w, cubic_dW(p, rel(p, xi, xyz(q)), sd));
         if (nb_fluid(e)) {
           float k_sum = k_i + k_j;
           if (!p.has_transit) vi = vsub(vi, smul(dt * k_sum, gg));
@@ -501,8 +512,9 @@ __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
 // v*_j = v_j + a_j dt is formed on the fly exactly as compute_v_star does.
 template <int MODE, bool XM>
 __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound,
-                                                    const float4* __restrict__ xm) {
+                                                    Packed pk) {
   FLUID_ENTRY();
+  const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
   v3 vort = mk(0.0f, 0.0f, 0.0f);
   v3 curl = mk(0.0f, 0.0f, 0.0f);
@@ -512,13 +524,13 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
   struct Gt { float4 xmv; v3 v, a; };
   nbr_loop(nl, ns, nn,
            [&](uint32_t j) {
-             return Gt{nb_xmv<XM>(st, xm, j), ld3s(st.v, j),
+             return Gt{nb_xmv<XM>(st, pk.xm, j), ld3s(st.v, j),
                        (MODE & 4) ? ld3s(st.acceleration, j) : mk(0.0f, 0.0f, 0.0f)};
            },
            [&](int, uint32_t e, const Gt& g) {
              if (!nb_fluid(e)) return;
              v3 vj = g.v;
-             v3 kd = cubic_dW(p, rel(p, xi, xyz(g.xmv)));
+             v3 kd = cubic_dW(p, rel(p, xi, xyz(g.xmv)), sd);
              v3 vji = vsub(vj, vi);
              float mv = g.xmv.w;
              if (MODE & 1) vort = vadd(vort, smul(mv, vcross(vji, kd)));
@@ -643,7 +655,7 @@ __device__ __forceinline__ float epan(const dimcv_params& p, v3 r) {
 template <bool XM>
 __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_params p, int32_t bound,
                                                          double* __restrict__ partials,
-                                                         const float4* __restrict__ xm) {
+                                                         Packed pk) {
   __shared__ double lds[256];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   double contrib = 0.0;
@@ -662,7 +674,7 @@ __global__ void __launch_bounds__(256) k_smooth_vis_kden(dimcv_state st, dimcv_p
     struct G { float4 xmv; v3 dv, vo; float kvn; };
     nbr_loop<8>(nl, ns, nn,
              [&](uint32_t j) {
-               return G{nb_xmv<XM>(st, xm, j), ld3s(st.d_vort, j), ld3s(st.vorticity, j),
+               return G{nb_xmv<XM>(st, pk.xm, j), ld3s(st.d_vort, j), ld3s(st.vorticity, j),
                         ld1s(st.kinematic_vort_num, j)};
              },
              [&](int, uint32_t e, const G& g) {
@@ -730,29 +742,26 @@ int compact_samples(const dimcv_state* st, int64_t n, Workspace& ws, hipStream_t
 // which 1 and 3 also write the residual's block partials when `partials`
 // is given (offset: that of the pressure residual).
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
-                       const int32_t* done, hipStream_t s, double* partials, float offset, const float4* xm,
-                       const float4* xw, float* kap) {
+                       const int32_t* done, hipStream_t s, double* partials, float offset, const Packed* pk) {
   if (fluid_bound <= 0) return DIMCV_OK;
   const dim3 g(nblocks(fluid_bound)), b(256);
-  const bool packed = xm && xw && kap;
+  const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
   switch (which) {
     case 0:
-      if (packed) hipLaunchKernelGGL(k_div_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
-      else hipLaunchKernelGGL(k_div_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      if (pk) hipLaunchKernelGGL(k_div_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, q);
+      else hipLaunchKernelGGL(k_div_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, q);
       break;
     case 1:
-      if (packed) hipLaunchKernelGGL(k_density_change<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, xm, kap);
-      else hipLaunchKernelGGL(k_density_change<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, xm, kap);
+      if (pk) hipLaunchKernelGGL(k_density_change<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, q);
+      else hipLaunchKernelGGL(k_density_change<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, q);
       break;
     case 2:
-      if (packed) hipLaunchKernelGGL(k_pressure_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
-      else hipLaunchKernelGGL(k_pressure_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, xw, kap);
+      if (pk) hipLaunchKernelGGL(k_pressure_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, q);
+      else hipLaunchKernelGGL(k_pressure_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, q);
       break;
     default:
-      if (packed)
-        hipLaunchKernelGGL(k_density_adv<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, xw, kap);
-      else
-        hipLaunchKernelGGL(k_density_adv<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, xw, kap);
+      if (pk) hipLaunchKernelGGL(k_density_adv<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, q);
+      else hipLaunchKernelGGL(k_density_adv<false>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, q);
       break;
   }
   DIMCV_LAUNCHED("jacobi pass");
@@ -760,24 +769,26 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 }
 
 // compute_non_pressure_forces (dimcv_sph.py:150-208): non-fluid slots, then
-// the fluid pass (gathering the packed (x, m_V) records when xm is given).
+// the fluid pass (gathering the packed records when pk is given).
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const float4* xm) {
+                       hipStream_t s, const Packed* pk) {
+  const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
   if (particle_bound > 0)
     hipLaunchKernelGGL(k_nonpressure_other, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p,
                        particle_bound);
   DIMCV_LAUNCHED("k_nonpressure_other");
   if (fluid_bound > 0) {
     const dim3 g(nblocks(fluid_bound)), b(256);
-    if (xm) hipLaunchKernelGGL(k_nonpressure_fluid<true>, g, b, 0, s, *st, *p, fluid_bound, xm);
-    else hipLaunchKernelGGL(k_nonpressure_fluid<false>, g, b, 0, s, *st, *p, fluid_bound, xm);
+    if (pk) hipLaunchKernelGGL(k_nonpressure_fluid<true>, g, b, 0, s, *st, *p, fluid_bound, q);
+    else hipLaunchKernelGGL(k_nonpressure_fluid<false>, g, b, 0, s, *st, *p, fluid_bound, q);
   }
   DIMCV_LAUNCHED("k_nonpressure_fluid");
   return DIMCV_OK;
 }
 
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
-                 double* partials, const float4* xm) {
+                 double* partials, const Packed* pk) {
+  const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
   if (which == 1) {  // v*, vorticity, grad_v, d_vort: zero the non-fluid slots first
     int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
     if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
@@ -794,20 +805,20 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
   dim3 g(nblocks(fluid_bound)), b(256);
   switch (which) {
     case 0:
-      if (xm) hipLaunchKernelGGL((k_density_factor<false, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
-      else hipLaunchKernelGGL((k_density_factor<false, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      if (pk) hipLaunchKernelGGL((k_density_factor<false, true>), g, b, 0, s, *st, *p, fluid_bound, q);
+      else hipLaunchKernelGGL((k_density_factor<false, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
     case 1:
-      if (xm) hipLaunchKernelGGL((k_vort_gradv<7, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
-      else hipLaunchKernelGGL((k_vort_gradv<7, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      if (pk) hipLaunchKernelGGL((k_vort_gradv<7, true>), g, b, 0, s, *st, *p, fluid_bound, q);
+      else hipLaunchKernelGGL((k_vort_gradv<7, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
     case 3:
-      if (xm) hipLaunchKernelGGL((k_density_factor<true, true>), g, b, 0, s, *st, *p, fluid_bound, xm);
-      else hipLaunchKernelGGL((k_density_factor<true, false>), g, b, 0, s, *st, *p, fluid_bound, xm);
+      if (pk) hipLaunchKernelGGL((k_density_factor<true, true>), g, b, 0, s, *st, *p, fluid_bound, q);
+      else hipLaunchKernelGGL((k_density_factor<true, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
     case 4:
-      if (xm) hipLaunchKernelGGL(k_smooth_vis_kden<true>, g, b, 0, s, *st, *p, fluid_bound, partials, xm);
-      else hipLaunchKernelGGL(k_smooth_vis_kden<false>, g, b, 0, s, *st, *p, fluid_bound, partials, xm);
+      if (pk) hipLaunchKernelGGL(k_smooth_vis_kden<true>, g, b, 0, s, *st, *p, fluid_bound, partials, q);
+      else hipLaunchKernelGGL(k_smooth_vis_kden<false>, g, b, 0, s, *st, *p, fluid_bound, partials, q);
       break;
     default: hipLaunchKernelGGL(k_smooth_vis, g, b, 0, s, *st, *p, fluid_bound); break;
   }
@@ -868,7 +879,7 @@ int dimcv_compute_DFSPH_factor(const dimcv_state* st, const dimcv_params* p, int
 }
 
 int dimcv_compute_density_change(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_change<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr, nullptr);
+  LAUNCH_FLUID(k_density_change<false>, *st, *p, fluid_bound, nullptr, nullptr, Packed{});
   return DIMCV_OK;
 }
 
@@ -900,7 +911,7 @@ int dimcv_compute_density_error(const dimcv_state* st, const dimcv_params* p, fl
 }
 
 int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_div_iter<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr);
+  LAUNCH_FLUID(k_div_iter<false>, *st, *p, fluid_bound, nullptr, Packed{});
   return DIMCV_OK;
 }
 
@@ -920,12 +931,12 @@ int dimcv_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_
 }
 
 int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_density_adv<false>, *st, *p, fluid_bound, nullptr, nullptr, 0.0f, nullptr, nullptr);
+  LAUNCH_FLUID(k_density_adv<false>, *st, *p, fluid_bound, nullptr, nullptr, 0.0f, Packed{});
   return DIMCV_OK;
 }
 
 int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
-  LAUNCH_FLUID(k_pressure_iter<false>, *st, *p, fluid_bound, nullptr, nullptr, nullptr);
+  LAUNCH_FLUID(k_pressure_iter<false>, *st, *p, fluid_bound, nullptr, Packed{});
   return DIMCV_OK;
 }
 
@@ -957,21 +968,21 @@ int dimcv_compute_v_star(const dimcv_state* st, const dimcv_params* p, int32_t f
 int dimcv_compute_vorticity(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->vorticity, 3);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv_1, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_vort_gradv_1, *st, *p, fluid_bound, Packed{});
   return DIMCV_OK;
 }
 
 int dimcv_compute_grad_v(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->grad_v, 9);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv_2, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_vort_gradv_2, *st, *p, fluid_bound, Packed{});
   return DIMCV_OK;
 }
 
 int dimcv_dimcv_init(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   int rc = zero_nonfluid(st, p, (hipStream_t)stream, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
   if (rc) return rc;
-  LAUNCH_FLUID(k_vort_gradv_3, *st, *p, fluid_bound, nullptr);
+  LAUNCH_FLUID(k_vort_gradv_3, *st, *p, fluid_bound, Packed{});
   return DIMCV_OK;
 }
 

@@ -14,17 +14,16 @@ namespace dimcv {
 
 int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        const int32_t* done, hipStream_t s, double* partials = nullptr,
-                       float offset = 0.0f, const float4* xm = nullptr, const float4* xw = nullptr,
-                       float* kap = nullptr);
-int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, float4* xm, float4* xw,
+                       float offset = 0.0f, const Packed* pk = nullptr);
+int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s);
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
                        float* kap, hipStream_t s);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const float4* xm);
+                       hipStream_t s, const Packed* pk);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
-                 double* partials = nullptr, const float4* xm = nullptr);
+                 double* partials = nullptr, const Packed* pk = nullptr);
 
 // Events for the optional far-field timing (created on first use, kept for
 // the life of the process).
@@ -45,17 +44,16 @@ static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv
   const double eta = which == 0 ? o->eta_v : o->eta_p;
   const float offset = which == 0 ? 0.0f : o->density0;
   const int chunk = o->chunk > 0 ? o->chunk : 1;
+  const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   hipLaunchKernelGGL(k_jacobi_reset, dim3(1), dim3(1), 0, s, st->counters);
   int32_t host[2] = {0, 0};
   for (;;) {
     for (int c = 0; c < chunk; ++c) {
       // iteration pass; density pass with the residual's block partials;
       // the FP64 reduction and the convergence test in one launch
-      int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s, nullptr, 0.0f, ws.xm, ws.xw,
-                                  ws.kap);
+      int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s, nullptr, 0.0f, &pk);
       if (!rc)
-        rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s, ws.partials, offset, ws.xm, ws.xw,
-                                ws.kap);
+        rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s, ws.partials, offset, &pk);
       if (!rc)
         rc = reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_RESIDUAL, st->counters, done, 1,
                              o->fluid_count, eta, max_iter, s);
@@ -102,24 +100,25 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
   }
   Workspace ws;
   CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
-  CHECK(launch_pack_xm(st, p, particle_bound, ws.xm, ws.xw, s));
+  const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
+  CHECK(launch_pack_xm(st, p, particle_bound, pk, s));
 
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
-  CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, ws.xm));
+  CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, &pk));
   CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s));
   CHECK(jacobi(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
-  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, ws.xm));
+  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk));
   // dimcv() (dimcv_sph.py:812-819)
   // init() + compute_d_vorticity in one pass; then the KVN (and Bernoulli
   // sample update), which reads only grad_v; then the smoothing pass, which
   // also produces vorticity_vis (compute_vorticity_vis below in the
   // reference) and the kernel density with its kden_sum partials.
-  CHECK(launch_fused(1, st, p, fluid_bound, s, nullptr, ws.xm));
+  CHECK(launch_fused(1, st, p, fluid_bound, s, nullptr, &pk));
   if (o->m > 0) CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
   else CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
-  CHECK(launch_fused(4, st, p, fluid_bound, s, ws.partials, ws.xm));
+  CHECK(launch_fused(4, st, p, fluid_bound, s, ws.partials, &pk));
   CHECK(reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_KDEN_SUM, st->counters, nullptr, 0,
                         0, 0.0, 0, s));
   if (o->m > 0) CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
@@ -137,7 +136,7 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
   CHECK(dimcv_update_particle_life_time(st, p, fluid_bound, stream));
   CHECK(dimcv_predict_velocity(st, p, particle_bound, stream));
   // pressure_solve (dimcv_sph.py:597-614)
-  CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, ws.xm, ws.xw, ws.kap));
+  CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, &pk));
   CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s));
   CHECK(jacobi(1, st, p, o, fluid_bound, ws, s, &o->out_kp));
   CHECK(dimcv_copy_x_temp(st, p, particle_bound, stream));
