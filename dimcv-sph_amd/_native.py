@@ -176,6 +176,7 @@ def lib():
         "dimcv_direct_bs": (I32, [P, P, I32, P, I32, P]),
         "dimcv_ring_velocity": (I32, [I64, P, F64, I64, P, P, P]),
         "dimcv_substep": (I32, [P, P, P, I32, I32, P]),
+        "dimcv_step": (I32, [P, P, P, P, I32, P, P]),
         "dimcv_batch_workspace_bytes": (I64, [I32, P, I64]),
         "dimcv_batch_grid_build": (I32, [P, P, P]),
         "dimcv_batch_neighbor_build": (I32, [P, P, P]),

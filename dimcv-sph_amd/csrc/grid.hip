@@ -660,6 +660,32 @@ int batch_nbr_launch(const dimcv_state* st, const dimcv_params* p, hipStream_t s
   return DIMCV_OK;
 }
 
+// The list build without the width check (the caller reads DIMCV_C_NBR_MAX).
+int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s) {
+  if (p->n_max > DIMCV_MAX_SLOTS) {
+    set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
+    return DIMCV_ERR_ARG;
+  }
+  if (fluid_bound > 0) {
+    // with a workspace, candidates are read from packed (x, material) records
+    Workspace ws;
+    const bool packed = !p->periodic && st->workspace &&
+                        st->workspace_bytes >= workspace_layout(p->n_max, p->n_cells, 1, nullptr, nullptr) &&
+                        bind_workspace(st, p, &ws, 0) == DIMCV_OK;
+    const int32_t pb = (int32_t)std::min<int64_t>(p->n_max, INT32_MAX);
+    if (packed) {
+      hipLaunchKernelGGL(k_pack_candidates, dim3(nblocks(pb)), dim3(256), 0, s, *st, pb, ws.xw);
+      hipLaunchKernelGGL(k_nbr_build<CandRecords>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p,
+                         fluid_bound, CandRecords{ws.xw});
+    } else {
+      hipLaunchKernelGGL(k_nbr_build<CandFields>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
+                         CandFields{st->x, st->material});
+    }
+  }
+  DIMCV_LAUNCHED("k_nbr_build");
+  return DIMCV_OK;
+}
+
 }  // namespace dimcv
 
 using namespace dimcv;
@@ -741,30 +767,12 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
   return DIMCV_OK;
 }
 
+
 int dimcv_neighbor_build(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                          void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (p->n_max > DIMCV_MAX_SLOTS) {
-    set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
-    return DIMCV_ERR_ARG;
-  }
-  if (fluid_bound > 0) {
-    // with a workspace, candidates are read from packed (x, material) records
-    Workspace ws;
-    const bool packed = !p->periodic && st->workspace && p->n_max <= DIMCV_MAX_SLOTS &&
-                        st->workspace_bytes >= workspace_layout(p->n_max, p->n_cells, 1, nullptr, nullptr) &&
-                        bind_workspace(st, p, &ws, 0) == DIMCV_OK;
-    const int32_t pb = (int32_t)std::min<int64_t>(p->n_max, INT32_MAX);
-    if (packed) {
-      hipLaunchKernelGGL(k_pack_candidates, dim3(nblocks(pb)), dim3(256), 0, s, *st, pb, ws.xw);
-      hipLaunchKernelGGL(k_nbr_build<CandRecords>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p,
-                         fluid_bound, CandRecords{ws.xw});
-    } else {
-      hipLaunchKernelGGL(k_nbr_build<CandFields>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
-                         CandFields{st->x, st->material});
-    }
-  }
-  DIMCV_LAUNCHED("k_nbr_build");
+  int rc = dimcv::neighbor_build_launch(st, p, fluid_bound, s);
+  if (rc) return rc;
   int32_t mx = 0;
   DIMCV_HIP(hipMemcpyAsync(&mx, st->counters + DIMCV_C_NBR_MAX, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DIMCV_HIP(hipStreamSynchronize(s));

@@ -22,6 +22,7 @@ int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, 
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
                        hipStream_t s, const Packed* pk);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
+int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials = nullptr, const Packed* pk = nullptr);
 
@@ -37,8 +38,9 @@ __global__ void k_jacobi_reset(int32_t* counters) {
 
 // which = 0: divergence (iteration, density change, residual offset 0)
 // which = 1: pressure (iteration, density_adv, residual offset rho0)
+// With `all`, each host check reads the whole counter array into it.
 static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
-                  int32_t fluid_bound, Workspace& ws, hipStream_t s, int32_t* iters_out) {
+                  int32_t fluid_bound, Workspace& ws, hipStream_t s, int32_t* iters_out, int32_t* all = nullptr) {
   const int32_t* done = st->counters + DIMCV_C_JACOBI_DONE;
   const int32_t max_iter = which == 0 ? o->max_iter_v : o->max_iter_p;
   const double eta = which == 0 ? o->eta_v : o->eta_p;
@@ -46,7 +48,8 @@ static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv
   const int chunk = o->chunk > 0 ? o->chunk : 1;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   hipLaunchKernelGGL(k_jacobi_reset, dim3(1), dim3(1), 0, s, st->counters);
-  int32_t host[2] = {0, 0};
+  int32_t pair[2] = {0, 0};
+  int32_t* host = all ? all + DIMCV_C_JACOBI_DONE : pair;
   for (;;) {
     for (int c = 0; c < chunk; ++c) {
       // iteration pass; density pass with the residual's block partials;
@@ -60,8 +63,11 @@ static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv
       if (rc) return rc;
     }
     DIMCV_LAUNCHED("jacobi chunk");
-    DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, 2 * sizeof(int32_t),
-                             hipMemcpyDeviceToHost, s));
+    if (all)
+      DIMCV_HIP(hipMemcpyAsync(all, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    else
+      DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, 2 * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s));
     DIMCV_HIP(hipStreamSynchronize(s));
     if (host[0]) break;
   }
@@ -79,25 +85,11 @@ using namespace dimcv;
     if (rc_) return rc_;   \
   } while (0)
 
-extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
-                             int32_t particle_bound, int32_t fluid_bound, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (!st || !p || !o) {
-    set_error("dimcv_substep: null argument");
-    return DIMCV_ERR_ARG;
-  }
-  if (o->fluid_count <= 0) {
-    set_error("dimcv_substep: no fluid particles (the reference divides by fluid_particle_num)");
-    return DIMCV_ERR_ARG;
-  }
-  if (o->m == 0 && (!o->u1 || !o->u2)) {
-    set_error("dimcv_substep: Bernoulli uniforms required");
-    return DIMCV_ERR_ARG;
-  }
-  if (o->m > 0 && !o->u_cdf) {
-    set_error("dimcv_substep: inverse-CDF uniforms required");
-    return DIMCV_ERR_ARG;
-  }
+// The substep on `st`; with `all` the pressure solve's last host check
+// leaves the whole counter array there.
+static int substep_impl(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t particle_bound,
+                        int32_t fluid_bound, hipStream_t s, int32_t* all) {
+  void* stream = (void*)s;
   Workspace ws;
   CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
@@ -138,7 +130,7 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
   // pressure_solve (dimcv_sph.py:597-614)
   CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, &pk));
   CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s));
-  CHECK(jacobi(1, st, p, o, fluid_bound, ws, s, &o->out_kp));
+  CHECK(jacobi(1, st, p, o, fluid_bound, ws, s, &o->out_kp, all));
   CHECK(dimcv_copy_x_temp(st, p, particle_bound, stream));
   CHECK(dimcv_advect(st, p, particle_bound, stream));
   if (o->bounce_back) CHECK(dimcv_enforce_boundary_3d(st, p, particle_bound, stream));
@@ -146,6 +138,68 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
     // the pressure solve synchronised the stream after the far field
     DIMCV_HIP(hipEventSynchronize(g_step_ev[1]));
     DIMCV_HIP(hipEventElapsedTime(&o->out_far_ms, g_step_ev[0], g_step_ev[1]));
+  }
+  return DIMCV_OK;
+}
+
+extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                             int32_t particle_bound, int32_t fluid_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!st || !p || !o) {
+    set_error("dimcv_substep: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->fluid_count <= 0) {
+    set_error("dimcv_substep: no fluid particles (the reference divides by fluid_particle_num)");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m == 0 && (!o->u1 || !o->u2)) {
+    set_error("dimcv_substep: Bernoulli uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m > 0 && !o->u_cdf) {
+    set_error("dimcv_substep: inverse-CDF uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  return substep_impl(st, p, o, particle_bound, fluid_bound, s, nullptr);
+}
+
+extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                          int32_t particle_bound, int32_t* host_counters, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!pre || !st || !p || !o || !host_counters) {
+    set_error("dimcv_step: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m == 0 && (!o->u1 || !o->u2)) {
+    set_error("dimcv_step: Bernoulli uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  if (o->m > 0 && !o->u_cdf) {
+    set_error("dimcv_step: inverse-CDF uniforms required");
+    return DIMCV_ERR_ARG;
+  }
+  if (p->n_max > DIMCV_MAX_SLOTS) {
+    set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
+    return DIMCV_ERR_ARG;
+  }
+  // grid build into the sorted buffers of `pre` (= the fields of `st`), lists
+  // and substep, all bounded by particle_bound; the counts stay on the device
+  CHECK(dimcv_grid_build(pre, p, particle_bound, nullptr, stream));
+  CHECK(neighbor_build_launch(st, p, particle_bound, s));
+  o->fluid_count = 0;  // the residual divisor is read from the device counters
+  CHECK(substep_impl(st, p, o, particle_bound, particle_bound, s, host_counters));
+  if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
+    set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
+    return DIMCV_ERR_KEY;
+  }
+  if (host_counters[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
+    set_error("neighbour list width %d exceeds nbr_kmax %d", host_counters[DIMCV_C_NBR_MAX], p->nbr_kmax);
+    return DIMCV_ERR_NBR;
+  }
+  if (host_counters[DIMCV_C_FLUID_NUM] <= 0) {
+    set_error("dimcv_step: no fluid particles (the reference divides by fluid_particle_num)");
+    return DIMCV_ERR_ARG;
   }
   return DIMCV_OK;
 }

@@ -172,7 +172,9 @@ __global__ void k_reduce_tail(double* a, int64_t n, double* out, int32_t* counte
   *out = n > 0 ? r : 0.0;
   if (check) {
     const int32_t it = ++counters[DIMCV_C_JACOBI_ITERS];
-    const double avg = r / (double)fluid_count;
+    // fluid_count 0: the fluid count of the last grid build, on the device
+    const int32_t fc = fluid_count > 0 ? fluid_count : counters[DIMCV_C_FLUID_NUM];
+    const double avg = r / (double)fc;
     if (avg <= eta || it >= max_iter) counters[DIMCV_C_JACOBI_DONE] = 1;
   }
 }

@@ -85,6 +85,7 @@ class DIMCVSPHSolver(SPHBase):
         self.far_mode = {"exact": N.FAR_EXACT, "fast": N.FAR_FAST, "kahan": N.FAR_KAHAN}[far]
         seed = ps.cfg.get_cfg("rngSeed")
         self.rng = np.random.default_rng(0 if seed is None else seed)
+        self._staged = {}             # pinned host + device buffers of the per-step uniforms
         self._uniforms = (None, None, None)
         self.last_iterations_v = 0
         self.last_iterations_p = 0
@@ -264,6 +265,22 @@ class DIMCVSPHSolver(SPHBase):
     def _upload(self, a, dtype):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.ps.device)
 
+    def _upload_staged(self, key, a, dtype):
+        """Copy `a` into a device buffer kept for `key`, through a pinned host
+        buffer (an asynchronous copy on the current stream).  The host buffer
+        is reused by the next call on that key, which comes after the step's
+        own synchronisations have retired this copy."""
+        a = np.ascontiguousarray(a)
+        bufs = self._staged.get(key)
+        if bufs is None or bufs[0].numel() < a.size or bufs[0].dtype != dtype:
+            host = torch.empty(max(a.size, 1), dtype=dtype, pin_memory=True)
+            dev = torch.empty(max(a.size, 1), dtype=dtype, device=self.ps.device)
+            bufs = self._staged[key] = (host, dev)
+        host, dev = bufs
+        host[: a.size].numpy()[:] = a
+        dev[: a.size].copy_(host[: a.size], non_blocking=True)
+        return dev
+
     def compute_kinematic_vorticity_num_sample(self, u1=None, u2=None):
         """KVN + Bernoulli sample update; u1/u2 replace the two ti.random() draws."""
         if self.mc_samples:
@@ -337,19 +354,15 @@ class DIMCVSPHSolver(SPHBase):
         self.update_particle_life_time()
 
     # ------------------------------------------------- native substep --
-    def native_substep(self, u1=None, u2=None, u_cdf=None):
-        """substep() issued by libdimcv in one call (dimcv_substep): same kernels
-        in the same order; the Jacobi loops converge on the device and the host
-        checks every ``jacobiChunk`` iterations."""
+    def _step_opts(self, u1, u2, u_cdf):
+        """dimcv_step_opts of this step (uniforms uploaded; fluid_count unset)."""
         ps = self.ps
         o = N.StepOpts()
-        keep = []
         if self.mc_samples:
             m = int(self.mc_samples)
             if u_cdf is None:
                 u_cdf = self.rng.random(m)
-            t = self._upload(np.asarray(u_cdf)[:m], torch.float64)
-            keep.append(t)
+            t = self._upload_staged("u_cdf", np.asarray(u_cdf)[:m], torch.float64)
             o.u_cdf, o.m = t.data_ptr(), m
         else:
             pn = ps._pn
@@ -358,8 +371,8 @@ class DIMCVSPHSolver(SPHBase):
             if u2 is None:
                 u2 = self.rng.random(pn, dtype=np.float32)
             assert len(u1) >= pn and len(u2) >= pn
-            t1, t2 = self._upload(u1, torch.float32), self._upload(u2, torch.float32)
-            keep += [t1, t2]
+            t1 = self._upload_staged("u1", np.asarray(u1)[:pn], torch.float32)
+            t2 = self._upload_staged("u2", np.asarray(u2)[:pn], torch.float32)
             o.u1, o.u2, o.m = t1.data_ptr(), t2.data_ptr(), 0
         o.max_iter_v = self.m_max_iterations_v
         o.max_iter_p = self.m_max_iterations
@@ -370,18 +383,51 @@ class DIMCVSPHSolver(SPHBase):
         o.dt = np.float32(self.dt[None])
         o.inv_dt2 = np.float32(self.inv_dt2)
         o.density0 = np.float32(self.density_0)
-        o.fluid_count = self._fluid_count()
-        if o.fluid_count <= 0:
-            raise ZeroDivisionError("no fluid particles (fluid_particle_num is 0)")
         o.bounce_back = 0
         o.time_far = 1 if self.time_far_field else 0
-        ps.ensure_neighbor_lists()
-        ps.call("dimcv_substep", ctypes.byref(o), ps._pn, ps._nfl, ps.stream())
+        return o
+
+    def _step_outputs(self, o):
         self.last_iterations_v = int(o.out_kv)
         self.last_iterations_p = int(o.out_kp)
         self.last_far_ms = float(o.out_far_ms) if self.time_far_field else None
-        ps._nbr_valid = False
-        del keep
+        self.ps._nbr_valid = False
+
+    def native_substep(self, u1=None, u2=None, u_cdf=None):
+        """substep() issued by libdimcv in one call (dimcv_substep): same kernels
+        in the same order; the Jacobi loops converge on the device and the host
+        checks every ``jacobiChunk`` iterations."""
+        ps = self.ps
+        o = self._step_opts(u1, u2, u_cdf)
+        o.fluid_count = self._fluid_count()
+        if o.fluid_count <= 0:
+            raise ZeroDivisionError("no fluid particles (fluid_particle_num is 0)")
+        ps.ensure_neighbor_lists()
+        ps.call("dimcv_substep", ctypes.byref(o), ps._pn, ps._nfl, ps.stream())
+        self._step_outputs(o)
+
+    def native_step(self, u_cdf=None):
+        """initialize_particle_system() + substep() in one library call
+        (dimcv_step): the grid build, the neighbour lists and the substep run
+        back to back on the device, bounded by the particle count before the
+        build; the counts come back with the pressure solve's last check.
+        Fixed-M sampling only (the Bernoulli uniforms are drawn per particle
+        after the build)."""
+        ps = self.ps
+        o = self._step_opts(None, None, u_cdf)
+        pre = ps.state()
+        for name in N.SORTED_FIELDS:
+            getattr(ps, name).swap(getattr(ps, name + "_buffer"))
+        ps._parity ^= 1
+        ps._state = None
+        post = ps.state()
+        hc = ps._host_counters
+        rc = N.lib().dimcv_step(ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps.params), ctypes.byref(o),
+                                int(ps._pn), hc.ctypes.data, ps.stream())
+        ps._pn = int(hc[N.C_PARTICLE_NUM])
+        ps._nfl = int(hc[N.C_FLUID_LIST])
+        N.check(rc, "dimcv_step")
+        self._step_outputs(o)
 
     # --------------------------------------------------- comparators --
     def direct_biot_savart(self, precision: str = "f32"):

@@ -3,7 +3,9 @@
 ``KarmanVortexSolver`` keeps the cylinder carving (karman_vortex.py:10-36),
 the step/substep order (:216-242) and the exporters (:38-214).  ``step``
 issues the substep through one native call (``dimcv_substep``) unless
-``use_native_substep`` is False or a subclass overrides ``substep``.  ``step``
+``use_native_substep`` is False or a subclass overrides ``substep``; with
+fixed-M sampling the grid build and the neighbour lists join that call
+(``dimcv_step``).  ``step``
 accepts optional host uniforms so that two implementations can be driven with
 identical random inputs (``u1``/``u2``: per-slot f32 arrays for the Bernoulli
 sample update; ``u_cdf``: f64 array of M uniforms for inverse-CDF sampling).
@@ -111,11 +113,18 @@ class KarmanVortexSolver(DIMCVSPHSolver):
             self.emit_particle()
             self.dump_num_particles_each_emitters_np2ti()
         self.cnt += 1
-        self.ps.initialize_particle_system()
-        self.compute_moving_boundary_volume()
-        if self.use_native_substep and type(self).substep is KarmanVortexSolver.substep:
+        native = self.use_native_substep and type(self).substep is KarmanVortexSolver.substep
+        if native and self.mc_samples and type(self).compute_moving_boundary_volume is \
+                DIMCVSPHSolver.compute_moving_boundary_volume:
+            # fixed-M sampling: grid build, lists and substep in one library call
+            self.native_step(u_cdf)
+        elif native:
+            self.ps.initialize_particle_system()
+            self.compute_moving_boundary_volume()
             self.native_substep(u1, u2, u_cdf)
         else:
+            self.ps.initialize_particle_system()
+            self.compute_moving_boundary_volume()
             self._uniforms = (u1, u2, u_cdf)
             try:
                 self.substep()

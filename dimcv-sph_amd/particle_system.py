@@ -151,6 +151,10 @@ class ParticleSystem:
         self._nbr_valid = False
         self.solver = None
         self._state = None
+        # the grid build swaps the sorted fields with their buffers every step,
+        # so the state alternates between two pointer sets: one cached per parity
+        self._state_cache = {}
+        self._parity = 0
 
         self.params = N.Params()
         self._init_params()
@@ -213,6 +217,10 @@ class ParticleSystem:
         """The dimcv_state of the current buffers (cached; rebuilt after a swap)."""
         if self._state is not None:
             return self._state
+        s = self._state_cache.get(self._parity)
+        if s is not None:
+            self._state = s
+            return s
         s = N.State()
         for name in N.STATE_POINTERS:
             if name in ("d_vort", "d_vort_smoothed", "v_star", "grad_v", "x_temp", "sample_idx",
@@ -234,10 +242,13 @@ class ParticleSystem:
         s.workspace = self._workspace.data_ptr()
         s.workspace_bytes = self._workspace.numel()
         self._state = s
+        self._state_cache[self._parity] = s
         return s
 
     def invalidate_state(self):
+        """Forget the cached states (after any change of a field's storage)."""
         self._state = None
+        self._state_cache.clear()
 
     @staticmethod
     def stream() -> int:
@@ -320,6 +331,7 @@ class ParticleSystem:
         self.call("dimcv_grid_build", int(self._pn), hc.ctypes.data, self.stream())
         for name in N.SORTED_FIELDS:
             getattr(self, name).swap(getattr(self, name + "_buffer"))
+        self._parity ^= 1
         self._state = None
         self._pn = int(hc[N.C_PARTICLE_NUM])
         self._nfl = int(hc[N.C_FLUID_LIST])

@@ -403,6 +403,23 @@ typedef struct dimcv_step_opts {
 int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
                   int32_t particle_bound, int32_t fluid_bound, void* stream);
 
+/* One step after emission: ParticleSystem.initialize_particle_system
+ * (particle_system.py:555-560) and the step's neighbour lists, then the
+ * substep, in one call (karman_vortex.py:219-224 without the host round
+ * trips between them).  `pre` is the state before the grid build; `st` is
+ * the same state with every sorted field exchanged with its _buffer twin
+ * (the state the caller uses after the build).  particle_bound is the
+ * particle count before the build; all launches are bounded by it and the
+ * kernels read the exact counts from the device.  opts->fluid_count is
+ * ignored (the residual divisor is the device's fluid count).  The counter
+ * array (DIMCV_C_COUNT) is copied to host_counters at the pressure solve's
+ * last convergence check; the grid and list checks of dimcv_grid_build and
+ * dimcv_neighbor_build (DIMCV_ERR_KEY, DIMCV_ERR_NBR) are made from it after
+ * the step, so a failing step has already run on truncated lists or
+ * out-of-grid keys when the error is returned. */
+int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
+               int32_t particle_bound, int32_t* host_counters, void* stream);
+
 /* ---- comparators and initial conditions (build-defined) --------------------- */
 /* Direct O(N^2) Biot-Savart over every in-domain fluid particle: the sum the
  * far-field Monte Carlo term estimates.  precision: 0 = FP32, 1 = FP32 Kahan,
