@@ -314,7 +314,8 @@ template <class Pred>
 static int compact(Pred pred, int64_t n, int32_t* out, int32_t* total, Workspace& ws, hipStream_t s) {
   unsigned nb = nblocks(n);
   if (nb == 0) {
-    DIMCV_HIP(hipMemsetAsync(total, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, total);
+    DIMCV_LAUNCHED("k_zero_i32");
     return DIMCV_OK;
   }
   hipLaunchKernelGGL(k_compact_count<Pred>, dim3(nb), dim3(256), 0, s, pred, n, ws.compact_cnt);
@@ -743,7 +744,7 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
     return DIMCV_ERR_ARG;
   }
   const int64_t n_in = prev_particle_num;
-  DIMCV_HIP(hipMemsetAsync(st->counters + DIMCV_C_KEY_ERROR, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, st->counters + DIMCV_C_KEY_ERROR);
   if (n_in > 0)
     hipLaunchKernelGGL(k_activity_keys, dim3(nblocks(n_in)), dim3(256), 0, s, *st, *p, n_in, ws.keys[0],
                        ws.vals[0]);

@@ -217,7 +217,7 @@ __global__ void k_pack_xm(dimcv_state st, dimcv_params p, int32_t bound, float4*
 
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s) {
-  DIMCV_HIP(hipMemsetAsync((void*)pk.tiny, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, (int32_t*)pk.tiny);
   if (particle_bound <= 0) return DIMCV_OK;
   hipLaunchKernelGGL(k_pack_xm, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound,
                      (float4*)pk.xm, (float4*)pk.xw, (int32_t*)pk.tiny);

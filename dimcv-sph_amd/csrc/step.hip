@@ -8,6 +8,9 @@
 // kernel that raises a device flag; every kernel of later iterations reads
 // the flag first and returns, so the host may issue `chunk` iterations
 // between synchronisations without changing any result.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 
 namespace dimcv {
@@ -38,41 +41,84 @@ __global__ void k_jacobi_reset(int32_t* counters) {
 
 // which = 0: divergence (iteration, density change, residual offset 0)
 // which = 1: pressure (iteration, density_adv, residual offset rho0)
-// With `all`, each host check reads the whole counter array into it.
-static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
-                  int32_t fluid_bound, Workspace& ws, hipStream_t s, int32_t* iters_out, int32_t* all = nullptr) {
+// Iterations are issued in chunks; the first chunk is jacobi_first_chunk()
+// (chunk 0: the previous solve's iteration count passed in *iters, at least
+// one), later ones `chunk` (or one).
+static int jacobi_first_chunk(int which, const dimcv_step_opts* o, int32_t iters) {
+  const int32_t max_iter = which == 0 ? o->max_iter_v : o->max_iter_p;
+  return o->chunk > 0 ? o->chunk : std::max(1, std::min(iters, max_iter));
+}
+
+// The iteration reset and `n` iterations (kernels only).
+static int jacobi_issue(int which, const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o,
+                        int32_t fluid_bound, const Workspace& ws, hipStream_t s, int n, bool reset) {
   const int32_t* done = st->counters + DIMCV_C_JACOBI_DONE;
   const int32_t max_iter = which == 0 ? o->max_iter_v : o->max_iter_p;
   const double eta = which == 0 ? o->eta_v : o->eta_p;
   const float offset = which == 0 ? 0.0f : o->density0;
-  const int chunk = o->chunk > 0 ? o->chunk : 1;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
-  hipLaunchKernelGGL(k_jacobi_reset, dim3(1), dim3(1), 0, s, st->counters);
-  int32_t pair[2] = {0, 0};
-  int32_t* host = all ? all + DIMCV_C_JACOBI_DONE : pair;
-  for (;;) {
-    for (int c = 0; c < chunk; ++c) {
-      // iteration pass; density pass with the residual's block partials;
-      // the FP64 reduction and the convergence test in one launch
-      int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s, nullptr, 0.0f, &pk);
-      if (!rc)
-        rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s, ws.partials, offset, &pk);
-      if (!rc)
-        rc = reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_RESIDUAL, st->counters, done, 1,
-                             o->fluid_count, eta, max_iter, s);
-      if (rc) return rc;
-    }
-    DIMCV_LAUNCHED("jacobi chunk");
-    if (all)
-      DIMCV_HIP(hipMemcpyAsync(all, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    else
-      DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, 2 * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, s));
-    DIMCV_HIP(hipStreamSynchronize(s));
-    if (host[0]) break;
+  if (reset) hipLaunchKernelGGL(k_jacobi_reset, dim3(1), dim3(1), 0, s, st->counters);
+  for (int c = 0; c < n; ++c) {
+    // iteration pass; density pass with the residual's block partials;
+    // the FP64 reduction and the convergence test in one launch
+    int rc = launch_jacobi_pass(which == 0 ? 0 : 2, st, p, fluid_bound, done, s, nullptr, 0.0f, &pk);
+    if (!rc) rc = launch_jacobi_pass(which == 0 ? 1 : 3, st, p, fluid_bound, done, s, ws.partials, offset, &pk);
+    if (!rc)
+      rc = reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_RESIDUAL, st->counters, done, 1,
+                           o->fluid_count, eta, max_iter, s);
+    if (rc) return rc;
   }
-  *iters_out = host[1];
+  DIMCV_LAUNCHED("jacobi chunk");
   return DIMCV_OK;
+}
+
+// The counter array read back to the host for a convergence check: an
+// asynchronous copy into pinned memory and an event the host polls, which
+// returns within microseconds of the copy (a blocking synchronisation waits
+// for the runtime to wake the thread).  Buffer and event are made once and
+// kept for the life of the process.
+static int32_t* g_check_host = nullptr;
+static hipEvent_t g_check_ev = nullptr;
+
+static int read_counters(const dimcv_state* st, hipStream_t s, int32_t* out) {
+  if (!g_check_host) {
+    DIMCV_HIP(hipHostMalloc((void**)&g_check_host, DIMCV_C_COUNT * sizeof(int32_t), hipHostMallocDefault));
+    DIMCV_HIP(hipEventCreateWithFlags(&g_check_ev, hipEventDisableTiming));
+  }
+  DIMCV_HIP(hipMemcpyAsync(g_check_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipEventRecord(g_check_ev, s));
+  hipError_t e;
+  while ((e = hipEventQuery(g_check_ev)) == hipErrorNotReady) {
+  }
+  DIMCV_HIP(e);
+  std::copy(g_check_host, g_check_host + DIMCV_C_COUNT, out);
+  return DIMCV_OK;
+}
+
+// Host checks of an issued solve until it has converged, issuing further
+// chunks.  With `all`, each check reads the whole counter array into it.
+static int jacobi_wait(int which, const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o,
+                       int32_t fluid_bound, const Workspace& ws, hipStream_t s, int32_t* iters_out,
+                       int32_t* all = nullptr) {
+  int32_t local[DIMCV_C_COUNT];
+  int32_t* c = all ? all : local;
+  const int chunk = o->chunk > 0 ? o->chunk : 1;
+  for (;;) {
+    int rc0 = read_counters(st, s, c);
+    if (rc0) return rc0;
+    if (c[DIMCV_C_JACOBI_DONE]) break;
+    int rc = jacobi_issue(which, st, p, o, fluid_bound, ws, s, chunk, false);
+    if (rc) return rc;
+  }
+  *iters_out = c[DIMCV_C_JACOBI_ITERS];
+  return DIMCV_OK;
+}
+
+static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                  int32_t fluid_bound, const Workspace& ws, hipStream_t s, int32_t* iters, int32_t* all = nullptr) {
+  int rc = jacobi_issue(which, st, p, o, fluid_bound, ws, s, jacobi_first_chunk(which, o, *iters), true);
+  if (rc) return rc;
+  return jacobi_wait(which, st, p, o, fluid_bound, ws, s, iters, all);
 }
 
 }  // namespace dimcv
@@ -85,21 +131,27 @@ using namespace dimcv;
     if (rc_) return rc_;   \
   } while (0)
 
-// The substep on `st`; with `all` the pressure solve's last host check
-// leaves the whole counter array there.
-static int substep_impl(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t particle_bound,
-                        int32_t fluid_bound, hipStream_t s, int32_t* all) {
-  void* stream = (void*)s;
-  Workspace ws;
-  CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
+// The substep up to the divergence solve's first host check: kernels only
+// (no host synchronisation), so it can be captured into a graph.
+static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o,
+                        int32_t particle_bound, int32_t fluid_bound, hipStream_t s, const Workspace& ws,
+                        int n_first) {
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   CHECK(launch_pack_xm(st, p, particle_bound, pk, s));
-
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
   CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, &pk));
   CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s));
-  CHECK(jacobi(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
+  return jacobi_issue(0, st, p, o, fluid_bound, ws, s, n_first, true);
+}
+
+// The rest of the substep; with `all` the pressure solve's last host check
+// leaves the whole counter array there.
+static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t particle_bound,
+                        int32_t fluid_bound, hipStream_t s, const Workspace& ws, int32_t* all) {
+  void* stream = (void*)s;
+  const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
+  CHECK(jacobi_wait(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
   CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk));
   // dimcv() (dimcv_sph.py:812-819)
@@ -161,7 +213,66 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
     set_error("dimcv_substep: inverse-CDF uniforms required");
     return DIMCV_ERR_ARG;
   }
-  return substep_impl(st, p, o, particle_bound, fluid_bound, s, nullptr);
+  Workspace ws;
+  CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
+  CHECK(substep_head(st, p, o, particle_bound, fluid_bound, s, ws, jacobi_first_chunk(0, o, o->out_kv)));
+  return substep_tail(st, p, o, particle_bound, fluid_bound, s, ws, nullptr);
+}
+
+// The start of a step (grid build, lists, substep up to the divergence
+// solve's first host check) as a HIP graph, captured once per distinct set
+// of arguments (the two buffer parities of a run) on a private stream and
+// replayed on the caller's stream: one launch instead of ~40, whose small
+// kernels would otherwise wait for the host to issue them.  If capture is
+// unavailable the kernels are issued directly.  The cache holds the last
+// four argument sets; calls from several threads must not overlap.
+struct HeadKey {
+  std::vector<unsigned char> bytes;
+  void add(const void* q, size_t n) {
+    const unsigned char* c = (const unsigned char*)q;
+    bytes.insert(bytes.end(), c, c + n);
+  }
+};
+struct HeadGraph {
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec;
+};
+static std::vector<HeadGraph> g_head_graphs;
+static hipStream_t g_capture_stream = nullptr;
+static bool g_head_graph_off = false;
+
+template <class F>
+static int run_head(F&& issue, const HeadKey& key, hipStream_t s) {
+  if (g_head_graph_off) return issue(s);
+  for (const HeadGraph& g : g_head_graphs)
+    if (g.key == key.bytes) {
+      DIMCV_HIP(hipGraphLaunch(g.exec, s));
+      return DIMCV_OK;
+    }
+  if (!g_capture_stream) DIMCV_HIP(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(g_capture_stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  int rc = DIMCV_OK;
+  if (ok) {
+    rc = issue(g_capture_stream);
+    ok = hipStreamEndCapture(g_capture_stream, &graph) == hipSuccess && graph != nullptr && rc == DIMCV_OK;
+    ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  if (!ok) {  // no capture on this runtime: issue directly from now on
+    (void)hipGetLastError();
+    g_head_graph_off = true;
+    return issue(s);
+  }
+  if (g_head_graphs.size() >= 4) {
+    (void)hipGraphExecDestroy(g_head_graphs.front().exec);
+    g_head_graphs.erase(g_head_graphs.begin());
+  }
+  g_head_graphs.push_back({key.bytes, exec});
+  DIMCV_HIP(hipGraphLaunch(exec, s));
+  return DIMCV_OK;
 }
 
 extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
@@ -185,10 +296,26 @@ extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const d
   }
   // grid build into the sorted buffers of `pre` (= the fields of `st`), lists
   // and substep, all bounded by particle_bound; the counts stay on the device
-  CHECK(dimcv_grid_build(pre, p, particle_bound, nullptr, stream));
-  CHECK(neighbor_build_launch(st, p, particle_bound, s));
   o->fluid_count = 0;  // the residual divisor is read from the device counters
-  CHECK(substep_impl(st, p, o, particle_bound, particle_bound, s, host_counters));
+  Workspace ws;
+  CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
+  const int n_first = jacobi_first_chunk(0, o, o->out_kv);
+  auto head = [&](hipStream_t hs) -> int {
+    CHECK(dimcv_grid_build(pre, p, particle_bound, nullptr, hs));
+    CHECK(neighbor_build_launch(st, p, particle_bound, hs));
+    return substep_head(st, p, o, particle_bound, particle_bound, hs, ws, n_first);
+  };
+  HeadKey key;
+  key.add(pre, sizeof(*pre));
+  key.add(st, sizeof(*st));
+  key.add(p, sizeof(*p));
+  key.add(&particle_bound, sizeof(particle_bound));
+  key.add(&n_first, sizeof(n_first));
+  key.add(&o->inv_dt, sizeof(o->inv_dt));
+  key.add(&o->eta_v, sizeof(o->eta_v));
+  key.add(&o->max_iter_v, sizeof(o->max_iter_v));
+  CHECK(run_head(head, key, s));
+  CHECK(substep_tail(st, p, o, particle_bound, particle_bound, s, ws, host_counters));
   if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
     set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;

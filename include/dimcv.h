@@ -378,7 +378,9 @@ typedef struct dimcv_step_opts {
   int32_t m;                /* fixed-M sample count; 0 selects the Bernoulli sample set */
   int32_t max_iter_v;       /* m_max_iterations_v (dimcv_sph.py:22) */
   int32_t max_iter_p;       /* m_max_iterations (:23) */
-  int32_t chunk;            /* Jacobi iterations issued per host convergence check (>= 1) */
+  int32_t chunk;            /* Jacobi iterations issued per host convergence check (>= 1); 0: the
+                               first check after out_kv / out_kp iterations as passed in (the
+                               previous step's counts, at least 1), then after every iteration */
   double eta_v;             /* inv_dt * 0.1 * 0.01 * rho0 (:389) */
   double eta_p;             /* 0.05 * 0.01 * rho0 (:611) */
   float inv_dt;             /* f32(1 / dt) (:27, :381) */
