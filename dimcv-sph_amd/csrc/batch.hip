@@ -30,6 +30,12 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
                  double* partials = nullptr, const Packed* pk = nullptr);
 int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* p, float offset, int32_t bound,
                           double* partials, const int32_t* done, hipStream_t s);
+int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
+                   hipStream_t s);
+int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
+                       float* kap, hipStream_t s);
+int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
+                       hipStream_t s, const Packed* pk);
 int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,
                       int32_t* ucount, hipStream_t s);
 int batch_nbr_launch(const dimcv_state* st, const dimcv_params* p, hipStream_t s);
@@ -274,6 +280,7 @@ struct BatchCtx {
   const dimcv_step_opts* o;
   Workspace ws;
   BatchWs bw;
+  Packed pk;  // the step's packed records (workspace regions)
 };
 
 #define CHECK(call)      \
@@ -298,8 +305,9 @@ static int seg_grid(BatchCtx& c, hipStream_t s) {
 static int seg_div_begin(BatchCtx& c, hipStream_t s) {
   const dimcv_state* st = c.st;
   const int32_t nb = (int32_t)c.p->n_max;
-  CHECK(launch_fused(3, st, c.p, nb, s));
-  CHECK(dimcv_multiply_time_step(st, c.p, st->dfsph_factor, c.o->inv_dt, nb, s));
+  CHECK(launch_pack_xm(st, c.p, nb, c.pk, s));
+  CHECK(launch_fused(3, st, c.p, nb, s, nullptr, &c.pk));
+  CHECK(launch_scale_kappa(0, st, c.p, c.o->inv_dt, nb, c.ws.kap, s));
   hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
@@ -312,8 +320,8 @@ static int seg_iters(BatchCtx& c, int which, hipStream_t s) {
   const int32_t* done = st->scene_counters + DIMCV_SC_DONE;
   const int chunk = c.o->chunk > 0 ? c.o->chunk : 1;
   for (int k = 0; k < chunk; ++k) {
-    CHECK(launch_jacobi_pass(which == 0 ? 0 : 2, st, c.p, nb, done, s));
-    CHECK(launch_jacobi_pass(which == 0 ? 1 : 3, st, c.p, nb, done, s));
+    CHECK(launch_jacobi_pass(which == 0 ? 0 : 2, st, c.p, nb, done, s, nullptr, 0.0f, &c.pk));
+    CHECK(launch_jacobi_pass(which == 0 ? 1 : 3, st, c.p, nb, done, s, nullptr, 0.0f, &c.pk));
     CHECK(launch_block_partials(0, st, c.p, which == 0 ? 0.0f : c.o->density0, nb, c.ws.partials, done, s));
     hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
                        (int)DIMCV_SD_RESIDUAL, done);
@@ -329,10 +337,10 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   const dimcv_params* p = c.p;
   const int32_t nb = (int32_t)p->n_max;
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, c.o->dt, nb, s));
-  CHECK(dimcv_compute_non_pressure_forces(st, p, nb, nb, s));
-  CHECK(launch_fused(1, st, p, nb, s));
+  CHECK(launch_nonpressure(st, p, nb, nb, s, &c.pk));
+  CHECK(launch_fused(1, st, p, nb, s, nullptr, &c.pk));
   CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, nb, s));
-  CHECK(launch_fused(4, st, p, nb, s, c.ws.partials));
+  CHECK(launch_fused(4, st, p, nb, s, c.ws.partials, &c.pk));
   hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
                      (int)DIMCV_SD_KDEN_SUM, (const int32_t*)nullptr);
   DIMCV_LAUNCHED("batched kden");
@@ -340,8 +348,8 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   CHECK(batch_vort2vel_launch(st, p, c.ws, c.bw, s));
   CHECK(dimcv_update_particle_life_time(st, p, nb, s));
   CHECK(dimcv_predict_velocity(st, p, nb, s));
-  CHECK(dimcv_compute_density_adv(st, p, nb, s));
-  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, c.o->inv_dt2, nb, s));
+  CHECK(launch_jacobi_pass(3, st, p, nb, nullptr, s, nullptr, 0.0f, &c.pk));
+  CHECK(launch_scale_kappa(1, st, p, c.o->inv_dt2, nb, c.ws.kap, s));
   hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
@@ -362,6 +370,7 @@ static int make_ctx(BatchCtx& c, const dimcv_state* pre, const dimcv_state* st, 
   c.o = o;
   CHECK(bind_workspace(st, p, &c.ws, o ? (int64_t)o->m : 0));
   CHECK(bind_batch(st, p, &c.bw));
+  c.pk = Packed{c.ws.xm, c.ws.xw, c.ws.kap, c.ws.tiny};
   if (o) {
     if (o->m <= 0 || !o->u_cdf) {
       set_error("batched substep: fixed-M uniforms required (m > 0, u_cdf)");
