@@ -322,6 +322,12 @@ __device__ __forceinline__ void nbr_loop(const NbrList& nl, int64_t ns, int nn, 
 // Zeroes one counter (a kernel rather than a memset, so that the launch may
 // sit in a captured graph).
 static __global__ void k_zero_i32(int32_t* a) { *a = 0; }
+// Clears n words; a kernel rather than a memset so that the clear can sit in
+// a captured graph.
+static __global__ void k_zero_i32n(int32_t* a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
 
 // The step's packed per-slot records (built by the substep; absent in the
 // per-pass entry points): (x, m_V), (x, w) with w = m or m_V rho0, kappa,

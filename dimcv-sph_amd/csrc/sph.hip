@@ -1014,7 +1014,11 @@ int dimcv_record_sample_idx(const dimcv_state* st, const dimcv_params* p, int32_
   Workspace ws;
   int rc = bind_workspace(st, p, &ws, 0);
   if (rc) return rc;
-  DIMCV_HIP(hipMemsetAsync(st->sample_idx, 0, sizeof(int32_t) * p->n_max, s));
+  if (p->n_max > 0) {
+    hipLaunchKernelGGL(k_zero_i32n, dim3(((unsigned)p->n_max + 255u) / 256u), dim3(256), 0, s, st->sample_idx,
+                       (int)p->n_max);
+    DIMCV_LAUNCHED("k_zero_i32n");
+  }
   return compact_samples(st, particle_bound, ws, s);
 }
 

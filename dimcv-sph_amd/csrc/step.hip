@@ -145,13 +145,13 @@ static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimc
   return jacobi_issue(0, st, p, o, fluid_bound, ws, s, n_first, true);
 }
 
-// The rest of the substep; with `all` the pressure solve's last host check
-// leaves the whole counter array there.
-static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t particle_bound,
-                        int32_t fluid_bound, hipStream_t s, const Workspace& ws, int32_t* all) {
+// The substep after the divergence solve, in segments that hold kernels
+// only (each can be replayed as a graph), around the far field (issued
+// directly, between its timing events) and the pressure solve's checks.
+static int seg_mid1(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t particle_bound,
+                    int32_t fluid_bound, hipStream_t s, const Workspace& ws) {
   void* stream = (void*)s;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
-  CHECK(jacobi_wait(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
   CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
   CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk));
   // dimcv() (dimcv_sph.py:812-819)
@@ -167,6 +167,11 @@ static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step
                         0, 0.0, 0, s));
   if (o->m > 0) CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
   else CHECK(dimcv_record_sample_idx(st, p, particle_bound, stream));
+  return dimcv_vort2vel_near(st, p, fluid_bound, stream);
+}
+
+static int seg_far(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t fluid_bound,
+                   hipStream_t s) {
   if (o->time_far) {
     if (!g_step_ev[0]) {
       DIMCV_HIP(hipEventCreate(&g_step_ev[0]));
@@ -174,24 +179,52 @@ static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step
     }
     set_far_timing(g_step_ev[0], g_step_ev[1]);
   }
-  int rc_vv = dimcv_vort2vel(st, p, fluid_bound, (int32_t)p->n_max, stream);
+  int rc_vv = dimcv_vort2vel_far(st, p, fluid_bound, (int32_t)p->n_max, (void*)s);
   set_far_timing(nullptr, nullptr);
-  CHECK(rc_vv);
+  return rc_vv;
+}
+
+static int seg_mid2(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t particle_bound,
+                    int32_t fluid_bound, hipStream_t s, const Workspace& ws, int n_first_p) {
+  void* stream = (void*)s;
+  const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   CHECK(dimcv_update_particle_life_time(st, p, fluid_bound, stream));
   CHECK(dimcv_predict_velocity(st, p, particle_bound, stream));
   // pressure_solve (dimcv_sph.py:597-614)
   CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, &pk));
   CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s));
-  CHECK(jacobi(1, st, p, o, fluid_bound, ws, s, &o->out_kp, all));
+  return jacobi_issue(1, st, p, o, fluid_bound, ws, s, n_first_p, true);
+}
+
+static int seg_tail(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t particle_bound,
+                    hipStream_t s) {
+  void* stream = (void*)s;
   CHECK(dimcv_copy_x_temp(st, p, particle_bound, stream));
   CHECK(dimcv_advect(st, p, particle_bound, stream));
   if (o->bounce_back) CHECK(dimcv_enforce_boundary_3d(st, p, particle_bound, stream));
+  return DIMCV_OK;
+}
+
+// The far field's duration (the pressure solve's check has synchronised past it).
+static int far_time(const dimcv_params* p, dimcv_step_opts* o) {
   if (o->time_far && p->far_mode != DIMCV_FAR_EXACT) {
-    // the pressure solve synchronised the stream after the far field
     DIMCV_HIP(hipEventSynchronize(g_step_ev[1]));
     DIMCV_HIP(hipEventElapsedTime(&o->out_far_ms, g_step_ev[0], g_step_ev[1]));
   }
   return DIMCV_OK;
+}
+
+// The rest of the substep, issued directly; with `all` the pressure solve's
+// last host check leaves the whole counter array there.
+static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t particle_bound,
+                        int32_t fluid_bound, hipStream_t s, const Workspace& ws, int32_t* all) {
+  CHECK(jacobi_wait(0, st, p, o, fluid_bound, ws, s, &o->out_kv));
+  CHECK(seg_mid1(st, p, o, particle_bound, fluid_bound, s, ws));
+  CHECK(seg_far(st, p, o, fluid_bound, s));
+  CHECK(seg_mid2(st, p, o, particle_bound, fluid_bound, s, ws, jacobi_first_chunk(1, o, o->out_kp)));
+  CHECK(jacobi_wait(1, st, p, o, fluid_bound, ws, s, &o->out_kp, all));
+  CHECK(seg_tail(st, p, o, particle_bound, s));
+  return far_time(p, o);
 }
 
 extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
@@ -233,18 +266,21 @@ struct HeadKey {
     bytes.insert(bytes.end(), c, c + n);
   }
 };
-struct HeadGraph {
+struct SegGraph {
   std::vector<unsigned char> key;
   hipGraphExec_t exec;
 };
-static std::vector<HeadGraph> g_head_graphs;
+// one cache per captured segment of dimcv_step
+enum { SEG_HEAD, SEG_MID1, SEG_MID2, SEG_TAIL, SEG_COUNT };
+static std::vector<SegGraph> g_seg_graphs[SEG_COUNT];
 static hipStream_t g_capture_stream = nullptr;
-static bool g_head_graph_off = false;
+static bool g_graphs_off = false;
 
 template <class F>
-static int run_head(F&& issue, const HeadKey& key, hipStream_t s) {
-  if (g_head_graph_off) return issue(s);
-  for (const HeadGraph& g : g_head_graphs)
+static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s) {
+  if (g_graphs_off) return issue(s);
+  std::vector<SegGraph>& cache = g_seg_graphs[seg];
+  for (const SegGraph& g : cache)
     if (g.key == key.bytes) {
       DIMCV_HIP(hipGraphLaunch(g.exec, s));
       return DIMCV_OK;
@@ -263,14 +299,14 @@ static int run_head(F&& issue, const HeadKey& key, hipStream_t s) {
   if (rc) return rc;
   if (!ok) {  // no capture on this runtime: issue directly from now on
     (void)hipGetLastError();
-    g_head_graph_off = true;
+    g_graphs_off = true;
     return issue(s);
   }
-  if (g_head_graphs.size() >= 4) {
-    (void)hipGraphExecDestroy(g_head_graphs.front().exec);
-    g_head_graphs.erase(g_head_graphs.begin());
+  if (cache.size() >= 4) {
+    (void)hipGraphExecDestroy(cache.front().exec);
+    cache.erase(cache.begin());
   }
-  g_head_graphs.push_back({key.bytes, exec});
+  cache.push_back({key.bytes, exec});
   DIMCV_HIP(hipGraphLaunch(exec, s));
   return DIMCV_OK;
 }
@@ -299,23 +335,34 @@ extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const d
   o->fluid_count = 0;  // the residual divisor is read from the device counters
   Workspace ws;
   CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
-  const int n_first = jacobi_first_chunk(0, o, o->out_kv);
-  auto head = [&](hipStream_t hs) -> int {
-    CHECK(dimcv_grid_build(pre, p, particle_bound, nullptr, hs));
-    CHECK(neighbor_build_launch(st, p, particle_bound, hs));
-    return substep_head(st, p, o, particle_bound, particle_bound, hs, ws, n_first);
-  };
+  const int n_first_v = jacobi_first_chunk(0, o, o->out_kv);
+  const int n_first_p = jacobi_first_chunk(1, o, o->out_kp);
+  // every argument the captured segments depend on
   HeadKey key;
+  dimcv_step_opts ko = *o;
+  ko.out_kv = ko.out_kp = ko.time_far = 0;
+  ko.out_far_ms = 0.0f;
   key.add(pre, sizeof(*pre));
   key.add(st, sizeof(*st));
   key.add(p, sizeof(*p));
+  key.add(&ko, sizeof(ko));
   key.add(&particle_bound, sizeof(particle_bound));
-  key.add(&n_first, sizeof(n_first));
-  key.add(&o->inv_dt, sizeof(o->inv_dt));
-  key.add(&o->eta_v, sizeof(o->eta_v));
-  key.add(&o->max_iter_v, sizeof(o->max_iter_v));
-  CHECK(run_head(head, key, s));
-  CHECK(substep_tail(st, p, o, particle_bound, particle_bound, s, ws, host_counters));
+  key.add(&n_first_v, sizeof(n_first_v));
+  key.add(&n_first_p, sizeof(n_first_p));
+  const int32_t pb = particle_bound;
+  CHECK(run_graph(SEG_HEAD, [&](hipStream_t hs) -> int {
+    CHECK(dimcv_grid_build(pre, p, pb, nullptr, hs));
+    CHECK(neighbor_build_launch(st, p, pb, hs));
+    return substep_head(st, p, o, pb, pb, hs, ws, n_first_v);
+  }, key, s));
+  CHECK(jacobi_wait(0, st, p, o, pb, ws, s, &o->out_kv));
+  CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
+  CHECK(seg_far(st, p, o, pb, s));
+  CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
+                  s));
+  CHECK(jacobi_wait(1, st, p, o, pb, ws, s, &o->out_kp, host_counters));
+  CHECK(run_graph(SEG_TAIL, [&](hipStream_t hs) { return seg_tail(st, p, o, pb, hs); }, key, s));
+  CHECK(far_time(p, o));
   if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
     set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;
