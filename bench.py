@@ -43,10 +43,11 @@ for _p in (PKG, REPO):
 
 METRIC = "particle-steps/s at 1M, MC-VVPE vs direct Biot-Savart at fixed L2 vel err"
 FP32_PEAK_TFLOPS = 157.3
-# Issue ceiling of the far-field kernel as built: 16 vector instructions per
-# (target, sample) pair (gfx950 ISA of the pair loop), at most two vector
+# Issue ceiling of the far-field kernel as built: 14.75 vector instructions per
+# (target, sample) pair (59 per four-sample trip in the gfx950 ISA of the
+# loop), at most two vector
 # instructions per SIMD quad-cycle (dual issue), 1024 SIMDs at 2.4 GHz.
-VALU_PER_PAIR = 16
+VALU_PER_PAIR = 14.75
 FP32_ISSUE_TFLOPS = round(1024 * 2.4e9 / 4 * 2 * 64 / VALU_PER_PAIR * 24 / 1e12, 1)
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_MC = 24
@@ -412,7 +413,7 @@ def main():
                      "issue_ceiling": FP32_ISSUE_TFLOPS,
                      "frac_of_issue_ceiling": round(res["achieved_tflops"] / FP32_ISSUE_TFLOPS, 4),
                      "note": "FP32 vector-ALU bound. peak = MI355X FP32 157.3 TFLOP/s. Built without packed "
-                             "FP32 (-packed-fp32-ops); the pair loop issues 16 vector instructions per pair and "
+                             "FP32 (-packed-fp32-ops); the loop issues 59 vector instructions per four samples (14.75 per pair) and "
                              "the SIMDs dual-issue part of them (SQ_ACTIVE_INST_VALU2), so the issue ceiling is 2 "
                              "instructions per SIMD quad-cycle: issue_ceiling TFLOP/s at 24 flop per pair. A "
                              "matrix-core form (S = W Q on v_mfma_f32_16x16x4_f32) measured slower: no VALU/MFMA "

@@ -208,8 +208,49 @@ __device__ __forceinline__ void far_pair(float4 a0, float4 c0, float4 a1, float4
   rej += nrej;
 }
 
-// Sum over sources [j0, j1) in pairs (each pair decides its own path), four
-// samples per trip: the next four samples' data are loaded (scalar loads)
+// Four samples: the common path's reciprocals (shared by samples (0, 1) and
+// by (2, 3)), then one wave-wide cut-off test for the four (a three-way and a
+// two-way minimum) that replaces them with far_pair's exact path when any lane
+// is near the cut-off; accumulated in sample order.  Rejections are counted
+// only on the exact path.
+template <bool KAHAN, bool PERIODIC>
+__device__ __forceinline__ void far_quad(float4 a0, float4 c0, float4 a1, float4 c1, float4 a2, float4 c2,
+                                         float4 a3, float4 c3, v3 xi, float T, float T_hi, v3 L, float* A,
+                                         float* C, int& rej) {
+  float dx0, dy0, dz0, dx1, dy1, dz1, dx2, dy2, dz2, dx3, dy3, dz3;
+  far_delta<PERIODIC>(xi, a0, L, dx0, dy0, dz0);
+  far_delta<PERIODIC>(xi, a1, L, dx1, dy1, dz1);
+  far_delta<PERIODIC>(xi, a2, L, dx2, dy2, dz2);
+  far_delta<PERIODIC>(xi, a3, L, dx3, dy3, dz3);
+  const float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+  const float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+  const float r22 = fmaf(dz2, dz2, fmaf(dy2, dy2, dx2 * dx2));
+  const float r23 = fmaf(dz3, dz3, fmaf(dy3, dy3, dx3 * dx3));
+  // common path first (overwritten below when any lane is near the cut-off)
+  const float q01 = __builtin_amdgcn_rcpf(r20 * r21);
+  const float q23 = __builtin_amdgcn_rcpf(r22 * r23);
+  float inv0 = r21 * q01, inv1 = r20 * q01, inv2 = r23 * q23, inv3 = r22 * q23;
+  if (__builtin_expect(__any(fminf(fminf(r20, r21), fminf(r22, r23)) <= T_hi), 0)) {
+    // reference order r^2 and the exact acceptance test, per sample
+    const float e0 = __fadd_rn(__fadd_rn(__fmul_rn(dx0, dx0), __fmul_rn(dy0, dy0)), __fmul_rn(dz0, dz0));
+    const float e1 = __fadd_rn(__fadd_rn(__fmul_rn(dx1, dx1), __fmul_rn(dy1, dy1)), __fmul_rn(dz1, dz1));
+    const float e2 = __fadd_rn(__fadd_rn(__fmul_rn(dx2, dx2), __fmul_rn(dy2, dy2)), __fmul_rn(dz2, dz2));
+    const float e3 = __fadd_rn(__fadd_rn(__fmul_rn(dx3, dx3), __fmul_rn(dy3, dy3)), __fmul_rn(dz3, dz3));
+    const bool ok0 = e0 > T, ok1 = e1 > T, ok2 = e2 > T, ok3 = e3 > T;
+    inv0 = ok0 ? __builtin_amdgcn_rcpf(e0) : 0.0f;
+    inv1 = ok1 ? __builtin_amdgcn_rcpf(e1) : 0.0f;
+    inv2 = ok2 ? __builtin_amdgcn_rcpf(e2) : 0.0f;
+    inv3 = ok3 ? __builtin_amdgcn_rcpf(e3) : 0.0f;
+    rej += (ok0 ? 0 : 1) + (ok1 ? 0 : 1) + (ok2 ? 0 : 1) + (ok3 ? 0 : 1);
+  }
+  far_accumulate<KAHAN>(dx0, dy0, inv0, c0, A, C);
+  far_accumulate<KAHAN>(dx1, dy1, inv1, c1, A, C);
+  far_accumulate<KAHAN>(dx2, dy2, inv2, c2, A, C);
+  far_accumulate<KAHAN>(dx3, dy3, inv3, c3, A, C);
+}
+
+// Sum over sources [j0, j1) four samples per trip (far_quad; a remainder in
+// pairs, then one): the next four samples' data are loaded (scalar loads)
 // before the current four are evaluated.
 template <bool KAHAN, bool PERIODIC>
 __device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, int j0, int j1, v3 xi, float T, v3 L,
@@ -223,8 +264,7 @@ __device__ __forceinline__ void far_sum_range(const float4* __restrict__ src, in
       const int nx = js + 7 < j1 ? js + 4 : js;  // in-range prefetch index
       const float4 na0 = src[2 * nx], nc0 = src[2 * nx + 1], na1 = src[2 * nx + 2], nc1 = src[2 * nx + 3];
       const float4 na2 = src[2 * nx + 4], nc2 = src[2 * nx + 5], na3 = src[2 * nx + 6], nc3 = src[2 * nx + 7];
-      far_pair<KAHAN, PERIODIC>(a0, c0, a1, c1, xi, T, T_hi, L, A, C, rej);
-      far_pair<KAHAN, PERIODIC>(a2, c2, a3, c3, xi, T, T_hi, L, A, C, rej);
+      far_quad<KAHAN, PERIODIC>(a0, c0, a1, c1, a2, c2, a3, c3, xi, T, T_hi, L, A, C, rej);
       a0 = na0; c0 = nc0; a1 = na1; c1 = nc1;
       a2 = na2; c2 = nc2; a3 = na3; c3 = nc3;
     }

@@ -421,14 +421,11 @@ class DIMCVSPHSolver(SPHBase):
         ps = self.ps
         o = self._step_opts(None, None, u_cdf)
         pre = ps.state()
-        for name in N.SORTED_FIELDS:
-            getattr(ps, name).swap(getattr(ps, name + "_buffer"))
-        ps._parity ^= 1
-        ps._state = None
+        ps.swap_sorted_fields()
         post = ps.state()
         hc = ps._host_counters
         rc = N.lib().dimcv_step(ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps.params), ctypes.byref(o),
-                                int(ps._pn), hc.ctypes.data, ps.stream())
+                                ps._pn, ps._host_counters_ptr, ps.stream())
         ps._pn = int(hc[N.C_PARTICLE_NUM])
         ps._nfl = int(hc[N.C_FLUID_LIST])
         N.check(rc, "dimcv_step")

@@ -145,6 +145,7 @@ class ParticleSystem:
         ws = N.lib().dimcv_workspace_bytes(n, nc, n)
         self._workspace = torch.zeros(int(ws), dtype=torch.uint8, device=dev)
         self._host_counters = np.zeros(N.C_COUNT, dtype=np.int32)
+        self._host_counters_ptr = self._host_counters.ctypes.data
         self._pn = 0          # exact particle_num, mirrored on the host
         self._n_inserted = 0  # particles inserted so far (next particle_id)
         self._nfl = 0         # exact fluid-list length after the last grid build
@@ -155,6 +156,7 @@ class ParticleSystem:
         # so the state alternates between two pointer sets: one cached per parity
         self._state_cache = {}
         self._parity = 0
+        self._swap_pairs = None  # (field, buffer) of every sorted field
 
         self.params = N.Params()
         self._init_params()
@@ -249,10 +251,22 @@ class ParticleSystem:
         """Forget the cached states (after any change of a field's storage)."""
         self._state = None
         self._state_cache.clear()
+        self._swap_pairs = None
 
-    @staticmethod
-    def stream() -> int:
-        return torch.cuda.current_stream().cuda_stream
+    def swap_sorted_fields(self):
+        """Exchange every sorted field with its buffer (the grid build's
+        gather writes the buffers) and flip the cached state's parity."""
+        pairs = self._swap_pairs
+        if pairs is None:
+            pairs = self._swap_pairs = [(getattr(self, f), getattr(self, f + "_buffer")) for f in N.SORTED_FIELDS]
+        for a, b in pairs:
+            a.t, b.t = b.t, a.t
+        self._parity ^= 1
+        self._state = None
+
+    def stream(self) -> int:
+        """The current HIP stream of this system's device (raw handle)."""
+        return torch._C._cuda_getCurrentRawStream(self.device.index or 0)
 
     def call(self, name: str, *args):
         st = self.state()
@@ -328,11 +342,8 @@ class ParticleSystem:
     def initialize_particle_system(self):
         """particle_system.py:555-560 + the step's neighbour lists."""
         hc = self._host_counters
-        self.call("dimcv_grid_build", int(self._pn), hc.ctypes.data, self.stream())
-        for name in N.SORTED_FIELDS:
-            getattr(self, name).swap(getattr(self, name + "_buffer"))
-        self._parity ^= 1
-        self._state = None
+        self.call("dimcv_grid_build", int(self._pn), self._host_counters_ptr, self.stream())
+        self.swap_sorted_fields()
         self._pn = int(hc[N.C_PARTICLE_NUM])
         self._nfl = int(hc[N.C_FLUID_LIST])
         self.build_neighbor_lists()
