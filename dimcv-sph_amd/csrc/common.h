@@ -87,7 +87,23 @@ __device__ __forceinline__ v3 smul(float s, v3 a) { return mk(s * a.x, s * a.y, 
 __device__ __forceinline__ v3 vmuls(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ v3 vdivs(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
 __device__ __forceinline__ float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float vnorm(v3 a) { return sqrtf(vdot(a, a)); }
+// Correctly rounded square root of x >= 0 (a sum of squares): the rounding
+// steps of the IEEE square root hipcc emits for gfx950 (inputs below 2^-96
+// scaled by 2^32, the hardware estimate, its neighbours one ulp down and up
+// kept or dropped by the signs of their residuals fma(-s', s, x), the result
+// scaled back by 2^-16), without that sequence's final class test for zero
+// and infinity, whose results the steps already give for x = +0 and +inf.
+__device__ __forceinline__ float sqrt_nonneg(float x) {
+  const bool small = x < 0x1p-96f;
+  const float xs = small ? x * 0x1p32f : x;
+  const float s = __builtin_amdgcn_sqrtf(xs);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = fmaf(-dn, s, xs) <= 0.0f ? dn : s;
+  r = fmaf(-up, s, xs) > 0.0f ? up : r;
+  return small ? r * 0x1p-16f : r;
+}
+__device__ __forceinline__ float vnorm(v3 a) { return sqrt_nonneg(vdot(a, a)); }
 __device__ __forceinline__ v3 vcross(v3 a, v3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
@@ -98,8 +114,16 @@ __device__ __forceinline__ v3 ld3(const float* __restrict__ a, int64_t i) {
 // (wave-uniform) field base, so a gather is one scalar-base load with a
 // vector offset instead of 64-bit address arithmetic per lane.  Slots stay
 // below DIMCV_MAX_SLOTS, so 12 * slot fits in 32 bits.
+// Byte offset j * 12 as (j + 2j) * 4: two full-rate shift-adds in place of
+// the quarter-rate 32-bit integer multiply the compiler picks for j * 12 (the
+// empty asm keeps the two from being folded back into that multiply).
+__device__ __forceinline__ uint32_t off12(uint32_t j) {
+  uint32_t t = j + (j << 1);
+  asm("" : "+v"(t));
+  return t << 2;
+}
 __device__ __forceinline__ v3 ld3s(const float* __restrict__ a, uint32_t j) {
-  const float* q = (const float*)((const char*)a + j * 12u);
+  const float* q = (const float*)((const char*)a + off12(j));
   return mk(q[0], q[1], q[2]);
 }
 __device__ __forceinline__ float ld1s(const float* __restrict__ a, uint32_t j) {
@@ -113,19 +137,14 @@ __device__ __forceinline__ void st3(float* __restrict__ a, int64_t i, v3 q) {
 
 // Cubic spline kernel, sph_base.py:23-44 ((1-q)^3 evaluated as (t*t)*t).
 __device__ __forceinline__ float cubic_W(const dimcv_params& p, float r_norm) {
-  float res = 0.0f;
-  float q = r_norm * p.inv_support_radius;
-  if (q <= 1.0f) {
-    if (q <= 0.5f) {
-      float q2 = q * q;
-      float q3 = q2 * q;
-      res = p.kernel_k * (6.0f * q3 - 6.0f * q2 + 1.0f);
-    } else {
-      float t = 1.0f - q;
-      res = p.kernel_k * 2.0f * ((t * t) * t);
-    }
-  }
-  return res;
+  const float q = r_norm * p.inv_support_radius;
+  // both branches' values, then selects (see cubic_dW)
+  const float q2 = q * q;
+  const float q3 = q2 * q;
+  const float wa = p.kernel_k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+  const float t = 1.0f - q;
+  const float wb = p.kernel_k * 2.0f * ((t * t) * t);
+  return q <= 1.0f ? (q <= 0.5f ? wa : wb) : 0.0f;
 }
 
 // a / b for the three components of a, rounded exactly as the IEEE division
@@ -162,12 +181,12 @@ __device__ __forceinline__ v3 cubic_dW(const dimcv_params& p, v3 r, bool shared_
   v3 res = mk(0.0f, 0.0f, 0.0f);
   if (rn > 1e-5f && q <= 1.0f) {
     v3 gq = vmuls(shared_div ? vdivs_shared(r, rn) : vdivs(r, rn), p.inv_support_radius);
-    if (q <= 0.5f) {
-      res = smul(p.kernel_dk * q * (3.0f * q - 2.0f), gq);
-    } else {
-      float f = 1.0f - q;
-      res = smul(p.kernel_dk * (-f * f), gq);
-    }
+    // both branches' factors, then a select: the lanes of a wave sit on both
+    // sides of q = 1/2, where a branch would run both sides anyway
+    const float fa = p.kernel_dk * q * (3.0f * q - 2.0f);
+    const float f = 1.0f - q;
+    const float fb = p.kernel_dk * (-f * f);
+    res = smul(q <= 0.5f ? fa : fb, gq);
   }
   return res;
 }
