@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
              v3 r2 = rel(p, xi, g.x);
              v3 r = mk(r2.x, r2.y, 0.0f);
              float rn = vnorm(r);
-             float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
+             const float ex = exp_f32(-rn * p.inv_eps);  // both exponentials of the expression
+             float A = (1.0f - ex) / (rn * (rn + p.eps * ex));
              vv = vadd(vv, smul(A, vcross(r, g.w)));
              kns += g.kd;
            });
@@ -89,7 +90,8 @@ __global__ void __launch_bounds__(256) k_vv_far_exact(dimcv_state st, dimcv_para
     float rn = vnorm(r2);
     if (rn > p.support_radius) {
       v3 r = mk(r2.x, r2.y, 0.0f);
-      float A = (1.0f - exp_f32(-rn * p.inv_eps)) / (rn * (rn + p.eps * exp_f32(-rn * p.inv_eps)));
+      const float ex = exp_f32(-rn * p.inv_eps);  // both exponentials of the expression
+      float A = (1.0f - ex) / (rn * (rn + p.eps * ex));
       v3 t = smul(A, vcross(r, ld3(st.d_vort_smoothed, pj)));
       t = vmuls(t, S);
       t = vdivs(t, st.kernel_density[pj] + 1e-8f);
