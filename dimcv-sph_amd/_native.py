@@ -177,6 +177,8 @@ def lib():
         "dimcv_ring_velocity": (I32, [I64, P, F64, I64, P, P, P]),
         "dimcv_substep": (I32, [P, P, P, I32, I32, P]),
         "dimcv_step": (I32, [P, P, P, P, I32, P, P]),
+        "dimcv_step_begin": (I32, [P, P, P, P, I32, P]),
+        "dimcv_step_finish": (I32, [P, P, P, P, I32, P, P]),
         "dimcv_batch_workspace_bytes": (I64, [I32, P, I64]),
         "dimcv_batch_grid_build": (I32, [P, P, P]),
         "dimcv_batch_neighbor_build": (I32, [P, P, P]),

@@ -311,10 +311,17 @@ static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s) {
   return DIMCV_OK;
 }
 
-extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
-                          int32_t particle_bound, int32_t* host_counters, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (!pre || !st || !p || !o || !host_counters) {
+// Arguments, workspace and graph key of one dimcv_step (the same for its
+// begin and finish halves).
+struct StepPrep {
+  Workspace ws;
+  HeadKey key;
+  int n_first_v, n_first_p;
+};
+
+static int step_prepare(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                        int32_t particle_bound, StepPrep* sp) {
+  if (!pre || !st || !p || !o) {
     set_error("dimcv_step: null argument");
     return DIMCV_ERR_ARG;
   }
@@ -330,31 +337,58 @@ extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const d
     set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
     return DIMCV_ERR_ARG;
   }
-  // grid build into the sorted buffers of `pre` (= the fields of `st`), lists
-  // and substep, all bounded by particle_bound; the counts stay on the device
   o->fluid_count = 0;  // the residual divisor is read from the device counters
-  Workspace ws;
-  CHECK(bind_workspace(st, p, &ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
-  const int n_first_v = jacobi_first_chunk(0, o, o->out_kv);
-  const int n_first_p = jacobi_first_chunk(1, o, o->out_kp);
+  CHECK(bind_workspace(st, p, &sp->ws, p->far_mode == DIMCV_FAR_EXACT ? 0 : p->n_max));
+  sp->n_first_v = jacobi_first_chunk(0, o, o->out_kv);
+  sp->n_first_p = jacobi_first_chunk(1, o, o->out_kp);
   // every argument the captured segments depend on
-  HeadKey key;
   dimcv_step_opts ko = *o;
   ko.out_kv = ko.out_kp = ko.time_far = 0;
   ko.out_far_ms = 0.0f;
-  key.add(pre, sizeof(*pre));
-  key.add(st, sizeof(*st));
-  key.add(p, sizeof(*p));
-  key.add(&ko, sizeof(ko));
-  key.add(&particle_bound, sizeof(particle_bound));
-  key.add(&n_first_v, sizeof(n_first_v));
-  key.add(&n_first_p, sizeof(n_first_p));
+  sp->key.add(pre, sizeof(*pre));
+  sp->key.add(st, sizeof(*st));
+  sp->key.add(p, sizeof(*p));
+  sp->key.add(&ko, sizeof(ko));
+  sp->key.add(&particle_bound, sizeof(particle_bound));
+  sp->key.add(&sp->n_first_v, sizeof(sp->n_first_v));
+  sp->key.add(&sp->n_first_p, sizeof(sp->n_first_p));
+  return DIMCV_OK;
+}
+
+// The grid build into the sorted buffers of `pre` (= the fields of `st`),
+// the lists and the substep up to the divergence solve's first check, all
+// bounded by particle_bound; enqueued only (one graph launch).
+extern "C" int dimcv_step_begin(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                                dimcv_step_opts* o, int32_t particle_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  StepPrep sp;
+  CHECK(step_prepare(pre, st, p, o, particle_bound, &sp));
+  const Workspace& ws = sp.ws;
   const int32_t pb = particle_bound;
-  CHECK(run_graph(SEG_HEAD, [&](hipStream_t hs) -> int {
+  const int n_first_v = sp.n_first_v;
+  return run_graph(SEG_HEAD, [&](hipStream_t hs) -> int {
     CHECK(dimcv_grid_build(pre, p, pb, nullptr, hs));
     CHECK(neighbor_build_launch(st, p, pb, hs));
     return substep_head(st, p, o, pb, pb, hs, ws, n_first_v);
-  }, key, s));
+  }, sp.key, s);
+}
+
+// The rest of the step after dimcv_step_begin with the same arguments; the
+// counts come back in host_counters with the pressure solve's last check.
+extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                                 dimcv_step_opts* o, int32_t particle_bound, int32_t* host_counters,
+                                 void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!host_counters) {
+    set_error("dimcv_step: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  StepPrep sp;
+  CHECK(step_prepare(pre, st, p, o, particle_bound, &sp));
+  const Workspace& ws = sp.ws;
+  const HeadKey& key = sp.key;
+  const int32_t pb = particle_bound;
+  const int n_first_p = sp.n_first_p;
   CHECK(jacobi_wait(0, st, p, o, pb, ws, s, &o->out_kv));
   CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
   CHECK(seg_far(st, p, o, pb, s));
@@ -376,4 +410,14 @@ extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const d
     return DIMCV_ERR_ARG;
   }
   return DIMCV_OK;
+}
+
+extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
+                          int32_t particle_bound, int32_t* host_counters, void* stream) {
+  if (!host_counters) {
+    set_error("dimcv_step: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  CHECK(dimcv_step_begin(pre, st, p, o, particle_bound, stream));
+  return dimcv_step_finish(pre, st, p, o, particle_bound, host_counters, stream);
 }

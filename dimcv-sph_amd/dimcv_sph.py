@@ -265,20 +265,29 @@ class DIMCVSPHSolver(SPHBase):
     def _upload(self, a, dtype):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.ps.device)
 
+    def _staged_buffers(self, key, n, dtype):
+        """Pinned host and device buffers kept for `key`, with room for n
+        values: (host tensor, device tensor, numpy view of the host one)."""
+        bufs = self._staged.get(key)
+        if bufs is None or bufs[0].numel() < n or bufs[0].dtype != dtype:
+            host = torch.empty(max(n, 1), dtype=dtype, pin_memory=True)
+            dev = torch.empty(max(n, 1), dtype=dtype, device=self.ps.device)
+            bufs = self._staged[key] = (host, dev, host.numpy())
+        return bufs
+
     def _upload_staged(self, key, a, dtype):
         """Copy `a` into a device buffer kept for `key`, through a pinned host
         buffer (an asynchronous copy on the current stream).  The host buffer
         is reused by the next call on that key, which comes after the step's
         own synchronisations have retired this copy."""
         a = np.ascontiguousarray(a)
-        bufs = self._staged.get(key)
-        if bufs is None or bufs[0].numel() < a.size or bufs[0].dtype != dtype:
-            host = torch.empty(max(a.size, 1), dtype=dtype, pin_memory=True)
-            dev = torch.empty(max(a.size, 1), dtype=dtype, device=self.ps.device)
-            bufs = self._staged[key] = (host, dev)
-        host, dev = bufs
-        host[: a.size].numpy()[:] = a
-        dev[: a.size].copy_(host[: a.size], non_blocking=True)
+        n = a.size
+        host, dev, host_np = self._staged_buffers(key, n, dtype)
+        host_np[:n] = a
+        if n == dev.numel():
+            dev.copy_(host, non_blocking=True)
+        else:
+            dev[:n].copy_(host[:n], non_blocking=True)
         return dev
 
     def compute_kinematic_vorticity_num_sample(self, u1=None, u2=None):
@@ -354,15 +363,21 @@ class DIMCVSPHSolver(SPHBase):
         self.update_particle_life_time()
 
     # ------------------------------------------------- native substep --
-    def _step_opts(self, u1, u2, u_cdf):
-        """dimcv_step_opts of this step (uniforms uploaded; fluid_count unset)."""
+    def _step_opts(self, u1, u2, u_cdf, defer_cdf=False):
+        """dimcv_step_opts of this step (uniforms uploaded; fluid_count unset).
+        With defer_cdf the inverse-CDF uniforms are neither drawn nor
+        uploaded: u_cdf points at the device buffer that _upload_staged
+        ("u_cdf") fills later."""
         ps = self.ps
         o = N.StepOpts()
         if self.mc_samples:
             m = int(self.mc_samples)
-            if u_cdf is None:
-                u_cdf = self.rng.random(m)
-            t = self._upload_staged("u_cdf", np.asarray(u_cdf)[:m], torch.float64)
+            if defer_cdf:
+                t = self._staged_buffers("u_cdf", m, torch.float64)[1]
+            else:
+                if u_cdf is None:
+                    u_cdf = self.rng.random(m)
+                t = self._upload_staged("u_cdf", np.asarray(u_cdf)[:m], torch.float64)
             o.u_cdf, o.m = t.data_ptr(), m
         else:
             pn = ps._pn
@@ -412,23 +427,32 @@ class DIMCVSPHSolver(SPHBase):
         self._step_outputs(o)
 
     def native_step(self, u_cdf=None):
-        """initialize_particle_system() + substep() in one library call
-        (dimcv_step): the grid build, the neighbour lists and the substep run
-        back to back on the device, bounded by the particle count before the
-        build; the counts come back with the pressure solve's last check.
-        Fixed-M sampling only (the Bernoulli uniforms are drawn per particle
-        after the build)."""
+        """initialize_particle_system() + substep() in two library calls
+        (dimcv_step_begin / dimcv_step_finish): the grid build, the neighbour
+        lists and the substep run back to back on the device, bounded by the
+        particle count before the build; the counts come back with the
+        pressure solve's last check.  The step's inverse-CDF uniforms are
+        drawn and uploaded between the two calls, while the device runs the
+        build and the divergence solve.  Fixed-M sampling only (the Bernoulli
+        uniforms are drawn per particle after the build)."""
         ps = self.ps
-        o = self._step_opts(None, None, u_cdf)
+        m = int(self.mc_samples)
+        o = self._step_opts(None, None, None, defer_cdf=True)
         pre = ps.state()
         ps.swap_sorted_fields()
         post = ps.state()
         hc = ps._host_counters
-        rc = N.lib().dimcv_step(ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps.params), ctypes.byref(o),
-                                ps._pn, ps._host_counters_ptr, ps.stream())
+        lib = N.lib()
+        stream = ps.stream()
+        args = (ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps.params), ctypes.byref(o), ps._pn)
+        N.check(lib.dimcv_step_begin(*args, stream), "dimcv_step_begin")
+        u = self.rng.random(m) if u_cdf is None else np.asarray(u_cdf)[:m]
+        t = self._upload_staged("u_cdf", u, torch.float64)
+        assert t.data_ptr() == o.u_cdf
+        rc = lib.dimcv_step_finish(*args, ps._host_counters_ptr, stream)
         ps._pn = int(hc[N.C_PARTICLE_NUM])
         ps._nfl = int(hc[N.C_FLUID_LIST])
-        N.check(rc, "dimcv_step")
+        N.check(rc, "dimcv_step_finish")
         self._step_outputs(o)
 
     # --------------------------------------------------- comparators --

@@ -421,6 +421,18 @@ int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts*
  * out-of-grid keys when the error is returned. */
 int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
                int32_t particle_bound, int32_t* host_counters, void* stream);
+/* dimcv_step in two halves, called in this order with the same arguments:
+ * _begin enqueues the grid build, the lists and the substep up to the
+ * divergence solve's first convergence check and returns without waiting;
+ * _finish does the rest (dimcv_step = _begin + _finish).  The uniforms
+ * (opts->u_cdf, or u1/u2) are first read by work _finish enqueues, so the
+ * caller may fill them between the two calls with work enqueued on the same
+ * stream (karman_vortex.py:219-224 draws them inside the step, after the
+ * build). */
+int dimcv_step_begin(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                     dimcv_step_opts* opts, int32_t particle_bound, void* stream);
+int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                      dimcv_step_opts* opts, int32_t particle_bound, int32_t* host_counters, void* stream);
 
 /* ---- comparators and initial conditions (build-defined) --------------------- */
 /* Direct O(N^2) Biot-Savart over every in-domain fluid particle: the sum the

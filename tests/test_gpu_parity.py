@@ -175,3 +175,37 @@ def test_native_substep_equals_python_sequence():
             assert np.array_equal(a, b), (k, name)
         assert sol_a.last_iterations_v == sol_b.last_iterations_v
         assert sol_a.last_iterations_p == sol_b.last_iterations_p
+
+
+def test_step_entry_equals_begin_finish():
+    """dimcv_step (one call, uniforms uploaded before it) and the
+    dimcv_step_begin / dimcv_step_finish pair the solver uses (uniforms
+    uploaded between them) give bit-identical states."""
+    import ctypes
+    import scenes
+    import _native as N
+    runs = [scenes.build_ring_system(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="fast")[:2]
+            for _ in range(2)]
+    (ps_a, sol_a), (ps_b, sol_b) = runs
+    rng = np.random.default_rng(9)
+    for k in range(4):
+        u = rng.random(128)
+        sol_a.step(u_cdf=u)
+        o = sol_b._step_opts(None, None, u)
+        pre = ps_b.state()
+        ps_b.swap_sorted_fields()
+        post = ps_b.state()
+        rc = N.lib().dimcv_step(ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps_b.params),
+                                ctypes.byref(o), ps_b._pn, ps_b._host_counters_ptr, ps_b.stream())
+        N.check(rc, "dimcv_step")
+        ps_b._pn = int(ps_b._host_counters[N.C_PARTICLE_NUM])
+        ps_b._nfl = int(ps_b._host_counters[N.C_FLUID_LIST])
+        sol_b._step_outputs(o)
+        pn = ps_a.particle_num[None]
+        assert pn == ps_b.particle_num[None] > 0
+        for name in ("x", "v", "density", "vorticity", "kernel_density", "sample_idx"):
+            a = getattr(ps_a, name) if hasattr(ps_a, name) else getattr(sol_a, name)
+            b = getattr(ps_b, name) if hasattr(ps_b, name) else getattr(sol_b, name)
+            assert np.array_equal(a.to_numpy()[:pn], b.to_numpy()[:pn]), (k, name)
+        assert sol_a.last_iterations_v == sol_b.last_iterations_v
+        assert sol_a.last_iterations_p == sol_b.last_iterations_p
