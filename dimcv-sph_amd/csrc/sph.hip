@@ -42,24 +42,32 @@ __device__ __forceinline__ float4 nb_xw(const dimcv_state& st, const dimcv_param
 
 // Zero-fill of slots that a "field.fill(0) + fluid writes" kernel leaves at
 // zero: every slot >= particle_num or not fluid.
+// Up to five per-slot arrays (null = unused) and their floats per slot.
+struct ZeroSet {
+  float* a[5];
+  int c[5];
+};
+
 __global__ void k_zero_nonfluid(const int32_t* __restrict__ material, const int32_t* __restrict__ counters,
-                                int64_t n_max, float* a, int ca, float* b, int cb, float* c, int cc) {
+                                int64_t n_max, ZeroSet z) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_max) return;
   if (s < counters[DIMCV_C_PARTICLE_NUM] && material[s] == 1) return;
-  for (int k = 0; k < ca; ++k) a[s * ca + k] = 0.0f;
-  if (b)
-    for (int k = 0; k < cb; ++k) b[s * cb + k] = 0.0f;
-  if (c)
-    for (int k = 0; k < cc; ++k) c[s * cc + k] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    if (z.a[q])
+      for (int k = 0; k < z.c[q]; ++k) z.a[q][s * z.c[q] + k] = 0.0f;
 }
 
-static int zero_nonfluid(const dimcv_state* st, const dimcv_params* p, hipStream_t s, float* a, int ca,
-                         float* b = nullptr, int cb = 0, float* c = nullptr, int cc = 0) {
+static int zero_nonfluid(const dimcv_state* st, const dimcv_params* p, hipStream_t s, const ZeroSet& z) {
   hipLaunchKernelGGL(k_zero_nonfluid, dim3(nblocks(p->n_max)), dim3(256), 0, s, st->material, st->counters,
-                     p->n_max, a, ca, b, cb, c, cc);
+                     p->n_max, z);
   DIMCV_LAUNCHED("k_zero_nonfluid");
   return DIMCV_OK;
+}
+static int zero_nonfluid(const dimcv_state* st, const dimcv_params* p, hipStream_t s, float* a, int ca,
+                         float* b = nullptr, int cb = 0, float* c = nullptr, int cc = 0) {
+  return zero_nonfluid(st, p, s, ZeroSet{{a, b, c, nullptr, nullptr}, {ca, cb, cc, 0, 0}});
 }
 
 // dimcv_sph.py:126-148
@@ -228,9 +236,15 @@ int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particl
 // multiply_time_step (dimcv_sph.py:395-399) of the DFSPH factor at the start
 // of a Jacobi solve, also writing kappa_i of its first iteration:
 // b alpha rho (which 0, divergence) or (da - rho0) alpha (which 1, pressure).
+// With reset, it also clears the Jacobi counters (done flag, iteration count)
+// for the solve that follows it.
 __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int which, int32_t bound,
-                              float* __restrict__ kap) {
+                              float* __restrict__ kap, int reset) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  if (reset && f == 0) {
+    st.counters[DIMCV_C_JACOBI_DONE] = 0;
+    st.counters[DIMCV_C_JACOBI_ITERS] = 0;
+  }
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -241,10 +255,11 @@ __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int w
 }
 
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
-                       float* kap, hipStream_t s) {
-  if (fluid_bound <= 0) return DIMCV_OK;
-  hipLaunchKernelGGL(k_scale_kappa, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, scale, which,
-                     fluid_bound, kap);
+                       float* kap, hipStream_t s, bool reset_jacobi) {
+  // with reset_jacobi, at least one block runs (its first thread clears the counters)
+  if (fluid_bound <= 0 && !reset_jacobi) return DIMCV_OK;
+  hipLaunchKernelGGL(k_scale_kappa, dim3(fluid_bound > 0 ? nblocks(fluid_bound) : 1), dim3(256), 0, s, *st, *p,
+                     scale, which, fluid_bound, kap, reset_jacobi ? 1 : 0);
   DIMCV_LAUNCHED("k_scale_kappa");
   return DIMCV_OK;
 }
@@ -453,6 +468,49 @@ __global__ void k_copy_x_temp(dimcv_state st, int64_t n_max) {
   st3(st.x_temp, s, q);
 }
 
+// k_copy_x_temp then k_advect of each slot in one launch over n_max slots
+// (advection for slots below `bound`, as k_advect's launch).
+__global__ void k_copy_advect(dimcv_state st, dimcv_params p, int64_t n_max, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_max) return;
+  const int32_t pn = st.counters[DIMCV_C_PARTICLE_NUM];
+  const v3 x = s < pn ? ld3(st.x, s) : mk(0.0f, 0.0f, 0.0f);
+  st3(st.x_temp, s, x);
+  if (s >= bound || s >= pn || !st.is_dynamic[s]) return;
+  v3 v = ld3(st.v, s);
+  if (st.material[s] == 0) {
+    v = vadd(v, smul(p.dt, ld3(st.acceleration, s)));
+    st3(st.v, s, v);
+  }
+  st3(st.x, s, vadd(x, smul(p.dt, v)));
+}
+
+// k_life_time and k_predict of each slot in one launch: the fluid list holds
+// exactly the fluid slots below particle_num, in slot order, so the life-time
+// update runs by slot on the same particles.
+__global__ void k_predict_life(dimcv_state st, dimcv_params p, int32_t bound) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[s] != 1) return;
+  if (in_vortex_domain(p, ld3(st.x, s))) st.life_time[s] += p.dt;
+  if (st.is_dynamic[s]) st3(st.v, s, vadd(ld3(st.v, s), smul(p.dt, ld3(st.acceleration, s))));
+}
+
+int launch_copy_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s) {
+  if (p->n_max > 0)
+    hipLaunchKernelGGL(k_copy_advect, dim3(nblocks(p->n_max)), dim3(256), 0, s, *st, *p, p->n_max,
+                       particle_bound);
+  DIMCV_LAUNCHED("k_copy_advect");
+  return DIMCV_OK;
+}
+
+int launch_predict_life(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s) {
+  if (particle_bound > 0)
+    hipLaunchKernelGGL(k_predict_life, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound);
+  DIMCV_LAUNCHED("k_predict_life");
+  return DIMCV_OK;
+}
+
 // dimcv_sph.py:778-795
 __global__ void __launch_bounds__(256) k_vorticity_vis(dimcv_state st, dimcv_params p, int32_t bound) {
   FLUID_ENTRY();
@@ -510,6 +568,29 @@ __global__ void k_v_star(dimcv_state st, dimcv_params p, int32_t bound) {
 // 2 = grad_v, 3 = both (+ v*) in one neighbour pass.
 // MODE 7 also forms d_vort = vorticity - curl v* (:551-583) in the same loop:
 // v*_j = v_j + a_j dt is formed on the fly exactly as compute_v_star does.
+// Kinematic vorticity number |W| / (|S| + 1e-8) of a velocity gradient g
+// (row-major 3x3), W and S its antisymmetric and symmetric parts times two
+// (dimcv_sph.py:295-310).
+__device__ __forceinline__ float kvn_of(const float* g) {
+  float Wn = 0.0f, Sn = 0.0f;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      float w = g[3 * r + c] - g[3 * c + r];
+      float s = g[3 * r + c] + g[3 * c + r];
+      if (r == 0 && c == 0) {
+        Wn = w * w;
+        Sn = s * s;
+      } else {
+        Wn = Wn + w * w;
+        Sn = Sn + s * s;
+      }
+    }
+  return sqrtf(Wn) / (sqrtf(Sn) + 1e-8f);
+}
+
+// MODE bits: 1 vorticity, 2 grad v, 4 d_vort from v* (v* written when MODE
+// >= 3), 8 the kinematic vorticity number from the pass's own grad v (the
+// KVN pass of fixed-M sampling, which reads nothing else that changes).
 template <int MODE, bool XM>
 __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params p, int32_t bound,
                                                     Packed pk) {
@@ -550,6 +631,7 @@ __global__ void __launch_bounds__(256) k_vort_gradv(dimcv_state st, dimcv_params
     for (int q = 0; q < 9; ++q) st.grad_v[9 * i + q] = G[q];
   if (MODE >= 3) st3(st.v_star, i, vadd(vi, vmuls(ld3(st.acceleration, i), p.dt)));
   if (MODE & 4) st3(st.d_vort, i, vsub(vort, curl));
+  if ((MODE & 8) && in_vortex_domain(p, xi)) st.kinematic_vort_num[i] = kvn_of(G);
 }
 
 // dimcv_sph.py:551-583
@@ -612,20 +694,7 @@ __global__ void k_kvn_sample(dimcv_state st, dimcv_params p, const float* __rest
   const float* G = st.grad_v + 9 * i;
   float g[9];
   for (int q = 0; q < 9; ++q) g[q] = G[q];
-  float Wn = 0.0f, Sn = 0.0f;
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) {
-      float w = g[3 * r + c] - g[3 * c + r];
-      float s = g[3 * r + c] + g[3 * c + r];
-      if (r == 0 && c == 0) {
-        Wn = w * w;
-        Sn = s * s;
-      } else {
-        Wn = Wn + w * w;
-        Sn = Sn + s * s;
-      }
-    }
-  float kvn = sqrtf(Wn) / (sqrtf(Sn) + 1e-8f);
+  const float kvn = kvn_of(g);
   float t0 = fminf(fmaxf(st.life_time[i], 0.0f), 1.0f);
   float rr = 0.018f * exp_f32(p.neg_gen_a * t0) + 0.004f;
   float produce = 1.0f - 1.0f / pow_f32(kvn + 1.0f, rr);
@@ -786,6 +855,10 @@ int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t par
   return DIMCV_OK;
 }
 
+// which: 0 density + factor; 3 density + factor + first density change; 1
+// v*, vorticity, grad v, d_vort; 5 as 1 with the fixed-M KVN, also zeroing
+// vorticity_vis's non-fluid slots; 4 smoothing + AKDE with kden partials; 6
+// as 4 after a 5 (no zero fill); 2 smoothing alone.
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials, const Packed* pk) {
   const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
@@ -793,11 +866,15 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
     int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
     if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
     if (rc) return rc;
+  } else if (which == 5) {  // as 1, and vorticity_vis for a following which 6, in one launch
+    int rc = zero_nonfluid(st, p, s, ZeroSet{{st->v_star, st->vorticity, st->grad_v, st->d_vort, st->vorticity_vis},
+                                             {3, 3, 9, 3, 3}});
+    if (rc) return rc;
   } else if (which == 2 || which == 4) {
     int rc = zero_nonfluid(st, p, s, st->vorticity_vis, 3);
     if (rc) return rc;
   }
-  if (which == 4 && !partials) {
+  if ((which == 4 || which == 6) && !partials) {
     set_error("launch_fused: kden partials required");
     return DIMCV_ERR_ARG;
   }
@@ -816,7 +893,12 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
       if (pk) hipLaunchKernelGGL((k_density_factor<true, true>), g, b, 0, s, *st, *p, fluid_bound, q);
       else hipLaunchKernelGGL((k_density_factor<true, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
+    case 5:
+      if (pk) hipLaunchKernelGGL((k_vort_gradv<15, true>), g, b, 0, s, *st, *p, fluid_bound, q);
+      else hipLaunchKernelGGL((k_vort_gradv<15, false>), g, b, 0, s, *st, *p, fluid_bound, q);
+      break;
     case 4:
+    case 6:
       if (pk) hipLaunchKernelGGL(k_smooth_vis_kden<true>, g, b, 0, s, *st, *p, fluid_bound, partials, q);
       else hipLaunchKernelGGL(k_smooth_vis_kden<false>, g, b, 0, s, *st, *p, fluid_bound, partials, q);
       break;

@@ -21,7 +21,9 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s);
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
-                       float* kap, hipStream_t s);
+                       float* kap, hipStream_t s, bool reset_jacobi = false);
+int launch_copy_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
+int launch_predict_life(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
                        hipStream_t s, const Packed* pk);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
@@ -141,8 +143,8 @@ static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimc
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
   CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, &pk));
-  CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s));
-  return jacobi_issue(0, st, p, o, fluid_bound, ws, s, n_first, true);
+  CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s, true));  // also resets the solve
+  return jacobi_issue(0, st, p, o, fluid_bound, ws, s, n_first, false);
 }
 
 // The substep after the divergence solve, in segments that hold kernels
@@ -159,10 +161,15 @@ static int seg_mid1(const dimcv_state* st, const dimcv_params* p, const dimcv_st
   // sample update), which reads only grad_v; then the smoothing pass, which
   // also produces vorticity_vis (compute_vorticity_vis below in the
   // reference) and the kernel density with its kden_sum partials.
-  CHECK(launch_fused(1, st, p, fluid_bound, s, nullptr, &pk));
-  if (o->m > 0) CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, fluid_bound, stream));
-  else CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
-  CHECK(launch_fused(4, st, p, fluid_bound, s, ws.partials, &pk));
+  // With fixed-M sampling the KVN is computed inside the vorticity pass, and
+  // one zero fill covers the non-fluid slots of both passes.
+  if (o->m > 0) {
+    CHECK(launch_fused(5, st, p, fluid_bound, s, nullptr, &pk));
+  } else {
+    CHECK(launch_fused(1, st, p, fluid_bound, s, nullptr, &pk));
+    CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
+  }
+  CHECK(launch_fused(o->m > 0 ? 6 : 4, st, p, fluid_bound, s, ws.partials, &pk));
   CHECK(reduce_tail_f64(ws.partials, nblocks(fluid_bound), st->dscal + DIMCV_D_KDEN_SUM, st->counters, nullptr, 0,
                         0, 0.0, 0, s));
   if (o->m > 0) CHECK(dimcv_cdf_sample(st, p, o->u_cdf, o->m, fluid_bound, stream));
@@ -186,21 +193,20 @@ static int seg_far(const dimcv_state* st, const dimcv_params* p, const dimcv_ste
 
 static int seg_mid2(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t particle_bound,
                     int32_t fluid_bound, hipStream_t s, const Workspace& ws, int n_first_p) {
-  void* stream = (void*)s;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
-  CHECK(dimcv_update_particle_life_time(st, p, fluid_bound, stream));
-  CHECK(dimcv_predict_velocity(st, p, particle_bound, stream));
+  // update_particle_life_time + predict_velocity, by slot (the fluid list
+  // holds exactly the fluid slots below particle_num)
+  CHECK(launch_predict_life(st, p, particle_bound, s));
   // pressure_solve (dimcv_sph.py:597-614)
   CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, &pk));
-  CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s));
-  return jacobi_issue(1, st, p, o, fluid_bound, ws, s, n_first_p, true);
+  CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s, true));  // also resets the solve
+  return jacobi_issue(1, st, p, o, fluid_bound, ws, s, n_first_p, false);
 }
 
 static int seg_tail(const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o, int32_t particle_bound,
                     hipStream_t s) {
   void* stream = (void*)s;
-  CHECK(dimcv_copy_x_temp(st, p, particle_bound, stream));
-  CHECK(dimcv_advect(st, p, particle_bound, stream));
+  CHECK(launch_copy_advect(st, p, particle_bound, s));  // copy_x_temp + advect
   if (o->bounce_back) CHECK(dimcv_enforce_boundary_3d(st, p, particle_bound, stream));
   return DIMCV_OK;
 }
