@@ -277,7 +277,7 @@ struct SegGraph {
   hipGraphExec_t exec;
 };
 // one cache per captured segment of dimcv_step
-enum { SEG_HEAD, SEG_MID1, SEG_MID2, SEG_TAIL, SEG_COUNT };
+enum { SEG_HEAD, SEG_MID1, SEG_MID2, SEG_COUNT };
 static std::vector<SegGraph> g_seg_graphs[SEG_COUNT];
 static hipStream_t g_capture_stream = nullptr;
 static bool g_graphs_off = false;
@@ -401,7 +401,8 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
                   s));
   CHECK(jacobi_wait(1, st, p, o, pb, ws, s, &o->out_kp, host_counters));
-  CHECK(run_graph(SEG_TAIL, [&](hipStream_t hs) { return seg_tail(st, p, o, pb, hs); }, key, s));
+  // one or two kernels: a direct launch starts sooner than a graph launch
+  CHECK(seg_tail(st, p, o, pb, s));
   CHECK(far_time(p, o));
   if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
     set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);

@@ -86,6 +86,8 @@ class DIMCVSPHSolver(SPHBase):
         seed = ps.cfg.get_cfg("rngSeed")
         self.rng = np.random.default_rng(0 if seed is None else seed)
         self._staged = {}             # pinned host + device buffers of the per-step uniforms
+        self._upload_stream = None    # side stream (default priority) and event of those uploads
+        self._upload_event = None
         self._uniforms = (None, None, None)
         self.last_iterations_v = 0
         self.last_iterations_p = 0
@@ -447,7 +449,16 @@ class DIMCVSPHSolver(SPHBase):
         args = (ctypes.byref(pre), ctypes.byref(post), ctypes.byref(ps.params), ctypes.byref(o), ps._pn)
         N.check(lib.dimcv_step_begin(*args, stream), "dimcv_step_begin")
         u = self.rng.random(m) if u_cdf is None else np.asarray(u_cdf)[:m]
-        t = self._upload_staged("u_cdf", u, torch.float64)
+        # the copy runs on a side stream, overlapping the device's build and
+        # divergence solve; the step's stream waits for it before the sampling
+        if self._upload_stream is None:
+            self._upload_stream = torch.cuda.Stream(device=ps.device)
+            self._upload_event = torch.cuda.Event()
+        main = torch.cuda.current_stream(ps.device)
+        with torch.cuda.stream(self._upload_stream):
+            t = self._upload_staged("u_cdf", u, torch.float64)
+            self._upload_event.record(self._upload_stream)
+        main.wait_event(self._upload_event)
         assert t.data_ptr() == o.u_cdf
         rc = lib.dimcv_step_finish(*args, ps._host_counters_ptr, stream)
         ps._pn = int(hc[N.C_PARTICLE_NUM])
