@@ -130,15 +130,16 @@ __global__ void k_activity_keys(dimcv_state st, dimcv_params p, int64_t n_in,
   }
   int c[3];
   cell_of(p, xi, c);
-  bool ok = true;
-  for (int d = 0; d < 3; ++d)
-    if (c[d] < 0 || c[d] >= p.grid_num[d]) ok = false;
-  if (!ok) {
+  // the flat index as the reference forms it: a cell coordinate one past the
+  // grid in y or z (a particle exactly on the upper face) lands in the next
+  // row's cell there too; only a flat index outside the table is an error
+  const int64_t g = ((int64_t)c[0] * p.grid_num[1] + c[1]) * p.grid_num[2] + c[2];
+  if (g < 0 || g >= p.n_cells) {
     atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
     keys[i] = sentinel;
     return;
   }
-  keys[i] = (uint32_t)(c[0] * p.grid_num[1] * p.grid_num[2] + c[1] * p.grid_num[2] + c[2]);
+  keys[i] = (uint32_t)g;
 }
 
 // particle_num = fluid + solid (:467-468), never above the slots scanned.

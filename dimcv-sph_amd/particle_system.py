@@ -11,8 +11,10 @@ Differences of substance from the reference (all deliberate):
     reference slot order with a stable radix sort;
   * it also builds each fluid particle's neighbour list (the set and order
     ``for_all_neighbors`` visits), reused by every neighbour pass of the step;
-  * an active particle outside the cell grid raises instead of writing out of
-    bounds; slots at or beyond ``particle_num`` hold unspecified values
+  * an active particle whose flat cell index (formed as the reference forms
+    it, so one exactly on the upper y or z face lands in the next row's cell)
+    falls outside the cell table raises instead of writing out of bounds;
+    slots at or beyond ``particle_num`` hold unspecified values
     (the reference fills them with -1/0/255 sentinels; nothing reads them);
   * 3-D only; mesh rigid bodies (trimesh) and dynamic rigid blocks raise.
 """

@@ -209,3 +209,24 @@ def test_step_entry_equals_begin_finish():
             assert np.array_equal(a.to_numpy()[:pn], b.to_numpy()[:pn]), (k, name)
         assert sol_a.last_iterations_v == sol_b.last_iterations_v
         assert sol_a.last_iterations_p == sol_b.last_iterations_p
+
+
+def test_particle_on_upper_face_keys_like_the_reference():
+    """A fluid particle exactly on the domain's upper z face stays active and
+    takes the flat cell index the reference forms (the next row's first
+    cell); grid build and step match the oracle."""
+    ps, solver, ref, info = H.make_ring_pair(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="exact")
+    pn = ps.particle_num[None]
+    x = ps.x.to_numpy()
+    # a particle of the first x column, moved onto the z = domain_end face
+    # (grid 5 cells in z: its cell coordinate is one past the grid)
+    cand = np.nonzero(x[:pn, 0] < np.float32(0.04))[0]
+    k = int(cand[np.argmax(x[cand, 2])])
+    x[k, 2] = np.float32(ps.params.domain_end[2])
+    ps.x.from_numpy(x)
+    ref.field("x")[:] = x
+    u = np.random.default_rng(4).random(128)
+    solver.step(u_cdf=u)
+    ref.step(u_cdf=u)
+    bad = H.mismatches(ps, solver, ref)
+    assert not bad, bad
