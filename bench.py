@@ -405,7 +405,7 @@ def main():
                    "n_particles": res["n_fluid"], "lattice": [128, 128, 64], "rings": 8,
                    "mc_samples": res["m"], "dt": 2e-4, "far_field": args.far_field,
                    "mean_k_v": res["kv"], "mean_k_p": res["kp"], "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_vv_far_fast (MC-VVPE far field)",
+        "roofline": {"bound": "mfma", "kernel": "MC-VVPE far field: k_far_split over two sample halves + k_far_combine (k_vv_far_fast when unsplit)",
                      "achieved": round(res["achieved_tflops"], 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(res["achieved_tflops"] / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic, "avg_launch_ms": round(res["far_ms"], 5),

@@ -123,18 +123,29 @@ __global__ void k_activity_keys(dimcv_state st, dimcv_params p, int64_t n_in,
       }
     }
   }
+  // the flat index as the reference forms it: a cell coordinate one past the
+  // grid in y or z (a particle exactly on an upper face) lands in the next
+  // row's cell there too
+  int c[3];
+  cell_of(p, xi, c);
+  const int64_t g = ((int64_t)c[0] * p.grid_num[1] + c[1]) * p.grid_num[2] + c[2];
+  const bool in_table = g >= 0 && g < p.n_cells;
+  if (active && !in_table && mat == 1 && !p.periodic) {
+    // an active fluid particle whose index lies past the table (exactly on
+    // the domain's upper x face, or the last row's upper y/z face): the
+    // reference indexes out of bounds there; it leaves the domain as one
+    // past the face does
+    active = 0;
+    atomicSub(&st.counters[DIMCV_C_FLUID_NUM], 1);
+    int32_t o = st.object_id[i];
+    if (st.n_emitters > 0 && o >= 0 && o < st.n_emitters) atomicSub(&st.num_particles_each_emitter[o], 1);
+  }
   st.is_active[i] = active;
   if (!active) {
     keys[i] = sentinel;
     return;
   }
-  int c[3];
-  cell_of(p, xi, c);
-  // the flat index as the reference forms it: a cell coordinate one past the
-  // grid in y or z (a particle exactly on the upper face) lands in the next
-  // row's cell there too; only a flat index outside the table is an error
-  const int64_t g = ((int64_t)c[0] * p.grid_num[1] + c[1]) * p.grid_num[2] + c[2];
-  if (g < 0 || g >= p.n_cells) {
+  if (!in_table) {  // a solid outside the table
     atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
     keys[i] = sentinel;
     return;

@@ -461,17 +461,28 @@ void or_update_activity(Sim* s) {
         else if (q >= s->f_dend[j]) q -= s->f_box[j];
       }
     } else if (s->material[i] == 1) {
+      // outside the domain box; or, build-defined, an active particle whose
+      // flat cell index lies past the cell table (exactly on the upper x
+      // face, or the last row's upper y/z face), where the reference indexes
+      // out of bounds: it leaves as one past the face does
+      bool leave = false;
       for (int j = 0; j < 3; ++j) {
         float xj = s->x[3 * i + j];
-        if (xj < s->f_dstart[j] || xj > s->f_dend[j]) {
-          s->is_active[i] = 0;
-          s->fluid_particle_num -= 1;
-          if (s->cfg.num_emitters > 0) {
-            int32_t o = s->object_id[i];
-            if (o >= 0 && o < (int32_t)s->num_particles_each_emitter.size())
-              s->num_particles_each_emitter[o] -= 1;
-          }
-          break;
+        if (xj < s->f_dstart[j] || xj > s->f_dend[j]) leave = true;
+      }
+      if (!leave && s->is_active[i]) {
+        int c[3];
+        s->cell_of(s->X(i), c);
+        const int64_t g = s->flat(c);
+        leave = g < 0 || g >= s->ncells;
+      }
+      if (leave) {
+        s->is_active[i] = 0;
+        s->fluid_particle_num -= 1;
+        if (s->cfg.num_emitters > 0) {
+          int32_t o = s->object_id[i];
+          if (o >= 0 && o < (int32_t)s->num_particles_each_emitter.size())
+            s->num_particles_each_emitter[o] -= 1;
         }
       }
     }

@@ -214,7 +214,9 @@ def test_step_entry_equals_begin_finish():
 def test_particle_on_upper_face_keys_like_the_reference():
     """A fluid particle exactly on the domain's upper z face stays active and
     takes the flat cell index the reference forms (the next row's first
-    cell); grid build and step match the oracle."""
+    cell); one exactly on the upper x face (flat index past the table, where
+    the reference indexes out of bounds) leaves the domain.  Grid build and
+    step match the oracle."""
     ps, solver, ref, info = H.make_ring_pair(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="exact")
     pn = ps.particle_num[None]
     x = ps.x.to_numpy()
@@ -223,6 +225,8 @@ def test_particle_on_upper_face_keys_like_the_reference():
     cand = np.nonzero(x[:pn, 0] < np.float32(0.04))[0]
     k = int(cand[np.argmax(x[cand, 2])])
     x[k, 2] = np.float32(ps.params.domain_end[2])
+    kx = int(np.argmax(np.where(np.arange(pn) == k, -1.0, x[:pn, 0])))
+    x[kx, 0] = np.float32(ps.params.domain_end[0])
     ps.x.from_numpy(x)
     ref.field("x")[:] = x
     u = np.random.default_rng(4).random(128)
@@ -230,3 +234,4 @@ def test_particle_on_upper_face_keys_like_the_reference():
     ref.step(u_cdf=u)
     bad = H.mismatches(ps, solver, ref)
     assert not bad, bad
+    assert ps.particle_num[None] == ref.particle_num == pn - 1
