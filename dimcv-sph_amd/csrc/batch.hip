@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_batch_scene_setup(dimcv_state st, dimcv
     c[DIMCV_SC_SLOTS] = b - a;
     c[DIMCV_SC_FLUID_SRC] = lo;
     c[DIMCV_SC_FLUID] = hi - lo;
-    c[DIMCV_SC_SPLITS] = far_split_count(hi - lo);
+    c[DIMCV_SC_SPLITS] = far_split_count_mc(hi - lo);
   }
   __syncthreads();
   // far-field order: larger per-item source chunks first, then scene index

@@ -649,12 +649,9 @@ static int far_splits(int64_t targets, const Workspace& ws) {
   return (int64_t)s * targets > ws.far_partial_cap ? 1 : s;
 }
 
-// The MC far field splits its sample range for twice the waves of
-// far_split_count (two halves at 1M targets: 1.16 ms against 1.22 ms for one
-// pass, the tail of the last round of waves being the difference), as far as
-// the partials fit the workspace.
+// far_split_count_mc as far as the partials fit the workspace.
 static int far_splits_mc(int64_t targets, const Workspace& ws) {
-  int64_t s = far_split_count((targets + 1) / 2);
+  int64_t s = far_split_count_mc(targets);
   const int64_t fit = targets > 0 ? ws.far_partial_cap / targets : 1;
   if (s > fit) s = fit;
   return s < 1 ? 1 : (int)s;
