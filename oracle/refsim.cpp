@@ -12,7 +12,8 @@
 // OpenMP; this does not change any result.
 //
 // Build-defined deviations from the reference (each one is required by the
-// north-star contract or removes an unpinnable backend detail):
+// host-fed-uniform, FP64-reduction design of this path or removes an
+// unpinnable backend detail):
 //  * exp/log/pow: refmath.h (ti.exp / ti.pow last-ulp behaviour is backend
 //    defined);  ti.pow(t, 3.0) is evaluated as (t*t)*t.
 //  * residual sums (dimcv_sph.py:407-419) and kden_sum (:738-751) are FP64

@@ -3,8 +3,8 @@
 Bit-exact for integer/index work (sort order, cell table, neighbour lists,
 sample indices) and, because both sides evaluate every f32 expression in the
 same order without fused operations, for the SPH fields too.  The "fast" and
-"kahan" far-field modes are checked at 1e-5 relative L2 (the north-star FP32
-tolerance) against the exact mode on the same state.
+"kahan" far-field modes are checked at 1e-5 relative L2 (the FP32 tolerance
+of the velocity field) against the exact mode on the same state.
 """
 import numpy as np
 import pytest
