@@ -44,7 +44,7 @@ typedef enum dimcv_status {
   DIMCV_ERR_ARG = -1,        /* invalid argument / shape */
   DIMCV_ERR_HIP = -2,        /* HIP runtime error */
   DIMCV_ERR_WORKSPACE = -3,  /* workspace too small */
-  DIMCV_ERR_KEY = -4,        /* active particle outside the cell grid */
+  DIMCV_ERR_KEY = -4,        /* active solid particle whose cell key lies past the cell table */
   DIMCV_ERR_NBR = -5,        /* neighbour list width exceeded */
   DIMCV_ERR_RANGE = -6       /* size beyond a documented maximum */
 } dimcv_status;
@@ -57,7 +57,7 @@ typedef enum dimcv_counter {
   DIMCV_C_NUM_SAMPLES = 3,    /* solver.num_samples         dimcv_sph.py:46 */
   DIMCV_C_FLUID_LIST = 4,     /* entries in state->fluid_list (= fluid slots) */
   DIMCV_C_NBR_MAX = 5,        /* largest neighbour count seen by the last build */
-  DIMCV_C_KEY_ERROR = 6,      /* active particles with an out-of-grid cell key */
+  DIMCV_C_KEY_ERROR = 6,      /* active solids whose cell key lies past the cell table */
   DIMCV_C_PREV_NUM = 7,       /* particle count before the last activity update */
   DIMCV_C_SAMPLE_SLOTS = 8,   /* is_sample count (compaction scratch) */
   DIMCV_C_JACOBI_DONE = 9,    /* native substep: current solve has converged */
@@ -289,8 +289,11 @@ int dimcv_init_cylinder(const dimcv_state* st, const dimcv_params* p, int32_t ob
  * Slot order equals the serial reference order (ascending compaction, stable
  * counting sort).  The caller swaps every [sorted] field with its buffer
  * afterwards.  Synchronises the stream and copies counters[] to host_counters
- * (DIMCV_C_COUNT int32).  Returns DIMCV_ERR_KEY when an active particle lies
- * outside the grid (undefined behaviour in the reference). */
+ * (DIMCV_C_COUNT int32).  Cell keys are the reference's flat index; a fluid
+ * particle whose key lies past the cell table (exactly on the upper x face)
+ * is deactivated as one outside the domain, and DIMCV_ERR_KEY is returned
+ * when an active solid's key does (both undefined behaviour in the
+ * reference). */
 int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_particle_num,
                      int32_t* host_counters, void* stream);
 /* Fixed-radius neighbour list of every fluid slot, the set and order that
