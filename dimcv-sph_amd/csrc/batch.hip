@@ -399,7 +399,7 @@ static int read_iters(const dimcv_state* st, int32_t* out_iters, int32_t* tmp, h
 
 static int check_counters(const int32_t* hc, const int32_t* misc_err, const dimcv_params* p) {
   if (hc[DIMCV_C_KEY_ERROR] != 0) {
-    set_error("%d active particles lie outside their scene's cell grid", hc[DIMCV_C_KEY_ERROR]);
+    set_error("%d active solid particles lie past their scene's cell table", hc[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;
   }
   if (hc[DIMCV_C_NBR_MAX] > p->nbr_kmax) {

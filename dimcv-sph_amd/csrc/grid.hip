@@ -581,22 +581,29 @@ __global__ void k_batch_keys(dimcv_state st, dimcv_params p, int64_t n_in, uint3
       }
     }
   }
+  // the scene's flat index, with the solo step's rule at the upper faces
+  // (k_activity_keys), so a batched scene keys as a solo one
+  int c[3];
+  c[0] = (int)(xi.x / p.grid_size);
+  c[1] = (int)(xi.y / p.grid_size);
+  c[2] = (int)(xi.z / p.grid_size);
+  const int64_t g = ((int64_t)c[0] * sc.grid_num[1] + c[1]) * sc.grid_num[2] + c[2];
+  const bool in_table = g >= 0 && g < (int64_t)sc.n_cells;
+  if (active && !in_table && mat == 1) {
+    active = 0;
+    atomicSub(&st.counters[DIMCV_C_FLUID_NUM], 1);
+  }
   st.is_active[i] = active;
   if (!active) {
     keys[i] = sentinel;
     return;
   }
-  int c[3];
-  c[0] = (int)(xi.x / p.grid_size);
-  c[1] = (int)(xi.y / p.grid_size);
-  c[2] = (int)(xi.z / p.grid_size);
-  for (int d = 0; d < 3; ++d)
-    if (c[d] < 0 || c[d] >= sc.grid_num[d]) {
-      atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
-      keys[i] = sentinel;
-      return;
-    }
-  keys[i] = (uint32_t)(sc.cell_base + (c[0] * sc.grid_num[1] * sc.grid_num[2] + c[1] * sc.grid_num[2] + c[2]));
+  if (!in_table) {
+    atomicAdd(&st.counters[DIMCV_C_KEY_ERROR], 1);
+    keys[i] = sentinel;
+    return;
+  }
+  keys[i] = (uint32_t)(sc.cell_base + g);
 }
 
 // Neighbour lists of fluid-list entry q over the 9 z-runs of its scene's
@@ -773,7 +780,7 @@ int dimcv_grid_build(const dimcv_state* st, const dimcv_params* p, int32_t prev_
                              hipMemcpyDeviceToHost, s));
     DIMCV_HIP(hipStreamSynchronize(s));
     if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
-      set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
+      set_error("%d active solid particles lie past the cell table", host_counters[DIMCV_C_KEY_ERROR]);
       return DIMCV_ERR_KEY;
     }
   }

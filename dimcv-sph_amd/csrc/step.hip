@@ -405,7 +405,7 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   CHECK(seg_tail(st, p, o, pb, s));
   CHECK(far_time(p, o));
   if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
-    set_error("%d active particles lie outside the cell grid", host_counters[DIMCV_C_KEY_ERROR]);
+    set_error("%d active solid particles lie past the cell table", host_counters[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;
   }
   if (host_counters[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
