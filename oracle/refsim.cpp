@@ -1111,6 +1111,43 @@ void or_compute_vorticity_vis(Sim* s) {
   }
 }
 
+// sph_base.py:118-123,149-179 (enforce_boundary_3D with particle_type =
+// fluid, called by karman_vortex.py:226-230 when bounceBackBoundary is set).
+// The bounds are f32(domain_size - padding) and f32(padding), padding =
+// grid_size (particle_system.py:58); every test reads the position captured
+// before the clamps.  Slots below particle_num (the rest hold nothing the
+// step reads).
+void or_enforce_boundary_3d(Sim* s) {
+  const float pad = (float)s->support;
+  float hi[3];
+  for (int d = 0; d < 3; ++d)
+    hi[d] = (float)((s->cfg.domain_end[d] - s->cfg.domain_start[d]) - s->support);
+  for (int64_t p_i = 0; p_i < s->particle_num; ++p_i) {
+    if (!s->is_fluid(p_i) || !s->is_dynamic[p_i]) continue;
+    V3 pos = s->X(p_i);
+    float q[3] = {pos.x, pos.y, pos.z};
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < 3; ++d) {
+      if (q[d] > hi[d]) {
+        n[d] += 1.0f;
+        s->x[3 * p_i + d] = hi[d];
+      }
+      if (q[d] <= pad) {
+        n[d] += -1.0f;
+        s->x[3 * p_i + d] = pad;
+      }
+    }
+    V3 cn{n[0], n[1], n[2]};
+    float len = vnorm(cn);
+    if (len > 1e-6f) {
+      V3 vec = vdivs(cn, len);
+      V3 v = s->Vv(p_i);
+      const float c_f = 0.5f;
+      Sim::put(s->v, p_i, vsub(v, smul((1.0f + c_f) * vdot(v, vec), vec)));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Build-defined comparators (no reference function; see the module header).
 

@@ -96,7 +96,7 @@ def lib():
                  "or_record_vort_part_sample_idx", "or_compute_kernel_density",
                  "or_update_particle_life_time", "or_compute_density_adv",
                  "or_pressure_solve_iteration_kernel", "or_copy_x_temp",
-                 "or_compute_vorticity_vis"]:
+                 "or_compute_vorticity_vis", "or_enforce_boundary_3d"]:
         fn = getattr(L, name)
         fn.restype = None
         fn.argtypes = [P]
@@ -353,6 +353,9 @@ class RefSim:
         self.initialize_particle_system()
         L.or_compute_moving_boundary_volume(self.sim)
         self.substep(u1, u2, u_cdf)
+        # karman_vortex.py:226-230 (3-D scenes: enforce_boundary_3D on fluid)
+        if self.cfg.get("bounceBackBoundary"):
+            L.or_enforce_boundary_3d(self.sim)
 
     def substep(self, u1=None, u2=None, u_cdf=None):
         L, s = lib(), self.sim
