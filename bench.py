@@ -14,8 +14,8 @@ Also measured in the same run:
     stream -> roofline vs the FP32 vector peak;
   * the direct O(N^2) Biot-Savart comparator at N = 131,072 (64 x 64 x 32
     lattice, same generator) in FP32, against which the FP32-compensated
-    MC-VVPE step is compared at the relative L2 velocity error both reach
-    against the FP64 direct sum;
+    MC-VVPE recovery is compared at its relative L2 error of the recovered
+    field against the FP64 direct sum, M swept up to N;
   * the CPU oracle (a C++ restatement of the reference step, OpenMP) on a
     bounded sample of the same 1M workload.
 
@@ -186,17 +186,34 @@ def run_recovery_comparison(nx, ny, nz, ms_list, direct_reps=2):
         row["speedup_vs_direct_f32"] = round(out["direct_f32_ms"] / row["kahan_ms"], 2)
         sweep.append(row)
     out["mc"] = sweep
-    ok = [r for r in sweep if r["kahan_rel_l2_err_velocity"] <= 1e-2]
+    # the comparison point: the recovered field v_vort (what the MC estimator
+    # computes) against the FP64 direct sum; MC noise falls as M^-1/2
+    ok = [r for r in sweep if r["kahan_rel_l2_err_recovered"] <= 1e-2]
     if ok:
         best = max(ok, key=lambda r: r["speedup_vs_direct_f32"])
-        out["at_1e-2_velocity_error"] = {"m": best["m"], "kahan_ms": best["kahan_ms"],
-                                         "speedup_vs_direct_f32": best["speedup_vs_direct_f32"],
-                                         "rel_l2_err_velocity": best["kahan_rel_l2_err_velocity"]}
+        out["at_1e-2_recovered_error"] = {"reached": True, "m": best["m"], "kahan_ms": best["kahan_ms"],
+                                          "speedup_vs_direct_f32": best["speedup_vs_direct_f32"],
+                                          "rel_l2_err_recovered": best["kahan_rel_l2_err_recovered"]}
+    else:
+        # fit err = a M^-1/2 through the sweep; the M it predicts for 1e-2 and
+        # the MC time there (time linear in M) against the direct sum
+        ms_ = np.array([r["m"] for r in sweep], float)
+        es = np.array([r["kahan_rel_l2_err_recovered"] for r in sweep], float)
+        a = float(np.exp(np.mean(np.log(es) + 0.5 * np.log(ms_))))
+        m_need = (a / 1e-2) ** 2
+        t_per_m = float(np.mean([r["kahan_ms"] / r["m"] for r in sweep[-2:]]))
+        out["at_1e-2_recovered_error"] = {
+            "reached": False, "max_m_swept": int(ms_.max()),
+            "rel_l2_err_recovered_at_max_m": float(es[np.argmax(ms_)]),
+            "fit": f"err = {a:.3f} M^-1/2", "m_predicted_for_1e-2": int(round(m_need)),
+            "predicted_speedup_vs_direct_f32": round(out["direct_f32_ms"] / (t_per_m * m_need), 4),
+            "note": "the importance-sampled estimator does not reach 1e-2 at any M <= N on this state; "
+                    "the north-star target of >= 20x the direct sum at <= 1e-2 is not met"}
     out["error_definitions"] = (
-        "velocity: relative L2 error over fluid particles of the velocity after the recovery update "
-        "(the quantity the step advances) against the same update with the FP64 direct Biot-Savart field; "
-        "recovered: relative L2 error of the recovered field v_vort alone against the FP64 direct field "
-        "(Monte Carlo noise; approaches 1e-2 only as M approaches N)")
+        "recovered: relative L2 error over fluid particles of the recovered field v_vort (near + MC far "
+        "part) against the FP64 direct Biot-Savart field -- the comparison point; velocity: the same for "
+        "the velocity after the gated update, dominated by the unchanged velocity, reported for reference "
+        "only")
     ps.v.t.copy_(v0)
     log(f"recovery comparison at N={pn}: {json.dumps(out)}")
     del solver, ps
@@ -377,8 +394,8 @@ def main():
     res, info = run_gpu(args)
     direct = None
     if not args.no_direct:
-        direct = {"comparator_131072": run_recovery_comparison(64, 64, 32, [256, 1024, 4096]),
-                  "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536])}
+        direct = {"comparator_131072": run_recovery_comparison(64, 64, 32, [256, 1024, 4096, 16384, 131072]),
+                  "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536, 262144])}
     batching = None if args.no_batching else run_batching(steps=args.batch_steps)
     adjoint = None if args.no_adjoint else run_adjoint()
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(info, steps=5)
@@ -405,14 +422,14 @@ def main():
                    "n_particles": res["n_fluid"], "lattice": [128, 128, 64], "rings": 8,
                    "mc_samples": res["m"], "dt": 2e-4, "far_field": args.far_field,
                    "mean_k_v": res["kv"], "mean_k_p": res["kp"], "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "MC-VVPE far field: k_far_split over two sample halves + k_far_combine (k_vv_far_fast when unsplit)",
+        "roofline": {"bound": "valu", "kernel": "MC-VVPE far field: k_far_split over two sample halves + k_far_combine (k_vv_far_fast when unsplit)",
                      "achieved": round(res["achieved_tflops"], 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(res["achieved_tflops"] / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic, "avg_launch_ms": round(res["far_ms"], 5),
                      "work_per_launch": f"{res['pairs']} (target, sample) pairs x {FLOPS_PER_PAIR_MC} flop",
                      "issue_ceiling": FP32_ISSUE_TFLOPS,
                      "frac_of_issue_ceiling": round(res["achieved_tflops"] / FP32_ISSUE_TFLOPS, 4),
-                     "note": "FP32 vector-ALU bound. peak = MI355X FP32 157.3 TFLOP/s. Built without packed "
+                     "note": "FP32 vector-ALU bound (no matrix-core work). peak = MI355X FP32 vector 157.3 TFLOP/s. Built without packed "
                              "FP32 (-packed-fp32-ops); the loop issues 59 vector instructions per four samples (14.75 per pair) and "
                              "the SIMDs dual-issue part of them (SQ_ACTIVE_INST_VALU2), so the issue ceiling is 2 "
                              "instructions per SIMD quad-cycle: issue_ceiling TFLOP/s at 24 flop per pair. A "

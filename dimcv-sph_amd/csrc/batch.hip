@@ -98,6 +98,7 @@ int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw) {
 // ---------------------------------------------------------------------------
 __global__ void k_batch_begin(int32_t* counters, int32_t* misc) {
   counters[DIMCV_C_KEY_ERROR] = 0;
+  counters[DIMCV_C_STEP_ERROR] = 0;
   misc[BM_ERROR] = 0;
 }
 
@@ -436,7 +437,7 @@ int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_parti
   int64_t cap = 0;
   for (int32_t s = 0; s < n_scenes; ++s) {
     const int64_t n = scene_particles[s] > 0 ? scene_particles[s] : 0;
-    cap += 16 * n < (int64_t)(1 << 21) ? 16 * n : (int64_t)(1 << 21);
+    cap += 16 * n < DIMCV_FAR_PARTIAL_MAX ? 16 * n : DIMCV_FAR_PARTIAL_MAX;
   }
   return batch_layout(n_scenes, n_max, cap, nullptr, nullptr);
 }

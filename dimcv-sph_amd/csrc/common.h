@@ -422,6 +422,12 @@ __device__ __forceinline__ int block_scene_of(const dimcv_state& st) {
   return st.block_scene ? st.block_scene[xcd_block()] : 0;
 }
 
+// A native step whose build failed (DIMCV_C_STEP_ERROR): the kernels that
+// write x, v, is_sample and life_time return without writing.
+__device__ __forceinline__ bool step_failed(const dimcv_state& st) {
+  return st.counters[DIMCV_C_STEP_ERROR] != 0;
+}
+
 // Jacobi gate: the solve of this block's scene (batched: done points at
 // scene_counters + DIMCV_SC_DONE) or the single solve has converged.
 // Launches using it are 256-wide over fluid-list entries, in xcd_block order.
@@ -458,9 +464,16 @@ __host__ __device__ __forceinline__ int far_split_count(int64_t targets) {
 // particles, the same in the single-scene and batched steps (so a batched
 // scene sums in the order of a solo one): from 2^19 targets, twice the waves
 // of far_split_count (two halves at 1M targets run 1.16 ms against 1.22 ms
-// for one pass, the tail of the last round of waves being the difference).
+// for one pass, the tail of the last round of waves being the difference),
+// clamped so that splits * targets <= 2^21, the partial capacity both steps
+// reserve (the solo workspace min(16 n_max, 2^21), a batched scene min(16 n_s,
+// 2^21)).
+#define DIMCV_FAR_PARTIAL_MAX (1LL << 21)
 __host__ __device__ __forceinline__ int far_split_count_mc(int64_t targets) {
-  return far_split_count(targets >= (1 << 19) ? (targets + 1) / 2 : targets);
+  int64_t s = far_split_count(targets >= (1 << 19) ? (targets + 1) / 2 : targets);
+  const int64_t fit = targets > 0 ? DIMCV_FAR_PARTIAL_MAX / targets : s;
+  if (s > fit) s = fit;
+  return s < 1 ? 1 : (int)s;
 }
 
 // ---------------------------------------------------------------------------

@@ -159,6 +159,7 @@ __global__ void k_set_particle_num(int32_t* counters, int32_t n_in) {
   counters[DIMCV_C_PREV_NUM] = counters[DIMCV_C_PARTICLE_NUM];
   counters[DIMCV_C_PARTICLE_NUM] = pn < n_in ? pn : n_in;
   counters[DIMCV_C_NBR_MAX] = 0;
+  counters[DIMCV_C_STEP_ERROR] = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -242,7 +242,8 @@ __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int w
                               float* __restrict__ kap, int reset) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (reset && f == 0) {
-    st.counters[DIMCV_C_JACOBI_DONE] = 0;
+    // a failed build leaves the solve done before its first iteration
+    st.counters[DIMCV_C_JACOBI_DONE] = step_failed(st) ? 1 : 0;
     st.counters[DIMCV_C_JACOBI_ITERS] = 0;
   }
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
@@ -476,7 +477,7 @@ __global__ void k_copy_advect(dimcv_state st, dimcv_params p, int64_t n_max, int
   const int32_t pn = st.counters[DIMCV_C_PARTICLE_NUM];
   const v3 x = s < pn ? ld3(st.x, s) : mk(0.0f, 0.0f, 0.0f);
   st3(st.x_temp, s, x);
-  if (s >= bound || s >= pn || !st.is_dynamic[s]) return;
+  if (s >= bound || s >= pn || !st.is_dynamic[s] || step_failed(st)) return;
   v3 v = ld3(st.v, s);
   if (st.material[s] == 0) {
     v = vadd(v, smul(p.dt, ld3(st.acceleration, s)));
@@ -490,7 +491,7 @@ __global__ void k_copy_advect(dimcv_state st, dimcv_params p, int64_t n_max, int
 // update runs by slot on the same particles.
 __global__ void k_predict_life(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM] || step_failed(st)) return;
   if (st.material[s] != 1) return;
   if (in_vortex_domain(p, ld3(st.x, s))) st.life_time[s] += p.dt;
   if (st.is_dynamic[s]) st3(st.v, s, vadd(ld3(st.v, s), smul(p.dt, ld3(st.acceleration, s))));
@@ -528,7 +529,7 @@ __global__ void __launch_bounds__(256) k_vorticity_vis(dimcv_state st, dimcv_par
 // sph_base.py:118-123,149-179 for fluid particles
 __global__ void k_bounce(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM] || step_failed(st)) return;
   if (st.material[s] != 1 || !st.is_dynamic[s]) return;
   v3 pos = ld3(st.x, s);
   float q[3] = {pos.x, pos.y, pos.z};
@@ -700,7 +701,7 @@ __global__ void k_kvn_sample(dimcv_state st, dimcv_params p, const float* __rest
   float produce = 1.0f - 1.0f / pow_f32(kvn + 1.0f, rr);
   float del = exp_f32(p.neg_del_b * kvn);
   st.kinematic_vort_num[i] = kvn;
-  if (!do_bernoulli) return;
+  if (!do_bernoulli || step_failed(st)) return;
   uint8_t smp = st.is_sample[i];
   if (smp == 0 && u1[i] < produce) st.is_sample[i] = 1;
   else if (smp == 1 && kvn < 2.0f && u2[i] < del) st.is_sample[i] = 0;

@@ -36,8 +36,37 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
 static hipEvent_t g_step_ev[2] = {nullptr, nullptr};
 
 __global__ void k_jacobi_reset(int32_t* counters) {
-  counters[DIMCV_C_JACOBI_DONE] = 0;
+  counters[DIMCV_C_JACOBI_DONE] = counters[DIMCV_C_STEP_ERROR] != 0 ? 1 : 0;
   counters[DIMCV_C_JACOBI_ITERS] = 0;
+}
+
+// The build's outcome, before the substep writes anything: an active solid
+// keyed past the cell table, a list wider than nbr_kmax (the lists are
+// truncated) or no fluid (the reference divides by fluid_particle_num).
+// Nonzero makes the substep's state writers return (step_failed).
+__global__ void k_step_guard(int32_t* counters, int32_t nbr_kmax) {
+  int32_t e = 0;
+  if (counters[DIMCV_C_KEY_ERROR] != 0) e |= 1;
+  if (counters[DIMCV_C_NBR_MAX] > nbr_kmax) e |= 2;
+  if (counters[DIMCV_C_FLUID_NUM] <= 0) e |= 4;
+  counters[DIMCV_C_STEP_ERROR] = e;
+}
+
+// Host side of the same test, from a copy of the counters.
+static int step_status(const int32_t* hc, const dimcv_params* p) {
+  if (hc[DIMCV_C_KEY_ERROR] != 0) {
+    set_error("%d active solid particles lie past the cell table", hc[DIMCV_C_KEY_ERROR]);
+    return DIMCV_ERR_KEY;
+  }
+  if (hc[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
+    set_error("neighbour list width %d exceeds nbr_kmax %d", hc[DIMCV_C_NBR_MAX], p->nbr_kmax);
+    return DIMCV_ERR_NBR;
+  }
+  if (hc[DIMCV_C_FLUID_NUM] <= 0) {
+    set_error("dimcv_step: no fluid particles (the reference divides by fluid_particle_num)");
+    return DIMCV_ERR_ARG;
+  }
+  return DIMCV_OK;
 }
 
 
@@ -139,6 +168,7 @@ static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimc
                         int32_t particle_bound, int32_t fluid_bound, hipStream_t s, const Workspace& ws,
                         int n_first) {
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
+  hipLaunchKernelGGL(k_step_guard, dim3(1), dim3(1), 0, s, st->counters, p->nbr_kmax);
   CHECK(launch_pack_xm(st, p, particle_bound, pk, s));
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
@@ -395,7 +425,11 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   const HeadKey& key = sp.key;
   const int32_t pb = particle_bound;
   const int n_first_p = sp.n_first_p;
-  CHECK(jacobi_wait(0, st, p, o, pb, ws, s, &o->out_kv));
+  // the divergence solve's checks also bring the build's outcome: a failed
+  // build ends the step here (its solve was done before it began, and no
+  // writer of x, v, is_sample or life_time has run)
+  CHECK(jacobi_wait(0, st, p, o, pb, ws, s, &o->out_kv, host_counters));
+  if (host_counters[DIMCV_C_STEP_ERROR] != 0) return step_status(host_counters, p);
   CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
   CHECK(seg_far(st, p, o, pb, s));
   CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
@@ -404,19 +438,7 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   // one or two kernels: a direct launch starts sooner than a graph launch
   CHECK(seg_tail(st, p, o, pb, s));
   CHECK(far_time(p, o));
-  if (host_counters[DIMCV_C_KEY_ERROR] != 0) {
-    set_error("%d active solid particles lie past the cell table", host_counters[DIMCV_C_KEY_ERROR]);
-    return DIMCV_ERR_KEY;
-  }
-  if (host_counters[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
-    set_error("neighbour list width %d exceeds nbr_kmax %d", host_counters[DIMCV_C_NBR_MAX], p->nbr_kmax);
-    return DIMCV_ERR_NBR;
-  }
-  if (host_counters[DIMCV_C_FLUID_NUM] <= 0) {
-    set_error("dimcv_step: no fluid particles (the reference divides by fluid_particle_num)");
-    return DIMCV_ERR_ARG;
-  }
-  return DIMCV_OK;
+  return step_status(host_counters, p);
 }
 
 extern "C" int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,

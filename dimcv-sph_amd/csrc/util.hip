@@ -57,7 +57,7 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.tiny = (int32_t*)take(256);
   // split far-field partials: splits * targets of far_split_count for any
   // target count up to n (16 n when n < 65536, below 2^21 otherwise)
-  w.far_partial_cap = n * 16 < (int64_t)(1 << 21) ? n * 16 : (int64_t)(1 << 21);
+  w.far_partial_cap = n * 16 < DIMCV_FAR_PARTIAL_MAX ? n * 16 : DIMCV_FAR_PARTIAL_MAX;
   w.far_partials = take(32 * w.far_partial_cap);
   int64_t ms = max_samples > 0 ? max_samples : 1;
   w.samples = (float4*)take(32 * ms);

@@ -54,6 +54,7 @@ __global__ void __launch_bounds__(256) k_vv_near(dimcv_state st, dimcv_params p,
 // Final combination (dimcv_sph.py:711-719).
 __device__ __forceinline__ void vv_finish(dimcv_state& st, const dimcv_params& p, int64_t i, v3 near, v3 vs,
                                           int added) {
+  if (step_failed(st)) return;
   if (added == 0) vs = mk(0.0f, 0.0f, 0.0f);
   else vs = vmuls(vs, p.vp_factor / (float)added);
   v3 vvort = vadd(vs, near);
@@ -1005,6 +1006,8 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
   DIMCV_LAUNCHED("vort2vel far field");
   return DIMCV_OK;
 }
+
+int32_t dimcv_far_split_count(int64_t targets) { return targets < 0 ? -1 : far_split_count_mc(targets); }
 
 int dimcv_vort2vel(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, int32_t max_samples,
                    void* stream) {

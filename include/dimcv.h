@@ -63,6 +63,10 @@ typedef enum dimcv_counter {
   DIMCV_C_JACOBI_DONE = 9,    /* native substep: current solve has converged */
   DIMCV_C_JACOBI_ITERS = 10,  /* native substep: iterations of the current solve */
   DIMCV_C_REDUCE_TICKET = 11, /* native substep: groups done in the running FP64 reduction (0 between calls) */
+  DIMCV_C_STEP_ERROR = 12,    /* native substep: nonzero when the grid or list build failed (1 solid key past
+                                 the table, 2 list width over nbr_kmax, 4 no fluid); the substep's writers of
+                                 x, v, is_sample and life_time then return, so a failing step leaves them as
+                                 the build left them.  Cleared by every grid build. */
   DIMCV_C_COUNT = 16
 } dimcv_counter;
 
@@ -369,6 +373,11 @@ int dimcv_vort2vel(const dimcv_state* st, const dimcv_params* p, int32_t fluid_b
 int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream);
 int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        int32_t max_samples, void* stream);
+/* Number of source splits the fast/Kahan far field uses for `targets` fluid
+ * particles, in the single-scene and the batched steps alike (host query, no
+ * device work; -1 for a negative count).  Build-defined: a launch-shape
+ * policy, no reference counterpart; splits * targets <= 2^21 always. */
+int32_t dimcv_far_split_count(int64_t targets);
 int dimcv_update_particle_life_time(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream); /* :753-760 */
 
 /* ---- native substep ---------------------------------------------------------- */

@@ -973,48 +973,58 @@ int32_t or_cdf_sample(Sim* s, const double* u, int32_t M) {
   return 0;
 }
 
-// dimcv_sph.py:649-658,679-719 in reference order.  v_vort_out (optional,
-// 3N) receives near+far for every processed particle (analysis only).
-void or_vort2vel(Sim* s, float* v_vort_out) {
+// One target of vort2vel (dimcv_sph.py:649-658,679-711), in reference order:
+// the near part (scaled), the far part (scaled by fac/added, zero if no
+// sample was accepted) and the accepted-sample count.
+static void vort2vel_one(const Sim* s, int64_t p_i, V3& near, V3& far, int& added) {
   const float fac = s->f_vp_factor;
   const float eps = s->f_eps, inv_eps = s->f_inv_eps;
+  V3 xi = s->X(p_i);
+  V3 vv{0.0f, 0.0f, 0.0f};
+  float kns = s->kernel_density[p_i];
+  s->for_all_neighbors(p_i, [&](int64_t p_j) {
+    if (s->is_fluid(p_j)) {
+      V3 r2 = s->rel(xi, s->X(p_j));
+      V3 r{r2.x, r2.y, 0.0f};
+      float rn = vnorm(r);
+      float A = (1.0f - expf_c(-rn * inv_eps)) / (rn * (rn + eps * expf_c(-rn * inv_eps)));
+      vv = vadd(vv, smul(A, vcross(r, Sim::get(s->d_vort_smoothed, p_j))));
+      kns += s->kernel_density[p_j];
+    }
+  });
+  near = vmuls(vv, fac);
+  V3 vs{0.0f, 0.0f, 0.0f};
+  added = 0;
+  float S = (float)(s->kden_sum - (double)kns);
+  for (int32_t j = 0; j < s->num_samples; ++j) {
+    int32_t p_j = s->sample_idx[j];
+    V3 r2 = s->rel(xi, s->X(p_j));
+    float rn = vnorm(r2);
+    if (rn > s->f_support) {
+      V3 r{r2.x, r2.y, 0.0f};
+      float A = (1.0f - expf_c(-rn * inv_eps)) / (rn * (rn + eps * expf_c(-rn * inv_eps)));
+      V3 t = smul(A, vcross(r, Sim::get(s->d_vort_smoothed, p_j)));
+      t = vmuls(t, S);
+      t = vdivs(t, s->kernel_density[p_j] + 1e-8f);
+      vs = vadd(vs, t);
+      added += 1;
+    }
+  }
+  if (added == 0) vs = {0.0f, 0.0f, 0.0f};
+  else vs = vmuls(vs, fac / (float)added);
+  far = vs;
+}
+
+// dimcv_sph.py:679-719.  v_vort_out (optional, 3N) receives near+far for
+// every processed particle (analysis only).
+void or_vort2vel(Sim* s, float* v_vort_out) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t p_i = 0; p_i < s->particle_num; ++p_i) {
     if (!s->is_fluid(p_i) || !s->in_vortex_domain(s->X(p_i))) continue;
-    V3 xi = s->X(p_i);
-    V3 vv{0.0f, 0.0f, 0.0f};
-    float kns = s->kernel_density[p_i];
-    s->for_all_neighbors(p_i, [&](int64_t p_j) {
-      if (s->is_fluid(p_j)) {
-        V3 r2 = s->rel(xi, s->X(p_j));
-        V3 r{r2.x, r2.y, 0.0f};
-        float rn = vnorm(r);
-        float A = (1.0f - expf_c(-rn * inv_eps)) / (rn * (rn + eps * expf_c(-rn * inv_eps)));
-        vv = vadd(vv, smul(A, vcross(r, Sim::get(s->d_vort_smoothed, p_j))));
-        kns += s->kernel_density[p_j];
-      }
-    });
-    vv = vmuls(vv, fac);
-    V3 vs{0.0f, 0.0f, 0.0f};
-    int added = 0;
-    float S = (float)(s->kden_sum - (double)kns);
-    for (int32_t j = 0; j < s->num_samples; ++j) {
-      int32_t p_j = s->sample_idx[j];
-      V3 r2 = s->rel(xi, s->X(p_j));
-      float rn = vnorm(r2);
-      if (rn > s->f_support) {
-        V3 r{r2.x, r2.y, 0.0f};
-        float A = (1.0f - expf_c(-rn * inv_eps)) / (rn * (rn + eps * expf_c(-rn * inv_eps)));
-        V3 t = smul(A, vcross(r, Sim::get(s->d_vort_smoothed, p_j)));
-        t = vmuls(t, S);
-        t = vdivs(t, s->kernel_density[p_j] + 1e-8f);
-        vs = vadd(vs, t);
-        added += 1;
-      }
-    }
-    if (added == 0) vs = {0.0f, 0.0f, 0.0f};
-    else vs = vmuls(vs, fac / (float)added);
-    V3 vvort = vadd(vs, vv);
+    V3 near, far;
+    int added;
+    vort2vel_one(s, p_i, near, far, added);
+    V3 vvort = vadd(far, near);
     if (v_vort_out) {
       v_vort_out[3 * p_i] = vvort.x;
       v_vort_out[3 * p_i + 1] = vvort.y;
@@ -1022,6 +1032,25 @@ void or_vort2vel(Sim* s, float* v_vort_out) {
     }
     V3 vi = s->Vv(p_i);
     if (vnorm(vvort) < vnorm(vi)) Sim::put(s->v, p_i, vadd(vi, vvort));
+  }
+}
+
+// vort2vel's recovered velocity at selected slots without updating v
+// (analysis): v_vort (near + far), far part and accepted count per target;
+// slots that vort2vel skips (solid, outside the vortex domain) get zeros.
+void or_vort2vel_targets(Sim* s, const int64_t* slots, int64_t n, float* v_vort_out, float* far_out,
+                         int32_t* added_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t p_i = slots[k];
+    V3 near{0.0f, 0.0f, 0.0f}, far{0.0f, 0.0f, 0.0f};
+    int added = 0;
+    if (p_i >= 0 && p_i < s->particle_num && s->is_fluid(p_i) && s->in_vortex_domain(s->X(p_i)))
+      vort2vel_one(s, p_i, near, far, added);
+    V3 vv = vadd(far, near);
+    v_vort_out[3 * k] = vv.x; v_vort_out[3 * k + 1] = vv.y; v_vort_out[3 * k + 2] = vv.z;
+    far_out[3 * k] = far.x; far_out[3 * k + 1] = far.y; far_out[3 * k + 2] = far.z;
+    added_out[k] = added;
   }
 }
 

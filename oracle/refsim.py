@@ -74,6 +74,7 @@ def lib():
         "or_compute_kinematic_vorticity_num_sample": (None, [P, P, P, I32]),
         "or_cdf_sample": (I32, [P, P, I32]),
         "or_vort2vel": (None, [P, P]),
+        "or_vort2vel_targets": (None, [P, P, I64, P, P, P]),
         "or_direct_bs_f64": (None, [P, P]),
         "or_ring_velocity": (None, [I32, P, P, P, P, F64, I32, I64, P, P]),
         "or_canon_reduce": (F64, [P, I64]),
@@ -248,6 +249,9 @@ class RefSim:
         L.or_init_cylinder(self.sim, 2, np.float32(0.65), np.float32(0.5), np.float32(0.1))
         self.last_kv = 0
         self.last_kp = 0
+        # dimcv_sph.py:21-26 (max_error_V, max_error); tests may tighten them
+        self.max_error_V = 0.1
+        self.max_error = 0.05
 
     # -- field access ------------------------------------------------------
     def field(self, name) -> np.ndarray:
@@ -379,7 +383,7 @@ class RefSim:
             L.or_divergence_solver_iteration_kernel(s)
             L.or_compute_density_change(s)
             err = L.or_compute_density_error(s, np.float32(0.0)) / self.fluid_particle_num
-            eta = self.inv_dt * 0.1 * 0.01 * self.density_0
+            eta = self.inv_dt * self.max_error_V * 0.01 * self.density_0
             it += 1
             if err <= eta:
                 break
@@ -395,7 +399,7 @@ class RefSim:
             L.or_pressure_solve_iteration_kernel(s)
             L.or_compute_density_adv(s)
             err = L.or_compute_density_error(s, np.float32(self.density_0)) / self.fluid_particle_num
-            eta = 0.05 * 0.01 * self.density_0
+            eta = self.max_error * 0.01 * self.density_0
             it += 1
             if err <= eta:
                 break
@@ -435,6 +439,16 @@ class RefSim:
         k = lib().or_neighbors(self.sim, int(p_i), _ptr(buf), buf.size)
         assert k <= buf.size
         return buf[:k].astype(np.int64)
+
+    def vort2vel_targets(self, slots):
+        """(v_vort, far part, accepted count) of vort2vel at `slots`, v unchanged."""
+        slots = np.ascontiguousarray(slots, np.int64)
+        n = slots.size
+        vv = np.zeros((n, 3), np.float32)
+        far = np.zeros((n, 3), np.float32)
+        added = np.zeros(n, np.int32)
+        lib().or_vort2vel_targets(self.sim, _ptr(slots), n, _ptr(vv), _ptr(far), _ptr(added))
+        return vv, far, added
 
     def direct_bs_f64(self) -> np.ndarray:
         out = np.zeros((self.particle_max_num, 3), np.float64)

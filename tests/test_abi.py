@@ -90,3 +90,21 @@ def test_ctypes_structs_match_c_layout():
         tname, field = key.split(".")
         cls = {"dimcv_params": N.Params, "dimcv_state": N.State, "dimcv_scene": N.Scene}[tname]
         assert getattr(cls, field).offset == off, key
+
+
+def test_far_split_policy_fits_the_partial_capacity():
+    """The MC far field's source-split count (shared by the single-scene and
+    batched steps) never asks for more partials than either reserves:
+    splits * targets <= min(16 targets, 2^21) and 1 <= splits <= 16."""
+    import _native as N
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("libdimcv.so not built")
+    L = N.lib()
+    ts = list(range(1, 5000, 37)) + list(range((1 << 19) - 300, (1 << 21) + 300, 997)) + \
+        [525_285, 700_000, 1 << 19, 1 << 20, 1 << 21, (1 << 21) + 1, 3_000_000, 10_000_000]
+    for t in sorted(ts):
+        s = L.dimcv_far_split_count(t)
+        assert 1 <= s <= 16, (t, s)
+        assert s == 1 or s * t <= min(16 * t, 1 << 21), (t, s)
+    assert L.dimcv_far_split_count(1 << 20) == 2        # the two halves at 1M targets
+    assert L.dimcv_far_split_count(-1) == -1
