@@ -242,13 +242,13 @@ def test_list_width_overflow_fails_before_writing_state(kind):
     from particle_system import ParticleSystem
     import scenes
     if kind == "fixed-m-step":
-        info = scenes.make_ring_scene(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="fast")
-        scene = info["scene"]
-        scene["Configuration"]["neighborListWidth"] = 16     # the lattice has ~26 neighbours
-        ps = ParticleSystem(SimConfig(scene=scene))
-        scenes.add_ring_particles(ps, info["pos"], info["vel"])
-        solver = ps.build_solver()
-        solver.initialize()
+        ps, solver, info = scenes.build_ring_system(12, 12, 10, n_rings=2, seed=3, mc_samples=128,
+                                                    far_field="fast")
+        solver.step(u_cdf=np.random.default_rng(1).random(128))
+        # lists of the allocated width, read only up to 16 entries: the
+        # lattice has ~26 neighbours per particle
+        ps.nbr_kmax = 16
+        ps.params.nbr_kmax = 16
     else:
         scene = copy.deepcopy(H.load_scene("mini_karman.json"))
         scene["Configuration"]["neighborListWidth"] = 8
