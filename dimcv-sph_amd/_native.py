@@ -174,6 +174,7 @@ def lib():
         "dimcv_vort2vel_near": (I32, [P, P, I32, P]),
         "dimcv_vort2vel_far": (I32, [P, P, I32, I32, P]),
         "dimcv_far_split_count": (I32, [I64]),
+        "dimcv_cdf_sample_noalloc": (I32, [P, P, P, I32, I32, P]),
         "dimcv_direct_bs": (I32, [P, P, I32, P, I32, P]),
         "dimcv_ring_velocity": (I32, [I64, P, F64, I64, P, P, P]),
         "dimcv_substep": (I32, [P, P, P, I32, I32, P]),

@@ -15,6 +15,8 @@
 //   KAHAN: FAST with Kahan-compensated f32 accumulators.
 // Samples are read with wave-uniform addresses (scalar loads): every lane of a
 // wave walks the same sample at the same time; no LDS staging is needed.
+#include <mutex>
+
 #include "common.h"
 
 namespace dimcv {
@@ -520,9 +522,10 @@ __global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, dou
 }
 
 __global__ void __launch_bounds__(256) k_scan_blocks(const double* __restrict__ x, double* __restrict__ L,
-                                                     const int32_t* __restrict__ nptr, double* __restrict__ btot) {
+                                                     const int32_t* __restrict__ nptr, int64_t cap,
+                                                     double* __restrict__ btot) {
   __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
-  const int64_t n = *nptr;
+  const int64_t n = min((int64_t)*nptr, cap);
   const int64_t s = (int64_t)blockIdx.x * DIMCV_SCAN_TILE;
   if (s >= n) return;  // uniform per block
   const int64_t m = min((int64_t)DIMCV_SCAN_TILE, n - s);
@@ -533,17 +536,17 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const double* __restrict__ 
 // Scan of the block totals (at most DIMCV_MAX_SCAN_BLOCKS of them: one block-
 // local scan), then out = P[b-1] + L for b > 0.
 __global__ void __launch_bounds__(256) k_scan_totals(const double* __restrict__ btot, double* __restrict__ bpre,
-                                                     const int32_t* __restrict__ nptr) {
+                                                     const int32_t* __restrict__ nptr, int64_t cap) {
   __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
-  const int64_t n = *nptr;
+  const int64_t n = min((int64_t)*nptr, cap);
   const int64_t nb = (n + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
   if (nb <= 0) return;
   block_local_scan_f64(btot, bpre, nb, tt, l1s, L2s, L1s);
 }
 
 __global__ void k_scan_add(double* __restrict__ L, const double* __restrict__ bpre, const int32_t* __restrict__ nptr,
-                           double* __restrict__ total) {
-  const int64_t n = *nptr;
+                           int64_t cap, double* __restrict__ total) {
+  const int64_t n = min((int64_t)*nptr, cap);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t b = i / DIMCV_SCAN_TILE;
@@ -551,10 +554,10 @@ __global__ void k_scan_add(double* __restrict__ L, const double* __restrict__ bp
   if (i == n - 1) *total = L[i];
 }
 
-__global__ void k_cdf_draw(dimcv_state st, const double* __restrict__ cdf, const double* __restrict__ u, int32_t m,
-                           int64_t zero_cap) {
+__global__ void k_cdf_draw(dimcv_state st, const double* __restrict__ cdf, int64_t cdf_cap,
+                           const double* __restrict__ u, int32_t m, int64_t zero_cap) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t F = st.counters[DIMCV_C_FLUID_LIST];
+  const int64_t F = min((int64_t)st.counters[DIMCV_C_FLUID_LIST], cdf_cap);
   const double total = F > 0 ? cdf[F - 1] : 0.0;
   const bool ok = F > 0 && total > 0.0;
   if (k == 0) st.counters[DIMCV_C_NUM_SAMPLES] = ok ? m : 0;
@@ -908,6 +911,82 @@ int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspac
   return DIMCV_OK;
 }
 
+// ---- inverse-CDF sampling (B3) ---------------------------------------------
+static int cdf_check(const dimcv_params* p, int32_t m, int32_t fluid_bound, const char* who) {
+  if (m < 0 || m > p->n_max) {
+    set_error("%s: %d samples outside [0, n_max]", who, m);
+    return DIMCV_ERR_RANGE;
+  }
+  if (fluid_bound < 0 || (int64_t)fluid_bound > (int64_t)DIMCV_SCAN_TILE * DIMCV_MAX_SCAN_BLOCKS ||
+      fluid_bound > p->n_max) {
+    set_error("%s: fluid bound %d outside [0, min(n_max, scan limit)]", who, fluid_bound);
+    return DIMCV_ERR_RANGE;
+  }
+  return DIMCV_OK;
+}
+
+// Weights, canonical FP64 scan and draws into caller-chosen scratch of at
+// least fluid_bound doubles (weights, cdf) and fluid_bound / 4096 + 16
+// (block totals, prefixes); every kernel reads at most fluid_bound entries.
+static int cdf_launch(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
+                      int32_t fluid_bound, double* weights, double* cdf, double* btot, double* bpre,
+                      hipStream_t s) {
+  const int32_t* nF = st->counters + DIMCV_C_FLUID_LIST;
+  const int64_t cap = fluid_bound;
+  if (fluid_bound > 0) {
+    hipLaunchKernelGGL(k_cdf_weights, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, weights);
+    unsigned nb = nblocks(fluid_bound, DIMCV_SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, weights, cdf, nF, cap, btot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, s, btot, bpre, nF, cap);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(fluid_bound)), dim3(256), 0, s, cdf, bpre, nF, cap,
+                       st->dscal + DIMCV_D_CDF_TOTAL);
+    DIMCV_LAUNCHED("canonical scan");
+  }
+  const int64_t zero_cap = p->n_max;
+  hipLaunchKernelGGL(k_cdf_draw, dim3(nblocks(zero_cap)), dim3(256), 0, s, *st, cdf, cap, u, m, zero_cap);
+  DIMCV_LAUNCHED("k_cdf_draw");
+  return DIMCV_OK;
+}
+
+// Scratch of dimcv_cdf_sample_noalloc, in place of a workspace: FP64
+// weights and CDF for up to kScratchFluid fluid particles plus the scan's
+// block totals and prefixes (16 MiB and 4 KiB).  Every GPU the library runs on holds
+// one copy, at its own address, until the process ends; host code uses the
+// address hipGetSymbolAddress returns for the current device (the array's
+// own name is not a device address on the host).  All calls on one GPU share
+// it: calls that overlap on one GPU race.  Nothing in it needs clearing: each
+// call writes every entry it reads.
+constexpr int64_t kScratchFluid = DIMCV_NOALLOC_MAX_FLUID;
+constexpr int64_t kScratchBlocks = kScratchFluid / DIMCV_SCAN_TILE + 16;
+constexpr size_t kScratchBytes = 8 * (size_t)(2 * kScratchFluid + 2 * kScratchBlocks);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch address on the current device, looked up once per device.
+static int scratch_base(unsigned char** out) {
+  int dev = -1;
+  DIMCV_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("dimcv_cdf_sample_noalloc: device %d beyond the %d the library tracks", dev, kMaxDevices);
+    return DIMCV_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* q = nullptr;
+    hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || q == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+      return DIMCV_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (unsigned char*)q;
+  }
+  *out = g_scratch_ptr[dev];
+  return DIMCV_OK;
+}
+
 }  // namespace dimcv
 
 using namespace dimcv;
@@ -916,33 +995,36 @@ extern "C" {
 
 int dimcv_cdf_sample(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
                      int32_t fluid_bound, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
   Workspace ws;
   int rc = bind_workspace(st, p, &ws, 0);
   if (rc) return rc;
-  if (m < 0 || m > p->n_max) {
-    set_error("dimcv_cdf_sample: %d samples outside [0, n_max]", m);
+  rc = cdf_check(p, m, fluid_bound, "dimcv_cdf_sample");
+  if (rc) return rc;
+  return cdf_launch(st, p, u, m, fluid_bound, ws.cdf_weights, ws.cdf, ws.block_tot, ws.block_pre,
+                    (hipStream_t)stream);
+}
+
+int dimcv_cdf_sample_noalloc(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
+                             int32_t fluid_bound, void* stream) {
+  if (!st || !p) {
+    set_error("dimcv_cdf_sample_noalloc: null state or params");
+    return DIMCV_ERR_ARG;
+  }
+  int rc = cdf_check(p, m, fluid_bound, "dimcv_cdf_sample_noalloc");
+  if (rc) return rc;
+  if ((int64_t)fluid_bound > kScratchFluid) {
+    set_error("dimcv_cdf_sample_noalloc: %d fluid particles exceed the scratch maximum %lld "
+              "(use dimcv_cdf_sample with a workspace)", fluid_bound, (long long)kScratchFluid);
     return DIMCV_ERR_RANGE;
   }
-  if ((int64_t)fluid_bound > (int64_t)DIMCV_SCAN_TILE * DIMCV_MAX_SCAN_BLOCKS) {
-    set_error("dimcv_cdf_sample: %d fluid particles exceed the scan limit", fluid_bound);
-    return DIMCV_ERR_RANGE;
-  }
-  const int32_t* nF = st->counters + DIMCV_C_FLUID_LIST;
-  if (fluid_bound > 0) {
-    hipLaunchKernelGGL(k_cdf_weights, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
-                       ws.cdf_weights);
-    unsigned nb = nblocks(fluid_bound, DIMCV_SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, ws.cdf_weights, ws.cdf, nF, ws.block_tot);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, s, ws.block_tot, ws.block_pre, nF);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(fluid_bound)), dim3(256), 0, s, ws.cdf, ws.block_pre, nF,
-                       st->dscal + DIMCV_D_CDF_TOTAL);
-    DIMCV_LAUNCHED("canonical scan");
-  }
-  int64_t cap = p->n_max;
-  hipLaunchKernelGGL(k_cdf_draw, dim3(nblocks(cap)), dim3(256), 0, s, *st, ws.cdf, u, m, cap);
-  DIMCV_LAUNCHED("k_cdf_draw");
-  return DIMCV_OK;
+  unsigned char* base = nullptr;
+  rc = scratch_base(&base);
+  if (rc) return rc;
+  double* weights = (double*)base;
+  double* cdf = weights + kScratchFluid;
+  double* btot = cdf + kScratchFluid;
+  double* bpre = btot + kScratchBlocks;
+  return cdf_launch(st, p, u, m, fluid_bound, weights, cdf, btot, bpre, (hipStream_t)stream);
 }
 
 int dimcv_vort2vel_near(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {

@@ -361,6 +361,17 @@ int dimcv_compute_kernel_density(const dimcv_state* st, const dimcv_params* p, i
  * indices by upper_bound.  Writes sample_idx[0..m) and num_samples. */
 int dimcv_cdf_sample(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
                      int32_t fluid_bound, void* stream);
+/* No counterpart in the reference: dimcv_cdf_sample for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments, kernels and results; st->workspace is
+ * not used (it may be NULL).  Its scratch (FP64 weights, CDF and scan block
+ * totals) is a fixed device array inside the library, one per GPU, held until
+ * the process ends; fluid_bound above DIMCV_NOALLOC_MAX_FLUID returns
+ * DIMCV_ERR_RANGE.  Allocates nothing.  Call with the stream's device
+ * current; calls that overlap on one GPU race (they share its scratch). */
+#define DIMCV_NOALLOC_MAX_FLUID (1 << 20)
+int dimcv_cdf_sample_noalloc(const dimcv_state* st, const dimcv_params* p, const double* u, int32_t m,
+                             int32_t fluid_bound, void* stream);
 /* vort2vel (:649-719): near field over the neighbour list, far field over the
  * sample set (p->far_mode), velocity gate |v_vort| < |v|.  max_samples is the
  * capacity of the packed sample table (pass n_max; num_samples never exceeds
