@@ -123,6 +123,12 @@ class AdjArgs(ctypes.Structure):
                 ("scalar", ctypes.c_float), ("mode", ctypes.c_int32)]
 
 
+class Emitter(ctypes.Structure):
+    """dimcv_emitter (include/dimcv.h)."""
+    _fields_ = [("pos", ctypes.c_void_p), ("n", ctypes.c_int32), ("object_id", ctypes.c_int32),
+                ("velocity", F3), ("density", ctypes.c_float), ("color", I3)]
+
+
 class Scene(ctypes.Structure):
     """dimcv_scene (include/dimcv.h)."""
     _fields_ = [("grid_num", I3), ("cell_base", ctypes.c_int32), ("n_cells", ctypes.c_int32),
@@ -161,6 +167,7 @@ def lib():
         "dimcv_workspace_bytes": (I64, [I64, I64, I64]),
         "dimcv_add_particles": (I32, [P, P, I32, I32, P, P, P, P, P, P, P, P, P]),
         "dimcv_init_cylinder": (I32, [P, P, I32, F32, F32, F32, P]),
+        "dimcv_emit": (I32, [P, P, P, I32, P]),
         "dimcv_grid_build": (I32, [P, P, I32, P, P]),
         "dimcv_neighbor_build": (I32, [P, P, I32, P]),
         "dimcv_compute_static_boundary_volume": (I32, [P, P, I32, P]),

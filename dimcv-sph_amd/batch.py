@@ -114,6 +114,9 @@ class SceneBatch:
         cfg["Configuration"]["domainEnd"] = [float(e) for e in ext]
         cfg["Configuration"]["reservedCapacity"] = total + 256 * S
         ps = ParticleSystem(SimConfig(scene=cfg), device=device)
+        # the batched activity kernel keeps no per-emitter counts
+        if ps.fluid_emitters:
+            raise ValueError("batched scenes have no fluid emitters")
         self.ps = ps
         dev = ps.device
 

@@ -17,7 +17,29 @@
 namespace dimcv {
 
 // ---------------------------------------------------------------------------
-// particle insertion (particle_system.py:316-380)
+// particle insertion (particle_system.py:316-380): add_particle into slot q
+__device__ __forceinline__ void add_one(dimcv_state& st, const dimcv_params& p, int64_t q, int32_t obj,
+                                        const float* pos3, const float* vel3, float dens, float pres, int32_t mat,
+                                        int32_t dyn, const int32_t* col3) {
+  st.object_id[q] = obj;
+  for (int d = 0; d < 3; ++d) {
+    st.x[3 * q + d] = pos3[d];
+    st.x_0[3 * q + d] = pos3[d];
+    st.v[3 * q + d] = vel3[d];
+    st.color[3 * q + d] = col3[d];
+    st.acceleration[3 * q + d] = 0.0f;
+  }
+  st.density[q] = dens;
+  st.m_V[q] = p.m_V0;
+  st.m[q] = p.m_V0 * dens;
+  st.pressure[q] = pres;
+  st.material[q] = mat;
+  st.is_dynamic[q] = dyn != 0 ? 1 : 0;
+  st.is_active[q] = 1;
+  st.is_sample[q] = 0;
+  st.life_time[q] = 0.0f;
+}
+
 __global__ void k_add_particles(dimcv_state st, dimcv_params p, int32_t obj, int32_t n,
                                 const float* __restrict__ pos, const float* __restrict__ vel,
                                 const float* __restrict__ dens, const float* __restrict__ pres,
@@ -28,24 +50,30 @@ __global__ void k_add_particles(dimcv_state st, dimcv_params p, int32_t obj, int
   int64_t pn = st.counters[DIMCV_C_PARTICLE_NUM];
   int64_t q = pn + k;
   if (q >= p.n_max) return;
-  st.object_id[q] = obj;
-  for (int d = 0; d < 3; ++d) {
-    st.x[3 * q + d] = pos[3 * k + d];
-    st.x_0[3 * q + d] = pos[3 * k + d];
-    st.v[3 * q + d] = vel[3 * k + d];
-    st.color[3 * q + d] = col[3 * k + d];
-    st.acceleration[3 * q + d] = 0.0f;
-  }
-  st.density[q] = dens[k];
-  st.m_V[q] = p.m_V0;
-  st.m[q] = p.m_V0 * dens[k];
-  st.pressure[q] = pres[k];
-  st.material[q] = mat[k];
-  st.is_dynamic[q] = dyn[k] != 0 ? 1 : 0;
-  st.is_active[q] = 1;
-  st.is_sample[q] = 0;
-  st.life_time[q] = 0.0f;
+  add_one(st, p, q, obj, pos + 3 * k, vel + 3 * k, dens[k], pres[k], mat[k], dyn[k], col + 3 * k);
   if (mat[k] == 1) atomicAdd(&st.counters[DIMCV_C_FLUID_NUM], 1);
+}
+
+// emit_particle (dimcv_sph.py:616-637) for one emitter: add_cube of its
+// lattice (kept on the device) at particle_num, velocity, density and colour
+// shared by the layer, fluid, dynamic, pressure 0.
+__global__ void k_emit(dimcv_state st, dimcv_params p, dimcv_emitter e) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.n) return;
+  const int64_t q = (int64_t)st.counters[DIMCV_C_PARTICLE_NUM] + k;
+  if (q >= p.n_max) return;  // dropped beyond capacity (particle_system.py:361)
+  add_one(st, p, q, e.object_id, e.pos + 3 * k, e.velocity, e.density, 0.0f, 1, 1, e.color);
+}
+
+// The layer's counts: particle_num and fluid_particle_num by the slots
+// written (:378-380, :330-331), the emitter's particleNum by the whole layer
+// (dimcv_sph.py:626-629 counts compute_cube_particle_num, drops included).
+__global__ void k_emit_finish(dimcv_state st, int64_t n_max, int32_t n, int32_t obj) {
+  const int64_t pn = st.counters[DIMCV_C_PARTICLE_NUM];
+  const int64_t got = pn + n < n_max ? pn + n : (pn < n_max ? n_max : pn);
+  st.counters[DIMCV_C_PARTICLE_NUM] = (int32_t)got;
+  st.counters[DIMCV_C_FLUID_NUM] += (int32_t)(got - pn);
+  if (obj >= 0 && obj < st.n_emitters) st.num_particles_each_emitter[obj] += n;
 }
 
 __global__ void k_add_finish(int32_t* counters, int64_t n_max, int32_t n, int32_t* dropped) {
@@ -573,6 +601,8 @@ __global__ void k_batch_keys(dimcv_state st, dimcv_params p, int64_t n_in, uint3
   if (mat == 0) {
     active = 1;
   } else if (mat == 1) {
+    // batched scenes have no emitters (SceneBatch rejects them), so unlike
+    // k_activity_keys no per-emitter count is decremented here
     const float q[3] = {xi.x, xi.y, xi.z};
     for (int j = 0; j < 3; ++j) {
       if (q[j] < sc.domain_start[j] || q[j] > sc.domain_end[j]) {
@@ -735,6 +765,26 @@ int dimcv_add_particles(const dimcv_state* st, const dimcv_params* p, int32_t ob
     DIMCV_HIP(hipMemcpyAsync(dropped, dev_drop, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DIMCV_HIP(hipStreamSynchronize(s));
   }
+  return DIMCV_OK;
+}
+
+int dimcv_emit(const dimcv_state* st, const dimcv_params* p, const dimcv_emitter* emitters, int32_t n_emitters,
+               void* stream) {
+  if (!st || !p || (n_emitters > 0 && !emitters) || n_emitters < 0) {
+    set_error("dimcv_emit: bad arguments");
+    return DIMCV_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (int32_t e = 0; e < n_emitters; ++e) {
+    const dimcv_emitter& em = emitters[e];
+    if (em.n < 0 || (em.n > 0 && !em.pos)) {
+      set_error("dimcv_emit: emitter %d has no lattice", e);
+      return DIMCV_ERR_ARG;
+    }
+    if (em.n > 0) hipLaunchKernelGGL(k_emit, dim3(nblocks(em.n)), dim3(256), 0, s, *st, *p, em);
+    hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, *st, p->n_max, em.n, em.object_id);
+  }
+  DIMCV_LAUNCHED("k_emit");
   return DIMCV_OK;
 }
 

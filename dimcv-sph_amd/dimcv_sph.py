@@ -78,6 +78,14 @@ class DIMCVSPHSolver(SPHBase):
             self.emit_interval = np.ceil(
                 ps.particle_diameter / (np.linalg.norm(init_v) * self.dt[None])).astype(int)
             break
+        # emission on the device (dimcv_emit): each emitter's layer, built
+        # once by the reference's add_cube lattice, stays on the device; an
+        # emission is one kernel per emitter with no host copy or sync, and
+        # the emitter counts live on the device.  emit_on_device = False
+        # restores the host path (add_cube per emission, counts mirrored).
+        self.emit_on_device = True
+        self._emitters = None
+        self._emitter_layers = []
 
         # build extensions
         self.mc_samples = ps.cfg.get_cfg("mcSamples")
@@ -216,8 +224,40 @@ class DIMCVSPHSolver(SPHBase):
         self.last_iterations_p = m_iterations
 
     # ------------------------------------------------------- emission --
+    def _emitter_table(self):
+        """dimcv_emitter entries (device lattices made on first use)."""
+        if self._emitters is None:
+            ps = self.ps
+            table = (N.Emitter * max(len(ps.fluid_emitters), 1))()
+            for k, emitter in enumerate(ps.fluid_emitters):
+                square_center = np.array(emitter["squareCenter"])
+                square_size = np.array(emitter["squareSize"])
+                init_v = np.array(emitter["velocity"])
+                lower = square_center - 0.5 * square_size
+                cube = square_size + ps.particle_radius * np.abs(init_v) / np.linalg.norm(init_v)
+                pos = ps.cube_lattice(lower, cube)
+                dev = torch.as_tensor(pos, device=ps.device).contiguous()
+                self._emitter_layers.append(dev)
+                e = table[k]
+                e.pos, e.n, e.object_id = dev.data_ptr(), pos.shape[0], int(emitter["objectId"])
+                for d in range(3):
+                    e.velocity[d] = np.float32(init_v[d])
+                    e.color[d] = int(emitter["color"][d])
+                e.density = np.float32(emitter["density"])
+            self._emitters = table
+        return self._emitters
+
     def emit_particle(self):
         ps = self.ps
+        if self.emit_on_device:
+            table = self._emitter_table()
+            ne = len(ps.fluid_emitters)
+            if ne == 0:
+                return
+            ps.call("dimcv_emit", table, ne, ps.stream())
+            for k in range(ne):
+                ps.appended(int(table[k].n))
+            return
         for emitter in ps.fluid_emitters:
             obj_id = emitter["objectId"]
             square_center = np.array(emitter["squareCenter"])
@@ -232,6 +272,8 @@ class DIMCVSPHSolver(SPHBase):
                         color=emitter["color"], material=1)
 
     def dump_num_particles_each_emitters_ti2np(self):
+        """Host mirror of the emitter counts (object_collection particleNum)
+        from the device (synchronises)."""
         if not self.ps.obj_id_emitters:
             return
         counts = self.ps.num_particles_each_emitter.to_numpy()

@@ -108,10 +108,15 @@ class KarmanVortexSolver(DIMCVSPHSolver):
                         f"{int(r)} {int(g)} {int(b)} {int(obj_id[i])} {int(ptype[i])}\n")
 
     def step(self, u1=None, u2=None, u_cdf=None):
-        self.dump_num_particles_each_emitters_ti2np()
-        if self.cnt % self.emit_interval == 0:
-            self.emit_particle()
-            self.dump_num_particles_each_emitters_np2ti()
+        if self.emit_on_device:
+            # emission and its counts stay on the device (dimcv_emit)
+            if self.cnt % self.emit_interval == 0:
+                self.emit_particle()
+        else:
+            self.dump_num_particles_each_emitters_ti2np()
+            if self.cnt % self.emit_interval == 0:
+                self.emit_particle()
+                self.dump_num_particles_each_emitters_np2ti()
         self.cnt += 1
         native = self.use_native_substep and type(self).substep is KarmanVortexSolver.substep
         if native and self.mc_samples and type(self).compute_moving_boundary_volume is \

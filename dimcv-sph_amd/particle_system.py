@@ -308,6 +308,18 @@ class ParticleSystem:
                   den.data_ptr(), pre.data_ptr(), mat.data_ptr(), dyn.data_ptr(), col.data_ptr(),
                   dropped.ctypes.data, self.stream())
         added = n - int(dropped[0])
+        self._note_appended(added)
+        return added
+
+    def appended(self, n: int) -> int:
+        """Host bookkeeping after n particles were appended on the device by
+        an asynchronous entry (dimcv_emit): the slots written are those below
+        n_max (the host count is exact, so no read-back is needed)."""
+        added = max(0, min(int(n), int(self.particle_max_num) - self._pn))
+        self._note_appended(added)
+        return added
+
+    def _note_appended(self, added: int) -> None:
         # build-owned identity: insertion index of each particle, carried
         # through every sort (particle_id)
         self.particle_id.t[self._pn:self._pn + added] = torch.arange(
@@ -315,20 +327,25 @@ class ParticleSystem:
         self._n_inserted += added
         self._pn += added
         self._nbr_valid = False
-        return added
 
     def compute_cube_particle_num(self, start, end):
         num_dim = [np.arange(start[i], end[i], self.particle_diameter) for i in range(self.dim)]
         return reduce(lambda a, b: a * b, [len(x) for x in num_dim])
 
+    def cube_lattice(self, lower_corner, cube_size) -> np.ndarray:
+        """The positions add_cube places (particle_system.py:658-672): numpy
+        arange per axis in f64, 'ij' meshgrid, f32, C order (n, 3)."""
+        num_dim = [np.arange(lower_corner[i], lower_corner[i] + cube_size[i], self.particle_diameter)
+                   for i in range(self.dim)]
+        pos = np.array(np.meshgrid(*num_dim, sparse=False, indexing="ij"), dtype=np.float32)
+        pos = pos.reshape(-1, reduce(lambda a, b: a * b, list(pos.shape[1:]))).transpose()
+        return np.ascontiguousarray(pos)
+
     def add_cube(self, object_id, lower_corner, cube_size, material, is_dynamic, color=(0, 0, 0),
                  density=None, pressure=None, velocity=None):
         """particle_system.py:647-696 (numpy lattice in 'ij' order, f64 arange -> f32)."""
-        num_dim = [np.arange(lower_corner[i], lower_corner[i] + cube_size[i], self.particle_diameter)
-                   for i in range(self.dim)]
-        n = reduce(lambda a, b: a * b, [len(x) for x in num_dim])
-        pos = np.array(np.meshgrid(*num_dim, sparse=False, indexing="ij"), dtype=np.float32)
-        pos = pos.reshape(-1, reduce(lambda a, b: a * b, list(pos.shape[1:]))).transpose()
+        pos = self.cube_lattice(lower_corner, cube_size)
+        n = pos.shape[0]
         if velocity is None:
             vel = np.full_like(pos, 0, dtype=np.float32)
         else:

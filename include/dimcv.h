@@ -279,6 +279,26 @@ int dimcv_add_particles(const dimcv_state* st, const dimcv_params* p, int32_t ob
                         const float* pos, const float* vel, const float* density,
                         const float* pressure, const int32_t* material, const int32_t* is_dynamic,
                         const int32_t* color, int32_t* dropped, void* stream);
+/* One fluid emitter (FluidEmitters entry of the scene JSON).  `pos` is the
+ * device copy of the layer add_cube builds for it (dimcv_sph.py:616-637:
+ * lower corner squareCenter - squareSize/2, size squareSize + r|v|/|v|, the
+ * numpy lattice of particle_system.py:647-696), the same at every emission. */
+typedef struct dimcv_emitter {
+  const float* pos;      /* device float[3 n] */
+  int32_t n;             /* compute_cube_particle_num of the layer */
+  int32_t object_id;
+  float velocity[3];     /* f32(velocity) */
+  float density;         /* f32(density) */
+  int32_t color[3];
+} dimcv_emitter;
+/* DIMCVSPHSolver.emit_particle (dimcv_sph.py:616-637) with the emitter counts
+ * of dump_num_particles_each_emitters_* (:639-647) kept on the device: each
+ * emitter's layer is appended at particle_num (dropped beyond n_max, as
+ * particle_system.py:361), particle_num and fluid_particle_num grow by the
+ * slots written, num_particles_each_emitter[object_id] by the whole layer.
+ * Asynchronous: no host copy, no synchronisation. */
+int dimcv_emit(const dimcv_state* st, const dimcv_params* p, const dimcv_emitter* emitters, int32_t n_emitters,
+               void* stream);
 /* KarmanVortexSolver.init_cylinder (karman_vortex.py:27-36). */
 int dimcv_init_cylinder(const dimcv_state* st, const dimcv_params* p, int32_t object_id, float cx,
                         float cy, float radius, void* stream);

@@ -265,6 +265,12 @@ class RefSim:
         p = lib().or_ptr(self.sim, name.encode())
         return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_int32)), (1,))
 
+    def emitter_field(self) -> np.ndarray:
+        """ps.num_particles_each_emitter (one entry per emitter, by object id)."""
+        p = lib().or_ptr(self.sim, b"num_particles_each_emitter")
+        n = max(1, len(self.fluid_emitters))
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_int32)), (n,))
+
     def grid_particles_num(self) -> np.ndarray:
         p = lib().or_ptr(self.sim, b"grid_particles_num")
         return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_int32)), (self.ncells,))
@@ -344,7 +350,11 @@ class RefSim:
             init_v = np.array(e["velocity"])
             lower = center - 0.5 * size
             cube = size + self.particle_radius * np.abs(init_v) / np.linalg.norm(init_v)
-            self.emitter_counts[obj_id] += self.compute_cube_particle_num(lower, lower + cube)
+            n_layer = self.compute_cube_particle_num(lower, lower + cube)
+            self.emitter_counts[obj_id] += n_layer
+            # dump_num_particles_each_emitters_ti2np / np2ti around the
+            # emission (karman_vortex.py:217-220): the field gains the layer
+            self.emitter_field()[obj_id] += n_layer
             self.add_cube(obj_id, lower, cube, material=1, is_dynamic=1, color=e["color"],
                           density=e["density"], velocity=init_v)
 

@@ -91,3 +91,44 @@ def test_graph_steps_equal_eager_steps_and_autotune_keeps_state():
             assert np.array_equal(eager.scene_samples(s), graph.scene_samples(s))
         assert np.array_equal(eager.last_iterations, graph.last_iterations)
     graph.release_graph()
+
+
+def test_batched_particle_on_upper_face_keys_like_solo():
+    """A particle exactly on a scene's upper z face keys into the next row's
+    first cell as in a solo run (the reference's flat index), and one on the
+    upper x face (flat index past the scene's table) leaves the scene; all
+    fields stay bitwise equal to the solo runs."""
+    import _native as N
+    from batch import SceneBatch, solo_system
+    specs = _specs()
+    batch = SceneBatch(specs, far_field="exact")
+    solos = [solo_system(sp, far_field="exact") for sp in specs]
+    bx = batch.ps.x.to_numpy()
+    lost = {}
+    for s, axis in ((0, 2), (2, 0)):
+        ps, sv = solos[s]
+        k = ps.particle_num[None]
+        x = ps.x.to_numpy()
+        start = int(batch.scene_counters_host()[s][N.SC_SLOT_START])
+        assert np.array_equal(bx[start:start + k], x[:k])      # same slots, same order
+        end = np.float32(ps.params.domain_end[axis])
+        if axis == 2:   # a particle of the first x column: its next-row cell is in the table
+            cand = np.nonzero(x[:k, 0] < np.float32(0.04))[0]
+            j = int(cand[np.argmax(x[cand, 2])])
+        else:           # the last x layer: flat index past the table
+            j = int(np.argmax(x[:k, 0]))
+        x[j, axis] = end
+        ps.x.from_numpy(x)
+        bx[start + j, axis] = end
+        lost[s] = 1 if axis == 0 else 0
+    batch.ps.x.from_numpy(bx)
+    n0 = [ps.particle_num[None] for ps, _ in solos]
+    for step in range(2):
+        batch.step()
+        for ps, sv in solos:
+            sv.step()
+        for s, (ps, sv) in enumerate(solos):
+            _compare_scene(batch, s, ps, sv)
+        if step == 0:   # the first build kept the z-face particle, dropped the x-face one
+            for s, d in lost.items():
+                assert solos[s][0].particle_num[None] == n0[s] - d
