@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
 SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip",
-           "csrc/batch.hip", "csrc/adjoint.hip"]
+           "csrc/batch.hip", "csrc/adjoint.hip", "csrc/export.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -168,6 +168,9 @@ def lib():
         "dimcv_add_particles": (I32, [P, P, I32, I32, P, P, P, P, P, P, P, P, P]),
         "dimcv_init_cylinder": (I32, [P, P, I32, F32, F32, F32, P]),
         "dimcv_emit": (I32, [P, P, P, I32, P]),
+        "dimcv_export_ply_size": (I32, [P, P, I32, P, I32, P, P, P]),
+        "dimcv_format_f7": (I32, [F32, ctypes.c_char_p]),
+        "dimcv_export_ply_rows": (I32, [P, P, I32, P, I32, P, I64, P]),
         "dimcv_grid_build": (I32, [P, P, I32, P, P]),
         "dimcv_neighbor_build": (I32, [P, P, I32, P]),
         "dimcv_compute_static_boundary_volume": (I32, [P, P, I32, P]),

@@ -13,6 +13,7 @@ sample update; ``u_cdf``: f64 array of M uniforms for inverse-CDF sampling).
 from __future__ import annotations
 
 import numpy as np
+import torch
 
 from dimcv_sph import DIMCVSPHSolver
 
@@ -36,18 +37,22 @@ class KarmanVortexSolver(DIMCVSPHSolver):
         self.ps._nbr_valid = False
 
     def export_png(self, cnt, image_path):
+        """karman_vortex.py:38-120: the fluid (coloured by vorticity_vis z) and
+        the cylinder (object 2) as a 3-D scatter.  The rows are selected on
+        the device; only the selected positions and values are copied."""
         import matplotlib
         matplotlib.use("Agg")
         import matplotlib.pyplot as plt
         from matplotlib.colors import Normalize
         N = self.ps.particle_num[None]
-        material = self.ps.material.to_numpy()[:N]
-        obj_id = self.ps.object_id.to_numpy()[:N]
+        material = self.ps.material.t[:N]
+        obj_id = self.ps.object_id.t[:N]
         fluid_mask = material == self.ps.material_fluid
         solid_mask = obj_id == 2
-        x = self.x_temp.to_numpy()[:N]
-        vort = self.ps.vorticity_vis.to_numpy()[:N][:, 2]
-        fluid_x, fluid_vort, solid_x = x[fluid_mask], vort[fluid_mask], x[solid_mask]
+        x = self.x_temp.t[:N]
+        fluid_x = x[fluid_mask].cpu().numpy()
+        fluid_vort = self.ps.vorticity_vis.t[:N, 2][fluid_mask].cpu().numpy()
+        solid_x = x[solid_mask].cpu().numpy()
         fig = plt.figure(figsize=(10, 4), dpi=200)
         ax = fig.add_subplot(111, projection="3d")
         ax.view_init(elev=30, azim=-60)
@@ -65,47 +70,48 @@ class KarmanVortexSolver(DIMCVSPHSolver):
         plt.cla()
         plt.close("all")
 
+    def _particle_type_table(self):
+        """Object id -> particle_type code (karman_vortex.py:163-176): 4, then
+        fluid block 0, emitter 1, rigid block 2, rigid body 3, later ones
+        overriding earlier ones as the reference's mask assignments do."""
+        ps = self.ps
+        groups = (ps.obj_id_fluid_blocks, ps.obj_id_emitters, ps.obj_id_rigid_blocks, ps.object_id_rigid_body)
+        ids = [int(o) for g in groups for o in g]
+        n = max(ids) + 1 if ids else 0
+        table = np.full(max(n, 1), 4, np.int32)
+        for code, g in enumerate(groups):
+            for o in g:
+                if int(o) >= 0:
+                    table[int(o)] = code
+        return torch.as_tensor(table, device=ps.device), n
+
     def export_ply(self, cnt, ply_path):
-        """ASCII PLY with the reference schema (karman_vortex.py:122-214)."""
-        N = self.ps.particle_num[None]
+        """ASCII PLY with the reference schema (karman_vortex.py:122-214).  The
+        data rows are formatted on the device (dimcv_export_ply_rows) from the
+        particle buffers; only the finished text is copied to the host."""
+        ps = self.ps
+        N = ps.particle_num[None]
         if N == 0:
             return
-        x = self.x_temp.to_numpy()[:N]
-        vorticity = self.ps.vorticity.to_numpy()[:N]
-        color = self.ps.color.to_numpy()[:N]
-        obj_id = self.ps.object_id.to_numpy()[:N]
-        is_active = self.ps.is_active.to_numpy()[:N].astype(bool)
-        x, vorticity, color, obj_id = x[is_active], vorticity[is_active], color[is_active], obj_id[is_active]
-        if x.shape[0] == 0:
+        table, n_types = self._particle_type_table()
+        rows = np.zeros(1, np.int64)
+        nbytes = np.zeros(1, np.int64)
+        ps.call("dimcv_export_ply_size", int(N), table.data_ptr(), n_types, rows.ctypes.data,
+                nbytes.ctypes.data, ps.stream())
+        if rows[0] == 0:
             return
-        nv = x.shape[0]
-        ptype = np.full(nv, 4, dtype=np.uint8)
-
-        def mask(ids):
-            ids = list(ids)
-            if not ids:
-                return np.zeros_like(obj_id, dtype=bool)
-            return np.isin(obj_id, np.array(ids, dtype=obj_id.dtype))
-
-        ptype[mask(self.ps.obj_id_fluid_blocks)] = 0
-        ptype[mask(self.ps.obj_id_emitters)] = 1
-        ptype[mask(self.ps.obj_id_rigid_blocks)] = 2
-        ptype[mask(self.ps.object_id_rigid_body)] = 3
+        out = torch.empty(int(nbytes[0]), dtype=torch.uint8, device=ps.device)
+        ps.call("dimcv_export_ply_rows", int(N), table.data_ptr(), n_types, out.data_ptr(), int(nbytes[0]),
+                ps.stream())
+        body = out.cpu().numpy().tobytes()
         ply_path.mkdir(parents=True, exist_ok=True)
-        with open(ply_path / f"frame_{cnt:04}.ply", "w") as f:
-            f.write("ply\nformat ascii 1.0\n")
-            f.write(f"element vertex {nv}\n")
-            for nm in ("x", "y", "z", "vort_x", "vort_y", "vort_z"):
-                f.write(f"property float {nm}\n")
-            for nm in ("red", "green", "blue"):
-                f.write(f"property uchar {nm}\n")
-            f.write("property int object_id\nproperty uchar particle_type\nend_header\n")
-            for i in range(nv):
-                px, py, pz = x[i]
-                vx, vy, vz = vorticity[i]
-                r, g, b = color[i]
-                f.write(f"{px:.7f} {py:.7f} {pz:.7f} {vx:.7f} {vy:.7f} {vz:.7f} "
-                        f"{int(r)} {int(g)} {int(b)} {int(obj_id[i])} {int(ptype[i])}\n")
+        header = ["ply", "format ascii 1.0", f"element vertex {int(rows[0])}"]
+        header += [f"property float {nm}" for nm in ("x", "y", "z", "vort_x", "vort_y", "vort_z")]
+        header += [f"property uchar {nm}" for nm in ("red", "green", "blue")]
+        header += ["property int object_id", "property uchar particle_type", "end_header"]
+        with open(ply_path / f"frame_{cnt:04}.ply", "wb") as f:
+            f.write(("\n".join(header) + "\n").encode())
+            f.write(body)
 
     def step(self, u1=None, u2=None, u_cdf=None):
         if self.emit_on_device:

@@ -299,6 +299,24 @@ typedef struct dimcv_emitter {
  * Asynchronous: no host copy, no synchronisation. */
 int dimcv_emit(const dimcv_state* st, const dimcv_params* p, const dimcv_emitter* emitters, int32_t n_emitters,
                void* stream);
+/* KarmanVortexSolver.export_ply (karman_vortex.py:122-214): the ASCII data
+ * rows of the PLY file, formatted on the device from x_temp, vorticity,
+ * color and object_id of the active slots below particle_num (slot order),
+ * each float as Python's "{:.7f}" of its float32 value, the particle type
+ * from obj_type[object_id] (device int32[n_obj_type]; 4 outside it).
+ * _size returns the row count and byte size (synchronises); _rows writes the
+ * rows into the device buffer `out` of out_bytes (at least that size).  The
+ * header is the caller's (the reference's fixed text with the row count). */
+int dimcv_export_ply_size(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
+                          const int32_t* obj_type, int32_t n_obj_type, int64_t* rows, int64_t* bytes,
+                          void* stream);
+int dimcv_export_ply_rows(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
+                          const int32_t* obj_type, int32_t n_obj_type, char* out, int64_t out_bytes,
+                          void* stream);
+/* The row formatter's float conversion, on the host (no device work): v as
+ * Python's "{:.7f}" writes float(v), NUL-terminated into out (at least 49
+ * bytes); returns the length.  For checking the device rows' digits. */
+int32_t dimcv_format_f7(float v, char* out);
 /* KarmanVortexSolver.init_cylinder (karman_vortex.py:27-36). */
 int dimcv_init_cylinder(const dimcv_state* st, const dimcv_params* p, int32_t object_id, float cx,
                         float cy, float radius, void* stream);

@@ -464,3 +464,85 @@ class RefSim:
         out = np.zeros((self.particle_max_num, 3), np.float64)
         lib().or_direct_bs_f64(self.sim, _ptr(out))
         return out
+
+
+def export_ply(ref: RefSim, cnt: int, ply_path) -> None:
+    """karman_vortex.py:122-214 restated on the oracle's fields: the ASCII PLY
+    of the active particles below particle_num, each row formatted with the
+    reference's own f-strings."""
+    N = ref.particle_num
+    if N == 0:
+        return
+    x = ref.field("x_temp")[:N]
+    vorticity = ref.field("vorticity")[:N]
+    color = ref.field("color")[:N]
+    obj_id = ref.field("object_id")[:N]
+    is_active = ref.field("is_active")[:N].astype(bool)
+    x, vorticity, color, obj_id = x[is_active], vorticity[is_active], color[is_active], obj_id[is_active]
+    if x.shape[0] == 0:
+        return
+    num_vertices = x.shape[0]
+    particle_type = np.full(num_vertices, 4, dtype=np.uint8)
+    sc = ref.scene
+
+    def mask_from_ids(ids_set):
+        ids = list(ids_set)
+        if len(ids) == 0:
+            return np.zeros_like(obj_id, dtype=bool)
+        return np.isin(obj_id, np.array(ids, dtype=obj_id.dtype))
+
+    particle_type[mask_from_ids({b["objectId"] for b in sc.get("FluidBlocks", [])})] = 0
+    particle_type[mask_from_ids({e["objectId"] for e in sc.get("FluidEmitters", [])})] = 1
+    particle_type[mask_from_ids({b["objectId"] for b in sc.get("RigidBlocks", [])})] = 2
+    particle_type[mask_from_ids({b["objectId"] for b in sc.get("RigidBodies", [])})] = 3
+    ply_path.mkdir(parents=True, exist_ok=True)
+    with open(ply_path / f"frame_{cnt:04}.ply", "w") as f:
+        f.write("ply\n")
+        f.write("format ascii 1.0\n")
+        f.write(f"element vertex {num_vertices}\n")
+        for nm in ("x", "y", "z", "vort_x", "vort_y", "vort_z"):
+            f.write(f"property float {nm}\n")
+        for nm in ("red", "green", "blue"):
+            f.write(f"property uchar {nm}\n")
+        f.write("property int object_id\n")
+        f.write("property uchar particle_type\n")
+        f.write("end_header\n")
+        for i in range(num_vertices):
+            px, py, pz = x[i]
+            vx, vy, vz = vorticity[i]
+            r, g, b = color[i]
+            f.write(f"{px:.7f} {py:.7f} {pz:.7f} "
+                    f"{vx:.7f} {vy:.7f} {vz:.7f} "
+                    f"{int(r)} {int(g)} {int(b)} {int(obj_id[i])} {int(particle_type[i])}\n")
+
+
+def export_png(ref: RefSim, cnt: int, image_path) -> None:
+    """karman_vortex.py:38-120 restated on the oracle's fields."""
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    from matplotlib.colors import Normalize
+    N = ref.particle_num
+    material = ref.field("material")[:N]
+    obj_id = ref.field("object_id")[:N]
+    fluid_mask = material == 1
+    solid_mask = obj_id == 2
+    x = ref.field("x_temp")[:N]
+    vort = ref.field("vorticity_vis")[:N][:, 2]
+    fluid_x, fluid_vort, solid_x = x[fluid_mask], vort[fluid_mask], x[solid_mask]
+    fig = plt.figure(figsize=(10, 4), dpi=200)
+    ax = fig.add_subplot(111, projection="3d")
+    ax.view_init(elev=30, azim=-60)
+    ax.scatter(fluid_x[:, 0], fluid_x[:, 1], fluid_x[:, 2], c=fluid_vort, cmap="coolwarm", s=0.2,
+               norm=Normalize(vmin=-40, vmax=40), edgecolors="none", alpha=0.6)
+    ax.scatter(solid_x[:, 0], solid_x[:, 1], solid_x[:, 2], color="#00C853", s=0.5, edgecolors="none")
+    ax.set_xlim(0, 4)
+    ax.set_ylim(0, 1)
+    ax.set_zlim(0, 1)
+    ax.set_box_aspect((4, 1, 1))
+    ax.set_axis_off()
+    plt.subplots_adjust(left=0, right=1, top=1, bottom=0)
+    plt.savefig(image_path / f"vorticity_{cnt:04}.png", bbox_inches="tight", pad_inches=0,
+                transparent=True, dpi=400)
+    plt.cla()
+    plt.close("all")
