@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
 SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip",
-           "csrc/batch.hip", "csrc/adjoint.hip", "csrc/export.hip"]
+           "csrc/batch.hip", "csrc/adjoint.hip", "csrc/export.hip",
+           "csrc/segment.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -74,6 +75,7 @@ class Params(ctypes.Structure):
         ("bounce_hi", F3), ("far_mode", ctypes.c_int32), ("m_V0", ctypes.c_float),
         ("nbr_r2_limit", ctypes.c_float),
         ("vp_factor_f64", ctypes.c_double),
+        ("dyn_rigid", ctypes.c_int32), ("inv_dt", ctypes.c_float),
     ]
 
 
@@ -129,6 +131,15 @@ class Emitter(ctypes.Structure):
                 ("velocity", F3), ("density", ctypes.c_float), ("color", I3)]
 
 
+class Segments(ctypes.Structure):
+    """dimcv_segments (include/dimcv.h)."""
+    _fields_ = [("x_minus", ctypes.c_void_p), ("x_plus", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
+                ("active", ctypes.c_void_p), ("age", ctypes.c_void_p), ("sources", ctypes.c_void_p),
+                ("n", ctypes.c_int32), ("reg_radius", ctypes.c_float), ("dt", ctypes.c_float),
+                ("dt6", ctypes.c_float), ("inv4pi", ctypes.c_float), ("gamma_decay", ctypes.c_float),
+                ("u_inf", F3)]
+
+
 class Scene(ctypes.Structure):
     """dimcv_scene (include/dimcv.h)."""
     _fields_ = [("grid_num", I3), ("cell_base", ctypes.c_int32), ("n_cells", ctypes.c_int32),
@@ -168,6 +179,8 @@ def lib():
         "dimcv_add_particles": (I32, [P, P, I32, I32, P, P, P, P, P, P, P, P, P]),
         "dimcv_init_cylinder": (I32, [P, P, I32, F32, F32, F32, P]),
         "dimcv_emit": (I32, [P, P, P, I32, P]),
+        "dimcv_segment_velocity": (I32, [P, I32, F32, P, P, P, P, P]),
+        "dimcv_segment_rk4": (I32, [P, P, P, P, P]),
         "dimcv_export_ply_size": (I32, [P, P, I32, P, I32, P, P, P]),
         "dimcv_format_f7": (I32, [F32, ctypes.c_char_p]),
         "dimcv_export_ply_rows": (I32, [P, P, I32, P, I32, P, I64, P]),

@@ -537,6 +537,10 @@ __device__ __forceinline__ int last_le(const int32_t* __restrict__ a, int n, int
 int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs* bw, void* base);
 int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw);
 
+// the pressure update's reaction on dynamic rigid particles (grid.hip); no
+// launch unless p->dyn_rigid
+int launch_rigid_reaction(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const int32_t* done,
+                          hipStream_t s);
 // host helpers implemented in util.hip
 int reduce_partials_f64(const double* partials, int64_t n, double* out, void* stream,
                         const int32_t* done = nullptr);

@@ -523,6 +523,57 @@ __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params 
   nbr_flush(st, p, stage[w], lane, f, cnt);
 }
 
+// dimcv_sph.py:536-549: the pressure update's reaction on a dynamic rigid
+// neighbour j of fluid particle i, acceleration[j] += -vel_change inv_dt
+// rho_i / rho_j with vel_change = -dt k_i m_V,j rho0 gradW(x_i - x_j) for
+// |k_i| > m_eps.  Gathered per rigid particle from its fluid neighbours in
+// ascending slot order, the order of the reference's serial adds, onto the
+// value the acceleration holds; no list holds cell 0, but a fluid particle in
+// cell 0 still visits j, so cell 0's slots (the lowest) are scanned first when
+// j's cell borders it.  A rigid particle in cell 0 is visited by nobody.
+__global__ void k_rigid_reaction(dimcv_state st, dimcv_params p, int32_t bound, const int32_t* done) {
+  if (jacobi_gated(st, done)) return;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= bound || j >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[j] != 0 || !st.is_dynamic[j]) return;
+  const v3 xj = ld3(st.x, j);
+  int cc[3];
+  cell_of(p, xj, cc);
+  const int nx = p.grid_num[0], ny = p.grid_num[1], nz = p.grid_num[2];
+  if (((int64_t)cc[0] * ny + cc[1]) * nz + cc[2] == 0) return;
+  const float mvj = st.m_V[j], rho_j = st.density[j];
+  v3 acc = ld3(st.acceleration, j);
+  auto visit = [&](int i) {
+    if (st.material[i] != 1) return;
+    const float k_i = (st.density_adv[i] - p.density0) * st.dfsph_factor[i];
+    if (fabsf(k_i) > p.m_eps) {
+      const v3 g = smul(mvj * p.density0, cubic_dW(p, rel(p, ld3(st.x, i), xj)));
+      const v3 vc = smul(-p.dt * 1.0f * k_i, g);
+      const float rho_i = st.density[i];
+      acc = vadd(acc, mk(((-vc.x) * p.inv_dt) * rho_i / rho_j, ((-vc.y) * p.inv_dt) * rho_i / rho_j,
+                         ((-vc.z) * p.inv_dt) * rho_i / rho_j));
+    }
+  };
+  bool near0 = true;
+  const int lim[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) near0 = near0 && (cc[d] <= 1 || (p.periodic && cc[d] == lim[d] - 1));
+  if (near0) {
+    const int32_t end0 = st.grid_particles_num[0];
+    for (int32_t i = 0; i < end0; ++i)
+      if (i != j && vnorm(rel(p, xj, ld3(st.x, i))) < p.support_radius) visit(i);
+  }
+  scan_cells(p, st.x, st.grid_particles_num, j, xj, visit);
+  st3(st.acceleration, j, acc);
+}
+
+int launch_rigid_reaction(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const int32_t* done,
+                          hipStream_t s) {
+  if (!p->dyn_rigid || particle_bound <= 0) return DIMCV_OK;
+  hipLaunchKernelGGL(k_rigid_reaction, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound, done);
+  DIMCV_LAUNCHED("k_rigid_reaction");
+  return DIMCV_OK;
+}
+
 // sph_base.py:91-112: Akinci boundary volume over solid neighbours.
 __global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound, int dynamic_sel) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -356,12 +356,16 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
 }
 
 // dimcv_sph.py:197-204 for non-fluid slots: static rigid -> 0, otherwise g.
+// One value and one store, chosen by selects: written as two stores under
+// if/else, the gfx950 build stored the static branch's zero for dynamic
+// rigid slots as well (the join overwrote the gravity registers).
 __global__ void k_nonpressure_other(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
-  if (st.material[s] == 1) return;
-  if (st.material[s] == 0 && !st.is_dynamic[s]) st3(st.acceleration, s, mk(0.0f, 0.0f, 0.0f));
-  else st3(st.acceleration, s, mk(p.gravity[0], p.gravity[1], p.gravity[2]));
+  const int32_t mat = st.material[s];
+  if (mat == 1) return;
+  const bool stat = mat == 0 && st.is_dynamic[s] == 0;
+  st3(st.acceleration, s, mk(stat ? 0.0f : p.gravity[0], stat ? 0.0f : p.gravity[1], stat ? 0.0f : p.gravity[2]));
 }
 
 // dimcv_sph.py:220-226
@@ -828,6 +832,8 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
     case 2:
       if (pk) hipLaunchKernelGGL(k_pressure_iter<true>, g, b, 0, s, *st, *p, fluid_bound, done, q);
       else hipLaunchKernelGGL(k_pressure_iter<false>, g, b, 0, s, *st, *p, fluid_bound, done, q);
+      // its reaction on dynamic rigid particles, before density_adv moves on
+      launch_rigid_reaction(st, p, (int32_t)p->n_max, done, s);
       break;
     default:
       if (pk) hipLaunchKernelGGL(k_density_adv<true>, g, b, 0, s, *st, *p, fluid_bound, done, partials, offset, q);
@@ -1020,7 +1026,7 @@ int dimcv_compute_density_adv(const dimcv_state* st, const dimcv_params* p, int3
 
 int dimcv_pressure_solve_iteration(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, void* stream) {
   LAUNCH_FLUID(k_pressure_iter<false>, *st, *p, fluid_bound, nullptr, Packed{});
-  return DIMCV_OK;
+  return dimcv::launch_rigid_reaction(st, p, (int32_t)p->n_max, nullptr, (hipStream_t)stream);
 }
 
 int dimcv_copy_x_temp(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {

@@ -405,6 +405,8 @@ extern "C" int dimcv_step_begin(const dimcv_state* pre, const dimcv_state* st, c
   return run_graph(SEG_HEAD, [&](hipStream_t hs) -> int {
     CHECK(dimcv_grid_build(pre, p, pb, nullptr, hs));
     CHECK(neighbor_build_launch(st, p, pb, hs));
+    // compute_moving_boundary_volume (karman_vortex.py:223) when the scene has dynamic rigid particles
+    if (p->dyn_rigid) CHECK(dimcv_compute_moving_boundary_volume(st, p, pb, hs));
     return substep_head(st, p, o, pb, pb, hs, ws, n_first_v);
   }, sp.key, s);
 }

@@ -105,6 +105,7 @@ class DIMCVSPHSolver(SPHBase):
         p = ps.params
         f32 = np.float32
         p.dt = f32(self.dt[None])
+        p.inv_dt = f32(self.inv_dt)
         if self.vortex_enforcing_domain_start is not None:
             p.has_vortex_domain = 1
             for d in range(3):

@@ -79,9 +79,10 @@ class ParticleSystem:
 
         rigid_blocks = self.cfg.get_rigid_blocks()
         rigid_particle_num = 0
+        # dynamic rigid blocks: moving boundary volumes every step and the
+        # pressure update's reactions on them (dimcv_sph.py:536-549)
+        self.has_dynamic_rigid = any(bool(r.get("isDynamic")) for r in rigid_blocks)
         for rigid in rigid_blocks:
-            if rigid.get("isDynamic"):
-                raise NotImplementedError("dynamic rigid blocks are not supported")
             n = self.compute_cube_particle_num(rigid["start"], rigid["end"])
             rigid["particleNum"] = n
             self.object_collection[rigid["objectId"]] = rigid
@@ -214,6 +215,7 @@ class ParticleSystem:
         p.inv_eps = f32(1.0 / 1e-6)
         p.bounce_padding = f32(self.padding)
         p.m_V0 = f32(self.m_V0)
+        p.dyn_rigid = 1 if self.has_dynamic_rigid else 0
         p.far_r2_max_rejected = float(far_threshold(f32(self.support_radius)))
         p.nbr_r2_limit = float(near_threshold(f32(self.support_radius)))
 

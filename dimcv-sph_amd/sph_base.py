@@ -36,6 +36,7 @@ class SPHBase:
 
     def _dt_changed(self, v):
         self.ps.params.dt = np.float32(v)
+        self.ps.params.inv_dt = np.float32(1.0 / float(np.float32(v)))
 
     # host evaluation of sph_base.py:23-68 (f32, same expression order)
     def cubic_kernel(self, r_norm):
@@ -76,9 +77,9 @@ class SPHBase:
         self.ps.call("dimcv_compute_static_boundary_volume", self.ps._pn, self.ps.stream())
 
     def compute_moving_boundary_volume(self):
-        # no dynamic rigid particle can exist (ParticleSystem rejects them), so
-        # the reference loop (sph_base.py:105-112) has no work
-        return None
+        # sph_base.py:105-112, over the dynamic rigid particles (none: no work)
+        if self.ps.has_dynamic_rigid:
+            self.ps.call("dimcv_compute_moving_boundary_volume", self.ps._pn, self.ps.stream())
 
     def substep(self):
         pass

@@ -127,6 +127,9 @@ typedef struct dimcv_params {
   float m_V0;                   /* f32(0.8 (2 r)^3)             particle_system.py:46 */
   float nbr_r2_limit;           /* smallest f32 t with sqrt_rn(t) >= support: |r| < h <=> r.r < t */
   double vp_factor_f64;         /* 0.1592 * volumeScale * m_V0 in FP64 (direct FP64 sum) */
+  int32_t dyn_rigid;            /* 1: the scene holds dynamic rigid particles (reaction writes,
+                                   moving boundary volumes each step) */
+  float inv_dt;                 /* f32(1 / dt), solver.inv_dt        dimcv_sph.py:28 */
 } dimcv_params;
 
 /* One scene of a batch (build-defined batching, dimcv_batch_*).  All scenes
@@ -508,6 +511,41 @@ int dimcv_direct_bs(const dimcv_state* st, const dimcv_params* p, int32_t precis
  * FP64 accumulation, written as f32. */
 int dimcv_ring_velocity(int64_t npts, const double* pts, double inv_sigma2, int64_t n,
                         const float* x, float* vel, void* stream);
+
+/* ---- segment solver: O(N^2) regularised Biot-Savart --------------------------- */
+/* The segment pool of SegmentSystem (segment_system.py:16-29) and the
+ * constants of SegmentSolver (segment_solver.py:9-20).  Device arrays hold
+ * segment_num = n entries (vectors packed float[3]); `sources` is scratch of at
+ * least 32 n bytes. */
+typedef struct dimcv_segments {
+  float* x_minus;
+  float* x_plus;
+  float* gamma;
+  const int32_t* active;   /* 1 = active */
+  float* age;
+  void* sources;           /* scratch: the packed source table */
+  int32_t n;               /* segment_num */
+  float reg_radius;        /* f32(regularizationRadiusR) */
+  float dt;                /* f32(timeStepSize) */
+  float dt6;               /* f32(dt / 6) */
+  float inv4pi;            /* f32(1 / (4 pi)) */
+  float gamma_decay;       /* f32(gammaDecay) */
+  float u_inf[3];          /* backgroundVelocity */
+} dimcv_segments;
+/* _compute_endpoint_velocity_bs (form 0, segment_solver.py:378-418: at the
+ * endpoints, vortex vector gamma L t) or _compute_velocity_at_factor (form 1,
+ * :468-521: at x + factor dt k, vortex vector gamma d).  Writes the two
+ * endpoint velocities of every active segment; inactive ones are left as
+ * they are.  Sums run over active j != i in ascending j, bitwise the serial
+ * reference arithmetic. */
+int dimcv_segment_velocity(const dimcv_segments* sg, int32_t form, float factor, const float* k_minus,
+                           const float* k_plus, float* out_minus, float* out_plus, void* stream);
+/* advect_segments_rk4 (segment_solver.py:420-447, 458-542): k1 = (v_minus,
+ * v_plus), k2..k4 by form 1 at factors 0.5, 0.5, 1, then the RK4 endpoint
+ * update, gamma *= gamma_decay, age += dt.  k is scratch of 24 n floats (the
+ * eight stage arrays). */
+int dimcv_segment_rk4(const dimcv_segments* sg, const float* v_minus, const float* v_plus, float* k,
+                      void* stream);
 
 /* ---- batched scenes (build-defined) ----------------------------------------- */
 /* Many independent scenes advance together in one set of launches.  Slots of

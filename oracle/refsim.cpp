@@ -21,8 +21,9 @@
 //    vort2vel uses S_i = f32(kden_sum - f64(kden_nbr)).
 //  * surface tension (sigma = 0, dimcv_sph.py:12,153-167) and boundary
 //    viscosity (= 0, :182-190) contribute exact zeros and are not evaluated.
-//  * dynamic rigid bodies are rejected by the Python layer (out of scope), so
-//    their neighbour reaction writes (:191-193, :465-468, :546-549) are absent.
+//  * the viscosity reaction on dynamic rigid neighbours (:191-193) is an exact
+//    zero (boundary viscosity 0) and is not evaluated; the Jacobi updates'
+//    reactions (:465-468, :546-549) are added serially in ascending p_i.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -171,7 +172,7 @@ struct Sim {
   float f_visc_coef, f_visc_h2, f_transit_a, f_transit_ba, f_vp_factor;
   float f_eps, f_inv_eps, f_neg_a, f_neg_b;
   float f_box[3];
-  float dt;
+  float dt, f_inv_dt;
   // scalar fields
   int32_t particle_num, fluid_particle_num, solid_particle_num, num_samples;
   double kden_sum;
@@ -196,6 +197,10 @@ struct Sim {
   V3 Vv(int64_t i) const { return {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; }
   static V3 get(const std::vector<float>& a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
   static void put(std::vector<float>& a, int64_t i, V3 q) {
+    a[3 * i] = q.x; a[3 * i + 1] = q.y; a[3 * i + 2] = q.z;
+  }
+  static V3 get_ptr(const float* a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+  static void put_ptr(float* a, int64_t i, V3 q) {
     a[3 * i] = q.x; a[3 * i + 1] = q.y; a[3 * i + 2] = q.z;
   }
   bool is_fluid(int64_t i) const { return material[i] == 1; }
@@ -367,6 +372,7 @@ Sim* or_new(const OrConfig* c) {
   s->f_neg_a = (float)(-c->gen_a);
   s->f_neg_b = (float)(-c->del_b);
   s->dt = (float)c->time_step;
+  s->f_inv_dt = (float)(1.0 / (double)s->dt);
   s->particle_num = 0;
   s->fluid_particle_num = 0;
   s->solid_particle_num = 0;
@@ -714,6 +720,31 @@ double or_compute_density_error(Sim* s, float offset) {
   return canon_reduce_vec(std::move(terms));
 }
 
+// dimcv_sph.py:465-468, 546-549: a Jacobi update's reaction on the dynamic
+// rigid neighbours of each fluid particle, acceleration[j] += -vel_change
+// inv_dt rho_i / rho_j, added serially in ascending p_i (kappa_i as the update
+// saw it: kappa(i) of the iteration's fields).
+}  // extern "C"
+template <class K>
+static void rigid_reactions(Sim* s, K kappa) {
+  const float dt = s->dt;
+  for (int64_t p_i = 0; p_i < s->particle_num; ++p_i) {
+    if (!s->is_fluid(p_i)) continue;
+    const float k_i = kappa(p_i);
+    if (!(std::fabs(k_i) > s->f_m_eps)) continue;
+    V3 xi = s->X(p_i);
+    s->for_all_neighbors(p_i, [&](int64_t p_j) {
+      if (!s->is_dynamic_rigid(p_j)) return;
+      V3 g = smul(s->m_V[p_j] * s->f_density0, s->gradW(s->rel(xi, s->X(p_j))));
+      V3 vc = smul(-dt * 1.0f * k_i, g);
+      const float ri = s->density[p_i], rj = s->density[p_j];
+      V3 add{((-vc.x) * s->f_inv_dt) * ri / rj, ((-vc.y) * s->f_inv_dt) * ri / rj, ((-vc.z) * s->f_inv_dt) * ri / rj};
+      Sim::put(s->acceleration, p_j, vadd(Sim::get(s->acceleration, p_j), add));
+    });
+  }
+}
+extern "C" {
+
 // dimcv_sph.py:421-468
 void or_divergence_solver_iteration_kernel(Sim* s) {
   const float dt = s->dt;
@@ -746,6 +777,7 @@ void or_divergence_solver_iteration_kernel(Sim* s) {
     else vi = vadd(vi, smul(s->transit_weight(xi.x), dv));
     Sim::put(s->v, p_i, vi);
   }
+  rigid_reactions(s, [&](int64_t i) { return s->density_adv[i] * s->dfsph_factor[i] * s->density[i]; });
 }
 
 // dimcv_sph.py:150-208 (surface tension and boundary viscosity are exact zeros)
@@ -1113,6 +1145,7 @@ void or_pressure_solve_iteration_kernel(Sim* s) {
     });
     Sim::put(s->v, p_i, vi);
   }
+  rigid_reactions(s, [&](int64_t i) { return (s->density_adv[i] - s->f_density0) * s->dfsph_factor[i]; });
 }
 
 // dimcv_sph.py:797-801
@@ -1174,6 +1207,85 @@ void or_enforce_boundary_3d(Sim* s) {
       const float c_f = 0.5f;
       Sim::put(s->v, p_i, vsub(v, smul((1.0f + c_f) * vdot(v, vec), vec)));
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Segment solver (segment_solver.py): the O(N^2) regularised Biot-Savart sums
+// and the RK4 endpoint update, serial j order, f32.
+struct OrSegConst {
+  float reg_radius, dt, dt6, inv4pi, gamma_decay;
+  float u_inf[3];
+};
+
+// form 0: _compute_endpoint_velocity_bs (:378-418); form 1:
+// _compute_velocity_at_factor (:468-521) with query offset (factor dt) k.
+void or_segment_velocity(int32_t n, const float* xm, const float* xp, const float* gamma, const int32_t* active,
+                         const OrSegConst* c, int32_t form, float factor, const float* km, const float* kp,
+                         float* out_m, float* out_p) {
+  const float R2 = c->reg_radius * c->reg_radius;
+  const float qs = factor * c->dt;
+  for (int32_t i = 0; i < n; ++i) {
+    if (active[i] != 1) continue;
+    V3 xmi = Sim::get_ptr(xm, i), xpi = Sim::get_ptr(xp, i);
+    if (form == 1) {
+      xmi = vadd(xmi, smul(qs, Sim::get_ptr(km, i)));
+      xpi = vadd(xpi, smul(qs, Sim::get_ptr(kp, i)));
+    }
+    V3 um{c->u_inf[0], c->u_inf[1], c->u_inf[2]}, up = um;
+    for (int32_t j = 0; j < n; ++j) {
+      if (j == i || active[j] != 1) continue;
+      V3 xmj = Sim::get_ptr(xm, j), xpj = Sim::get_ptr(xp, j);
+      V3 cj = smul(0.5f, vadd(xmj, xpj));
+      V3 dj = vsub(xpj, xmj);
+      V3 w;
+      if (form == 0) {
+        float Lj = vnorm(dj) + 1e-8f;
+        V3 tj = vdivs(dj, Lj);
+        w = smul(gamma[j] * Lj, tj);
+      } else {
+        w = smul(gamma[j], dj);
+      }
+      V3 r = vsub(xmi, cj);
+      float r2 = vdot(r, r) + R2;
+      float den = r2 * std::sqrt(r2) + 1e-12f;
+      um = vadd(um, vdivs(smul(c->inv4pi, vcross(w, r)), den));
+      V3 rp = vsub(xpi, cj);
+      float s2 = vdot(rp, rp) + R2;
+      float denp = s2 * std::sqrt(s2) + 1e-12f;
+      up = vadd(up, vdivs(smul(c->inv4pi, vcross(w, rp)), denp));
+    }
+    Sim::put_ptr(out_m, i, um);
+    Sim::put_ptr(out_p, i, up);
+  }
+}
+
+// advect_segments_rk4 (:420-447) with _update_endpoints_rk4 (:523-542);
+// k: 24 n floats of stage scratch (k1..k4 minus, then plus).
+void or_segment_rk4(int32_t n, float* xm, float* xp, float* gamma, const int32_t* active, float* age,
+                    const OrSegConst* c, const float* vm, const float* vp, float* k) {
+  const int64_t n3 = 3 * (int64_t)n;
+  float* km[4] = {k, k + n3, k + 2 * n3, k + 3 * n3};
+  float* kp[4] = {k + 4 * n3, k + 5 * n3, k + 6 * n3, k + 7 * n3};
+  for (int32_t i = 0; i < n; ++i)
+    if (active[i] == 1)
+      for (int d = 0; d < 3; ++d) {
+        km[0][3 * i + d] = vm[3 * i + d];
+        kp[0][3 * i + d] = vp[3 * i + d];
+      }
+  const float fac[3] = {0.5f, 0.5f, 1.0f};
+  for (int st = 0; st < 3; ++st)
+    or_segment_velocity(n, xm, xp, gamma, active, c, 1, fac[st], km[st], kp[st], km[st + 1], kp[st + 1]);
+  for (int32_t i = 0; i < n; ++i) {
+    if (active[i] != 1) continue;
+    V3 sm = vadd(vadd(vadd(Sim::get_ptr(km[0], i), smul(2.0f, Sim::get_ptr(km[1], i))),
+                      smul(2.0f, Sim::get_ptr(km[2], i))), Sim::get_ptr(km[3], i));
+    V3 sp = vadd(vadd(vadd(Sim::get_ptr(kp[0], i), smul(2.0f, Sim::get_ptr(kp[1], i))),
+                      smul(2.0f, Sim::get_ptr(kp[2], i))), Sim::get_ptr(kp[3], i));
+    Sim::put_ptr(xm, i, vadd(Sim::get_ptr(xm, i), smul(c->dt6, sm)));
+    Sim::put_ptr(xp, i, vadd(Sim::get_ptr(xp, i), smul(c->dt6, sp)));
+    gamma[i] *= c->gamma_decay;
+    age[i] += c->dt;
   }
 }
 

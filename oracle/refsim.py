@@ -179,9 +179,6 @@ class RefSim:
         rigid_num = sum(self.compute_cube_particle_num(b["start"], b["end"]) for b in rigid_blocks)
         if scene.get("RigidBodies"):
             raise NotImplementedError("mesh rigid bodies are out of scope")
-        for b in rigid_blocks:
-            if b.get("isDynamic"):
-                raise NotImplementedError("dynamic rigid blocks are out of scope")
         self.particle_max_num = fluid_num + rigid_num + (get("reservedCapacity") or 0)
         self.obj_emitters = [e["objectId"] for e in self.fluid_emitters]
         self.emitter_counts = {e["objectId"]: 0 for e in self.fluid_emitters}
@@ -546,3 +543,62 @@ def export_png(ref: RefSim, cnt: int, image_path) -> None:
                 transparent=True, dpi=400)
     plt.cla()
     plt.close("all")
+
+
+class OrSegConst(ctypes.Structure):
+    _fields_ = [("reg_radius", ctypes.c_float), ("dt", ctypes.c_float), ("dt6", ctypes.c_float),
+                ("inv4pi", ctypes.c_float), ("gamma_decay", ctypes.c_float), ("u_inf", ctypes.c_float * 3)]
+
+
+def seg_const(reg_radius=0.01, dt=0.002, gamma_decay=1.0, u_inf=(0.0, 0.0, 0.0)) -> OrSegConst:
+    """SegmentSolver constants as the kernels see them (segment_solver.py:9-20):
+    f32 of the Python values, dt/6 and 1/(4 pi) formed in f64 first."""
+    c = OrSegConst()
+    c.reg_radius = np.float32(reg_radius)
+    c.dt = np.float32(dt)
+    c.dt6 = np.float32(dt / 6.0)
+    c.inv4pi = np.float32(1.0 / (4.0 * np.pi))
+    c.gamma_decay = np.float32(gamma_decay)
+    for d in range(3):
+        c.u_inf[d] = np.float32(u_inf[d])
+    return c
+
+
+def _seg_lib():
+    L = lib()
+    if not getattr(L, "_seg_sigs", False):
+        P, I32, F32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
+        L.or_segment_velocity.restype = None
+        L.or_segment_velocity.argtypes = [I32, P, P, P, P, P, I32, F32, P, P, P, P]
+        L.or_segment_rk4.restype = None
+        L.or_segment_rk4.argtypes = [I32, P, P, P, P, P, P, P, P, P]
+        L._seg_sigs = True
+    return L
+
+
+def segment_velocity(xm, xp, gamma, active, const, form, factor=0.0, km=None, kp=None, out_m=None, out_p=None):
+    """Endpoint velocities (form 0, segment_solver.py:378-418) or velocities
+    at x + factor dt k (form 1, :468-521); out arrays updated in place for
+    active segments (copies of zeros when not given)."""
+    n = len(gamma)
+    a = [np.ascontiguousarray(v, np.float32) for v in (xm, xp, gamma)]
+    act = np.ascontiguousarray(active, np.int32)
+    km = np.zeros((n, 3), np.float32) if km is None else np.ascontiguousarray(km, np.float32)
+    kp = np.zeros((n, 3), np.float32) if kp is None else np.ascontiguousarray(kp, np.float32)
+    out_m = np.zeros((n, 3), np.float32) if out_m is None else out_m
+    out_p = np.zeros((n, 3), np.float32) if out_p is None else out_p
+    _seg_lib().or_segment_velocity(n, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(act), ctypes.byref(const), int(form),
+                                   float(factor), _ptr(km), _ptr(kp), _ptr(out_m), _ptr(out_p))
+    return out_m, out_p
+
+
+def segment_rk4(xm, xp, gamma, active, age, const, vm, vp):
+    """advect_segments_rk4 (segment_solver.py:420-447, 523-542) in place on
+    xm, xp, gamma, age (f32 numpy arrays)."""
+    n = len(gamma)
+    k = np.zeros(24 * n, np.float32)
+    vm = np.ascontiguousarray(vm, np.float32)
+    vp = np.ascontiguousarray(vp, np.float32)
+    act = np.ascontiguousarray(active, np.int32)
+    _seg_lib().or_segment_rk4(n, _ptr(xm), _ptr(xp), _ptr(gamma), _ptr(act), _ptr(age), ctypes.byref(const),
+                              _ptr(vm), _ptr(vp), _ptr(k))

@@ -851,3 +851,30 @@ def test_vort2vel_equals_direct_sum_for_constant_kden_and_all_samples():
     L.or_vort2vel(s.sim, R._ptr(near))
     far = d64[:F] - near[:F]
     assert np.linalg.norm(far) > 0.3 * np.linalg.norm(d64[:F])
+
+
+def test_rigid_reaction_is_the_opposite_impulse():
+    # dimcv_sph.py:546-549: the reaction on a dynamic rigid neighbour j is the
+    # fluid particle's velocity change from j, reversed and scaled by rho_i /
+    # rho_j (equal and opposite momentum for m = m_V0 rho): m_j dt a_j = -m_i dv_i
+    a = 0.35 * H
+    pos = [[0.5, 0.5, 0.5], [0.5 - a, 0.5, 0.5]]
+    s, slot = make(pos, material=[1, 0], dynamic=[1, 1], density=[1000.0, 600.0])
+    n = s.particle_num
+    s.field("density")[slot[0]] = 950.0
+    s.field("dfsph_factor")[:n] = 1.0
+    s.field("density_adv")[slot[0]] = RHO0 + 4.0      # kappa_i = 4
+    s.field("v")[:n] = 0.0
+    s.field("acceleration")[:n] = 0.0
+    L.or_pressure_solve_iteration_kernel(s.sim)
+    dv_i = s.field("v")[slot[0]].astype(np.float64)
+    acc_j = s.field("acceleration")[slot[1]].astype(np.float64)
+    assert dv_i[0] > 0                                   # pushed away from the rigid particle (+x)
+    np.testing.assert_allclose(acc_j * DT * 600.0, -dv_i * 950.0, rtol=1e-5)
+    # a static solid receives nothing
+    s2, slot2 = make(pos, material=[1, 0], dynamic=[1, 0])
+    s2.field("dfsph_factor")[:2] = 1.0
+    s2.field("density_adv")[slot2[0]] = RHO0 + 4.0
+    s2.field("acceleration")[:2] = 0.0
+    L.or_pressure_solve_iteration_kernel(s2.sim)
+    assert np.all(s2.field("acceleration")[slot2[1]] == 0.0)
