@@ -10,15 +10,15 @@
 // every canonical tree or scan of a scene covers exactly what it covers in a
 // solo run, and the per-scene results are bitwise those of the solo run.
 //
-// Per scene: Jacobi convergence flags (every Jacobi kernel reads the flag of
+// Per scene: Jacobi convergence flags (every Jacobi pass reads the flag of
 // its block's scene), FP64 reductions of the block partials, the CDF scan
 // and draws, and the far-field work (persistent kernel over a work queue).
 //
-// The step is issued as five segments separated by the host's convergence
-// checks: grid + neighbours + the first divergence iterations; further
-// divergence iterations; the middle of the substep + the first pressure
-// iterations; further pressure iterations; the tail.  None of them
-// synchronises or allocates, so each can be captured into a HIP graph.
+// Each Jacobi solve is one persistent launch (k_batch_solve) that iterates
+// until every scene has converged, with the convergence tests on the device,
+// so the step needs no host check: it is captured as one HIP graph (grid,
+// neighbour lists, substep) and the host reads the counters once at its end.
+// Nothing in the step synchronises or allocates.
 #include "common.h"
 
 namespace dimcv {
@@ -44,6 +44,9 @@ int batch_cdf_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws
                      int64_t m_total, hipStream_t s);
 int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, BatchWs& bw, hipStream_t s);
 void batch_set_far_workgroups(int w);
+int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
+                       int32_t* bar, hipStream_t s);
 
 static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
@@ -214,62 +217,39 @@ __global__ void k_batch_place_fluid(dimcv_state st, const int32_t* __restrict__ 
 }
 
 // Canonical FP64 reduction of each scene's block partials (one workgroup per
-// scene; the levels of k_reduce_levels) into scene_dscal[s][which].
-__global__ void __launch_bounds__(256) k_batch_reduce(dimcv_state st, double* __restrict__ partials, int which,
-                                                      const int32_t* __restrict__ done) {
+// scene) into scene_dscal[s][which].
+__global__ void __launch_bounds__(256) k_batch_reduce(dimcv_state st, double* __restrict__ partials, int which) {
   __shared__ double lds[256];
-  const int s = blockIdx.x;
-  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
-  if (done && done[(int64_t)s * DIMCV_SC_COUNT]) return;  // uniform per block
-  double* a = partials + c[DIMCV_SC_FLUID_START] / 256;
-  const int64_t n = ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256;
-  int64_t m = n;
-  while (m > 1) {
-    const int64_t groups = (m + 255) / 256;
-    for (int64_t g = 0; g < groups; ++g) {
-      const int64_t i = g * 256 + threadIdx.x;
-      const double v = i < m ? a[i] : 0.0;
-      const double r = block_tree_sum256(v, lds);
-      if (threadIdx.x == 0) a[g] = r;
-      __syncthreads();
-    }
-    m = groups;
-  }
-  if (threadIdx.x == 0) st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + which] = n > 0 ? a[0] : 0.0;
+  batch_reduce_scene(st, partials, which, blockIdx.x, lds);
 }
 
-__global__ void k_batch_jacobi_reset(dimcv_state st) {
+// Before a solve: every scene iterates (none when the step failed), no
+// iterations counted, the persistent solve's barrier words cleared.
+__global__ void k_batch_jacobi_reset(dimcv_state st, int32_t* __restrict__ bar) {
+  const int32_t d = step_failed(st) ? 1 : 0;
   for (int s = threadIdx.x; s < st.n_scenes; s += blockDim.x) {
-    st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_DONE] = 0;
+    st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_DONE] = d;
     st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_ITERS] = 0;
   }
   if (threadIdx.x == 0) {
     st.counters[DIMCV_C_JACOBI_DONE] = 0;
     st.counters[DIMCV_C_JACOBI_ITERS] = 0;
+    bar[0] = 0;  // arrivals
+    bar[2] = 0;  // abort
+    bar[3] = 0;  // go flags
+    bar[4] = 0;
   }
 }
 
-// Per-scene convergence test of dimcv_sph.py:384-391 / 606-613 (average =
-// FP64 residual / the scene's fluid count); counters[JACOBI_DONE] = every
-// scene converged.  A scene without fluid counts as converged.
-__global__ void __launch_bounds__(256) k_batch_jacobi_check(dimcv_state st, double eta, int32_t max_iter,
-                                                            int which) {
-  __shared__ int all;
-  if (threadIdx.x == 0) all = 1;
-  __syncthreads();
-  for (int s = threadIdx.x; s < st.n_scenes; s += blockDim.x) {
-    int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
-    if (!c[DIMCV_SC_DONE]) {
-      const int32_t it = ++c[DIMCV_SC_ITERS];
-      const int32_t F = c[DIMCV_SC_FLUID];
-      const double avg = F > 0 ? st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + DIMCV_SD_RESIDUAL] / (double)F : 0.0;
-      if (F <= 0 || avg <= eta || it >= max_iter) c[DIMCV_SC_DONE] = 1;
-      c[which ? DIMCV_SC_ITERS_P : DIMCV_SC_ITERS_V] = it;
-    }
-    if (!c[DIMCV_SC_DONE]) atomicAnd(&all, 0);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) st.counters[DIMCV_C_JACOBI_DONE] = all;
+// After the batched list build: a key past a scene's table, a list wider
+// than nbr_kmax or a batch capacity error makes the step's state writers
+// return (step_failed) and its solves start converged.
+__global__ void k_batch_guard(int32_t* counters, const int32_t* misc, int32_t nbr_kmax) {
+  int32_t e = 0;
+  if (counters[DIMCV_C_KEY_ERROR] != 0) e |= 1;
+  if (counters[DIMCV_C_NBR_MAX] > nbr_kmax) e |= 2;
+  if (misc[BM_ERROR] != 0) e |= 8;
+  counters[DIMCV_C_STEP_ERROR] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -309,28 +289,25 @@ static int seg_div_begin(BatchCtx& c, hipStream_t s) {
   CHECK(launch_pack_xm(st, c.p, nb, c.pk, s));
   CHECK(launch_fused(3, st, c.p, nb, s, nullptr, &c.pk));
   CHECK(launch_scale_kappa(0, st, c.p, c.o->inv_dt, nb, c.ws.kap, s));
-  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
 }
 
-// `chunk` Jacobi iterations of the divergence (which = 0) or pressure solve.
-static int seg_iters(BatchCtx& c, int which, hipStream_t s) {
+// The divergence (which = 0) or pressure solve of every scene, to
+// convergence: the first iteration's two passes (leaving the block residual
+// partials), then one persistent launch for the tests and any further
+// iterations.
+static int seg_solve(BatchCtx& c, int which, hipStream_t s) {
   const dimcv_state* st = c.st;
   const int32_t nb = (int32_t)c.p->n_max;
   const int32_t* done = st->scene_counters + DIMCV_SC_DONE;
-  const int chunk = c.o->chunk > 0 ? c.o->chunk : 1;
-  for (int k = 0; k < chunk; ++k) {
-    CHECK(launch_jacobi_pass(which == 0 ? 0 : 2, st, c.p, nb, done, s, nullptr, 0.0f, &c.pk));
-    CHECK(launch_jacobi_pass(which == 0 ? 1 : 3, st, c.p, nb, done, s, nullptr, 0.0f, &c.pk));
-    CHECK(launch_block_partials(0, st, c.p, which == 0 ? 0.0f : c.o->density0, nb, c.ws.partials, done, s));
-    hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
-                       (int)DIMCV_SD_RESIDUAL, done);
-    hipLaunchKernelGGL(k_batch_jacobi_check, dim3(1), dim3(256), 0, s, *st, which == 0 ? c.o->eta_v : c.o->eta_p,
-                       which == 0 ? c.o->max_iter_v : c.o->max_iter_p, which);
-    DIMCV_LAUNCHED("batched Jacobi iteration");
-  }
-  return DIMCV_OK;
+  CHECK(launch_jacobi_pass(which == 0 ? 0 : 2, st, c.p, nb, done, s, nullptr, 0.0f, &c.pk));
+  CHECK(launch_jacobi_pass(which == 0 ? 1 : 3, st, c.p, nb, done, s, c.ws.partials,
+                           which == 0 ? 0.0f : c.o->density0, &c.pk));
+  return launch_batch_solve(which, c.st, c.p, (int32_t)c.p->n_max, c.pk, c.ws.partials,
+                            which == 0 ? 0.0f : c.o->density0, which == 0 ? c.o->eta_v : c.o->eta_p,
+                            which == 0 ? c.o->max_iter_v : c.o->max_iter_p, c.bw.misc + BM_BAR, s);
 }
 
 static int seg_mid(BatchCtx& c, hipStream_t s) {
@@ -343,7 +320,7 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   CHECK(dimcv_compute_kvn_sample(st, p, nullptr, nullptr, 0, nb, s));
   CHECK(launch_fused(4, st, p, nb, s, c.ws.partials, &c.pk));
   hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)st->n_scenes), dim3(256), 0, s, *st, c.ws.partials,
-                     (int)DIMCV_SD_KDEN_SUM, (const int32_t*)nullptr);
+                     (int)DIMCV_SD_KDEN_SUM);
   DIMCV_LAUNCHED("batched kden");
   CHECK(batch_cdf_launch(st, p, c.ws, c.bw, c.o->u_cdf, c.o->m, s));
   CHECK(batch_vort2vel_launch(st, p, c.ws, c.bw, s));
@@ -351,7 +328,7 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   CHECK(dimcv_predict_velocity(st, p, nb, s));
   CHECK(launch_jacobi_pass(3, st, p, nb, nullptr, s, nullptr, 0.0f, &c.pk));
   CHECK(launch_scale_kappa(1, st, p, c.o->inv_dt2, nb, c.ws.kap, s));
-  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st);
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
 }
@@ -381,35 +358,34 @@ static int make_ctx(BatchCtx& c, const dimcv_state* pre, const dimcv_state* st, 
   return DIMCV_OK;
 }
 
-static int read_done(const dimcv_state* st, int32_t* host, hipStream_t s) {
-  DIMCV_HIP(hipMemcpyAsync(host, st->counters + DIMCV_C_JACOBI_DONE, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  DIMCV_HIP(hipStreamSynchronize(s));
-  return DIMCV_OK;
-}
-
 static int read_iters(const dimcv_state* st, int32_t* out_iters, int32_t* tmp, hipStream_t s) {
   const int64_t S = st->n_scenes;
   DIMCV_HIP(hipMemcpyAsync(tmp, st->scene_counters, sizeof(int32_t) * DIMCV_SC_COUNT * S, hipMemcpyDeviceToHost, s));
   DIMCV_HIP(hipStreamSynchronize(s));
-  for (int64_t k = 0; k < S; ++k) {
+  for (int64_t k = 0; out_iters && k < S; ++k) {
     out_iters[2 * k] = tmp[k * DIMCV_SC_COUNT + DIMCV_SC_ITERS_V];
     out_iters[2 * k + 1] = tmp[k * DIMCV_SC_COUNT + DIMCV_SC_ITERS_P];
   }
   return DIMCV_OK;
 }
 
-static int check_counters(const int32_t* hc, const int32_t* misc_err, const dimcv_params* p) {
-  if (hc[DIMCV_C_KEY_ERROR] != 0) {
+// hc: the counters; misc: the batch misc words (or null).
+static int check_counters(const int32_t* hc, const int32_t* misc, const dimcv_params* p) {
+  if (hc && hc[DIMCV_C_KEY_ERROR] != 0) {
     set_error("%d active solid particles lie past their scene's cell table", hc[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;
   }
-  if (hc[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
+  if (hc && hc[DIMCV_C_NBR_MAX] > p->nbr_kmax) {
     set_error("neighbour list width %d exceeds nbr_kmax %d", hc[DIMCV_C_NBR_MAX], p->nbr_kmax);
     return DIMCV_ERR_NBR;
   }
-  if (misc_err && *misc_err) {
-    set_error("batch capacity exceeded (flags %d: 1 fluid list, 2 far-field partials)", *misc_err);
+  if (misc && misc[BM_ERROR]) {
+    set_error("batch capacity exceeded (flags %d: 1 fluid list, 2 far-field partials)", misc[BM_ERROR]);
     return DIMCV_ERR_RANGE;
+  }
+  if (misc && misc[BM_BAR + 2]) {
+    set_error("batched Jacobi solve: a grid barrier wait timed out (blocks not all resident)");
+    return DIMCV_ERR_HIP;
   }
   return DIMCV_OK;
 }
@@ -421,12 +397,12 @@ using namespace dimcv;
 // ---------------------------------------------------------------------------
 // step graphs
 struct dimcv_batch_graph {
-  hipGraph_t graph[2][5];
-  hipGraphExec_t exec[2][5];
+  hipGraph_t graph[2];  // one step per buffer parity
+  hipGraphExec_t exec[2];
   dimcv_state st[2];
   dimcv_params p;
   dimcv_step_opts o;
-  int32_t* pinned;  // counters, misc error, scene counters
+  int32_t* pinned;  // counters, misc words, scene counters
   int32_t n_scenes;
 };
 
@@ -482,13 +458,13 @@ int dimcv_batch_status(const dimcv_state* st, const dimcv_params* p, int32_t* ho
   CHECK(make_ctx(c, st, st, p, nullptr));
   hipStream_t s = (hipStream_t)stream;
   int32_t hc[DIMCV_C_COUNT];
-  int32_t err = 0;
+  int32_t misc[BM_COUNT];
   DIMCV_HIP(hipMemcpyAsync(hc, st->counters, sizeof(hc), hipMemcpyDeviceToHost, s));
-  DIMCV_HIP(hipMemcpyAsync(&err, c.bw.misc + BM_ERROR, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipMemcpyAsync(misc, c.bw.misc, sizeof(misc), hipMemcpyDeviceToHost, s));
   DIMCV_HIP(hipStreamSynchronize(s));
   if (host_counters)
     for (int k = 0; k < DIMCV_C_COUNT; ++k) host_counters[k] = hc[k];
-  return check_counters(hc, &err, p);
+  return check_counters(hc, misc, p);
 }
 
 int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t* out_iters,
@@ -500,60 +476,51 @@ int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step
   hipStream_t s = (hipStream_t)stream;
   BatchCtx c;
   CHECK(make_ctx(c, st, st, p, o));
-  int32_t done = 0;
   CHECK(seg_div_begin(c, s));
-  do {
-    CHECK(seg_iters(c, 0, s));
-    CHECK(read_done(st, &done, s));
-  } while (!done);
+  CHECK(seg_solve(c, 0, s));
   CHECK(seg_mid(c, s));
-  do {
-    CHECK(seg_iters(c, 1, s));
-    CHECK(read_done(st, &done, s));
-  } while (!done);
+  CHECK(seg_solve(c, 1, s));
   CHECK(seg_post(c, s));
-  if (out_iters) {
-    int32_t* tmp = new int32_t[(size_t)st->n_scenes * DIMCV_SC_COUNT];
-    int rc = read_iters(st, out_iters, tmp, s);
-    delete[] tmp;
-    return rc;
-  }
-  return DIMCV_OK;
+  int32_t misc[BM_COUNT];
+  DIMCV_HIP(hipMemcpyAsync(misc, c.bw.misc, sizeof(misc), hipMemcpyDeviceToHost, s));
+  int32_t* tmp = new int32_t[(size_t)st->n_scenes * DIMCV_SC_COUNT];
+  int rc = read_iters(st, out_iters, tmp, s);  // synchronises the stream
+  delete[] tmp;
+  if (rc) return rc;
+  return check_counters(nullptr, misc, p);
 }
 
 void dimcv_batch_graph_destroy(dimcv_batch_graph* g) {
   if (!g) return;
-  for (int a = 0; a < 2; ++a)
-    for (int k = 0; k < 5; ++k) {
-      if (g->exec[a][k]) (void)hipGraphExecDestroy(g->exec[a][k]);
-      if (g->graph[a][k]) (void)hipGraphDestroy(g->graph[a][k]);
-    }
+  for (int a = 0; a < 2; ++a) {
+    if (g->exec[a]) (void)hipGraphExecDestroy(g->exec[a]);
+    if (g->graph[a]) (void)hipGraphDestroy(g->graph[a]);
+  }
   if (g->pinned) (void)hipHostFree(g->pinned);
   delete g;
 }
 
-static int capture(dimcv_batch_graph* g, int par, int seg, hipStream_t s) {
+// The whole step of buffer parity par as one graph: grid build, lists, the
+// build guard and the substep with both solves.
+static int capture(dimcv_batch_graph* g, int par, hipStream_t s) {
   const dimcv_state* pre = &g->st[par];
   const dimcv_state* cur = &g->st[par ^ 1];
   BatchCtx c;
   CHECK(make_ctx(c, pre, cur, &g->p, &g->o));
   DIMCV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-  int rc = DIMCV_OK;
-  switch (seg) {
-    case 0:
-      rc = seg_grid(c, s);
-      if (!rc) rc = batch_nbr_launch(cur, &g->p, s);
-      if (!rc) rc = seg_div_begin(c, s);
-      if (!rc) rc = seg_iters(c, 0, s);
-      break;
-    case 1: rc = seg_iters(c, 0, s); break;
-    case 2:
-      rc = seg_mid(c, s);
-      if (!rc) rc = seg_iters(c, 1, s);
-      break;
-    case 3: rc = seg_iters(c, 1, s); break;
-    default: rc = seg_post(c, s); break;
+  int rc = seg_grid(c, s);
+  if (!rc) rc = batch_nbr_launch(cur, &g->p, s);
+  if (!rc) {
+    hipLaunchKernelGGL(k_batch_guard, dim3(1), dim3(1), 0, s, cur->counters, (const int32_t*)c.bw.misc,
+                       g->p.nbr_kmax);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) rc = hip_fail(le, "k_batch_guard");
   }
+  if (!rc) rc = seg_div_begin(c, s);
+  if (!rc) rc = seg_solve(c, 0, s);
+  if (!rc) rc = seg_mid(c, s);
+  if (!rc) rc = seg_solve(c, 1, s);
+  if (!rc) rc = seg_post(c, s);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (rc) {
@@ -561,8 +528,8 @@ static int capture(dimcv_batch_graph* g, int par, int seg, hipStream_t s) {
     return rc;
   }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-  g->graph[par][seg] = graph;
-  DIMCV_HIP(hipGraphInstantiate(&g->exec[par][seg], graph, nullptr, nullptr, 0));
+  g->graph[par] = graph;
+  DIMCV_HIP(hipGraphInstantiate(&g->exec[par], graph, nullptr, nullptr, 0));
   return DIMCV_OK;
 }
 
@@ -583,7 +550,7 @@ int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_o
   g->p = *p;
   g->o = *opts;
   g->n_scenes = st_even->n_scenes;
-  hipError_t e = hipHostMalloc((void**)&g->pinned, sizeof(int32_t) * (DIMCV_C_COUNT + 1 + (size_t)DIMCV_SC_COUNT *
+  hipError_t e = hipHostMalloc((void**)&g->pinned, sizeof(int32_t) * (DIMCV_C_COUNT + BM_COUNT + (size_t)DIMCV_SC_COUNT *
                                                                       (size_t)(g->n_scenes > 0 ? g->n_scenes : 1)),
                                hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -591,18 +558,19 @@ int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_o
     dimcv_batch_graph_destroy(g);
     return hip_fail(e, "hipHostMalloc");
   }
-  for (int par = 0; par < 2; ++par)
-    for (int seg = 0; seg < 5; ++seg) {
-      int rc = capture(g, par, seg, (hipStream_t)stream);
-      if (rc) {
-        dimcv_batch_graph_destroy(g);
-        return rc;
-      }
+  for (int par = 0; par < 2; ++par) {
+    int rc = capture(g, par, (hipStream_t)stream);
+    if (rc) {
+      dimcv_batch_graph_destroy(g);
+      return rc;
     }
+  }
   *out = g;
   return DIMCV_OK;
 }
 
+// One graph launch, then one readback of the counters, the misc words and
+// the scene counters (iteration counts) for the step's status.
 int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_iters, void* stream) {
   if (!g || (parity != 0 && parity != 1)) {
     set_error("dimcv_batch_graph_step: bad arguments");
@@ -613,25 +581,13 @@ int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_it
   BatchWs bw;
   CHECK(bind_batch(cur, &g->p, &bw));
   int32_t* hc = g->pinned;
-  int32_t* herr = g->pinned + DIMCV_C_COUNT;
-  DIMCV_HIP(hipGraphLaunch(g->exec[parity][0], s));
+  int32_t* misc = g->pinned + DIMCV_C_COUNT;
+  int32_t* sc = misc + BM_COUNT;
+  DIMCV_HIP(hipGraphLaunch(g->exec[parity], s));
   DIMCV_HIP(hipMemcpyAsync(hc, cur->counters, sizeof(int32_t) * DIMCV_C_COUNT, hipMemcpyDeviceToHost, s));
-  DIMCV_HIP(hipMemcpyAsync(herr, bw.misc + BM_ERROR, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  DIMCV_HIP(hipStreamSynchronize(s));
-  CHECK(check_counters(hc, herr, &g->p));
-  while (!hc[DIMCV_C_JACOBI_DONE]) {
-    DIMCV_HIP(hipGraphLaunch(g->exec[parity][1], s));
-    CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
-  }
-  DIMCV_HIP(hipGraphLaunch(g->exec[parity][2], s));
-  CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
-  while (!hc[DIMCV_C_JACOBI_DONE]) {
-    DIMCV_HIP(hipGraphLaunch(g->exec[parity][3], s));
-    CHECK(read_done(cur, hc + DIMCV_C_JACOBI_DONE, s));
-  }
-  DIMCV_HIP(hipGraphLaunch(g->exec[parity][4], s));
-  if (out_iters) return read_iters(cur, out_iters, g->pinned + DIMCV_C_COUNT + 1, s);
-  return DIMCV_OK;
+  DIMCV_HIP(hipMemcpyAsync(misc, bw.misc, sizeof(int32_t) * BM_COUNT, hipMemcpyDeviceToHost, s));
+  CHECK(read_iters(cur, out_iters, sc, s));
+  return check_counters(hc, misc, &g->p);
 }
 
 }  // extern "C"

@@ -431,13 +431,17 @@ __device__ __forceinline__ bool step_failed(const dimcv_state& st) {
 // Jacobi gate: the solve of this block's scene (batched: done points at
 // scene_counters + DIMCV_SC_DONE) or the single solve has converged.
 // Launches using it are 256-wide over fluid-list entries, in xcd_block order.
-__device__ __forceinline__ bool jacobi_gated(const dimcv_state& st, const int32_t* done) {
+// jacobi_gated_lb takes the logical block explicitly (persistent loops).
+__device__ __forceinline__ bool jacobi_gated_lb(const dimcv_state& st, const int32_t* done, int64_t lb) {
   if (!done) return false;
   if (st.block_scene) {
-    const int s = st.block_scene[xcd_block()];
+    const int s = st.block_scene[lb];
     return s < 0 || done[(int64_t)s * DIMCV_SC_COUNT] != 0;
   }
   return *done != 0;
+}
+__device__ __forceinline__ bool jacobi_gated(const dimcv_state& st, const int32_t* done) {
+  return jacobi_gated_lb(st, done, xcd_block());
 }
 
 // kden_sum of the scene of this block.
@@ -492,6 +496,89 @@ __device__ inline double block_tree_sum256(double v, double* lds) {
   return r;
 }
 
+// Canonical FP64 sum of scene s's block partials (fluid-list blocks of 256,
+// reduced in place level by level with block_tree_sum256) into
+// scene_dscal[s][which].  Every thread of the calling block takes part.
+__device__ inline void batch_reduce_scene(const dimcv_state& st, double* __restrict__ partials, int which, int s,
+                                          double* lds) {
+  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  double* a = partials + c[DIMCV_SC_FLUID_START] / 256;
+  const int64_t n = ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256;
+  int64_t m = n;
+  while (m > 1) {
+    const int64_t groups = (m + 255) / 256;
+    for (int64_t g = 0; g < groups; ++g) {
+      const int64_t i = g * 256 + threadIdx.x;
+      const double v = i < m ? a[i] : 0.0;
+      const double r = block_tree_sum256(v, lds);
+      if (threadIdx.x == 0) a[g] = r;
+      __syncthreads();
+    }
+    m = groups;
+  }
+  if (threadIdx.x == 0) st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + which] = n > 0 ? a[0] : 0.0;
+}
+
+// Convergence test of scene s after an iteration (dimcv_sph.py:384-391 /
+// 606-613: average = FP64 residual / the scene's fluid count; a scene without
+// fluid counts as converged); returns whether the scene iterates on.
+__device__ inline bool batch_scene_check(const dimcv_state& st, double eta, int32_t max_iter, int which, int s) {
+  int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
+  if (c[DIMCV_SC_DONE]) return false;
+  const int32_t it = ++c[DIMCV_SC_ITERS];
+  const int32_t F = c[DIMCV_SC_FLUID];
+  const double avg = F > 0 ? st.scene_dscal[(int64_t)s * DIMCV_SD_COUNT + DIMCV_SD_RESIDUAL] / (double)F : 0.0;
+  if (F <= 0 || avg <= eta || it >= max_iter) c[DIMCV_SC_DONE] = 1;
+  c[which ? DIMCV_SC_ITERS_P : DIMCV_SC_ITERS_V] = it;
+  return c[DIMCV_SC_DONE] == 0;
+}
+
+// Grid-wide barrier of a persistent launch whose blocks are all resident.
+// bar[0]: arrivals, bar[1]: generation, bar[2]: abort (bar[0] and bar[2]
+// cleared before the launch); gen: the generation this block last saw (held
+// by thread 0).  Every wave waits for its stores to reach the XCD's L2, then
+// thread 0 releases them at agent scope (the L2s of the XCDs are not coherent
+// with each other: the release writes the L2's dirty lines back) and
+// arrives; the last arrival opens the next generation, the others wait for
+// it, and each block acquires (invalidating its CU's L1 and its XCD's L2).  A wait is bounded (about a second): past it the block raises
+// abort, and every waiting block returns false at once, so the launch always
+// ends; the caller reports the abort as an error.
+#define GRID_SPIN_LIMIT (1u << 20)
+__device__ inline bool grid_sync(int32_t* bar, int32_t& gen) {
+  __shared__ int ok;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores have reached the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int r = 1;
+    const int32_t g = gen;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's stores, from the L2 to memory
+    if (atomicAdd(&bar[0], 1) == (int32_t)gridDim.x - 1) {
+      atomicExch(&bar[0], 0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      atomicAdd(&bar[1], 1);
+    } else {
+      for (uint32_t spin = 0;; ++spin) {
+        if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) break;
+        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          r = 0;
+          break;
+        }
+        if (spin >= GRID_SPIN_LIMIT) {
+          atomicExch(&bar[2], 1);
+          r = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    gen = g + 1;
+    ok = r;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
 // Far-field partial of a split source range (A, Kahan compensation,
 // rejected samples).
 struct FarPartial {
@@ -508,7 +595,8 @@ enum BatchMisc {
   BM_ITEMS = 1,        // far-field work items
   BM_QUEUE = 2,        // far-field work queue head
   BM_TILES = 3,        // CDF scan tiles
-  BM_ERROR = 4,        // capacity error flags (1: fluid list, 2: far partials)
+  BM_ERROR = 4,        // error flags (1: fluid list, 2: far partials, 4: Jacobi barrier timed out)
+  BM_BAR = 5,          // persistent Jacobi solve: barrier arrivals, generation, abort, two go flags
   BM_COUNT = 16
 };
 struct BatchWs {

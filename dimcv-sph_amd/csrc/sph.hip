@@ -5,12 +5,17 @@
 // for_all_neighbors visits them (particle_system.py:562-586), so every
 // accumulation below is the reference's serial sum, term by term, in the
 // reference's expression order (dimcv_sph.py, sph_base.py).
+#include <algorithm>
+#include <mutex>
+
 #include "common.h"
 
 namespace dimcv {
 
-#define FLUID_ENTRY()                                                       \
-  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;        \
+#define FLUID_ENTRY() FLUID_ENTRY_LB(xcd_block())
+// The same for logical block lb of 256 fluid-list entries (persistent loops).
+#define FLUID_ENTRY_LB(lb)                                                  \
+  const int64_t f = (lb) * 256 + threadIdx.x;                              \
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;          \
   const int64_t i = st.fluid_list[f];                                      \
   if (i < 0) return;                                                       \
@@ -179,17 +184,14 @@ __device__ __forceinline__ void residual_partial(double v, double* __restrict__ 
   if (threadIdx.x == 0) partials[xcd_block()] = sum;
 }
 
-// dimcv_sph.py:337-374.  With `partials`, also the block partial of the
-// divergence residual (offset 0) of the values just written.
+// dimcv_sph.py:337-374 for the entries of logical block lb; returns the
+// entry's divergence-residual term (offset 0) of the value just written.
 template <bool XM>
-__global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done,
-                                                  double* __restrict__ partials, Packed pk) {
-  __shared__ double lds[256];
-  if (jacobi_gated(st, done)) return;  // uniform per block
+__device__ __forceinline__ double density_change_body(dimcv_state& st, const dimcv_params& p, int32_t bound,
+                                                      const Packed& pk, int64_t lb) {
   double res = 0.0;
   [&] {
-    FLUID_ENTRY();
+    FLUID_ENTRY_LB(lb);
     const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float dadv = 0.0f;
@@ -204,6 +206,17 @@ __global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_pa
     if (XM) pk.kap[i] = d * st.dfsph_factor[i] * st.density[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, 0.0f);
   }();
+  return res;
+}
+
+// With `partials`, also the block partial of the divergence residual.
+template <bool XM>
+__global__ void __launch_bounds__(256) k_density_change(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done,
+                                                  double* __restrict__ partials, Packed pk) {
+  __shared__ double lds[256];
+  if (jacobi_gated(st, done)) return;  // uniform per block
+  const double res = density_change_body<XM>(st, p, bound, pk, xcd_block());
   if (partials) residual_partial(res, partials, lds);
 }
 
@@ -298,10 +311,9 @@ __global__ void __launch_bounds__(256) k_residual(dimcv_state st, dimcv_params p
 // XM from the array the previous density change wrote); positions and mass
 // factors only for the pairs that pass |kappa_i + kappa_j| > eps.
 template <bool XM>
-__global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done, Packed pk) {
-  if (jacobi_gated(st, done)) return;
-  FLUID_ENTRY();
+__device__ __forceinline__ void div_iter_body(dimcv_state& st, const dimcv_params& p, int32_t bound,
+                                              const Packed& pk, int64_t lb) {
+  FLUID_ENTRY_LB(lb);
   const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -329,6 +341,13 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
   if (!p.has_transit) vi = vadd(vi, dv);
   else vi = vadd(vi, smul(transit_weight(p, xi.x), dv));
   st3(st.v, i, vi);
+}
+
+template <bool XM>
+__global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done, Packed pk) {
+  if (jacobi_gated(st, done)) return;
+  div_iter_body<XM>(st, p, bound, pk, xcd_block());
 }
 
 // dimcv_sph.py:150-208, fluid particles (surface tension sigma = 0 and the
@@ -371,7 +390,7 @@ __global__ void k_nonpressure_other(dimcv_state st, dimcv_params p, int32_t boun
 // dimcv_sph.py:220-226
 __global__ void k_predict(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM] || step_failed(st)) return;
   if (st.is_dynamic[s] && st.material[s] == 1)
     st3(st.v, s, vadd(ld3(st.v, s), smul(p.dt, ld3(st.acceleration, s))));
 }
@@ -379,7 +398,7 @@ __global__ void k_predict(dimcv_state st, dimcv_params p, int32_t bound) {
 // dimcv_sph.py:210-218
 __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM] || step_failed(st)) return;
   if (!st.is_dynamic[s]) return;
   v3 v = ld3(st.v, s);
   if (st.material[s] == 0) {
@@ -389,19 +408,14 @@ __global__ void k_advect(dimcv_state st, dimcv_params p, int32_t bound) {
   st3(st.x, s, vadd(ld3(st.x, s), smul(p.dt, v)));
 }
 
-// dimcv_sph.py:470-493
-// With `partials`, also the block partial of the pressure residual
-// (offset: the caller's density0) of the values just written.
+// dimcv_sph.py:470-493 for the entries of logical block lb; returns the
+// entry's pressure-residual term (offset: the caller's density0).
 template <bool XM>
-__global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
-                                                  const int32_t* __restrict__ done,
-                                                  double* __restrict__ partials, float offset,
-                                                  Packed pk) {
-  __shared__ double lds[256];
-  if (jacobi_gated(st, done)) return;  // uniform per block
+__device__ __forceinline__ double density_adv_body(dimcv_state& st, const dimcv_params& p, int32_t bound,
+                                                   float offset, const Packed& pk, int64_t lb) {
   double res = 0.0;
   [&] {
-    FLUID_ENTRY();
+    FLUID_ENTRY_LB(lb);
     const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
     v3 xi = ld3(st.x, i), vi = ld3(st.v, i);
     float delta = 0.0f;
@@ -425,17 +439,27 @@ __global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_param
     if (XM) pk.kap[i] = (d - p.density0) * st.dfsph_factor[i];  // the next iteration's kappa_i
     res = residual_term(p, st, i, d, offset);
   }();
+  return res;
+}
+
+// With `partials`, also the block partial of the pressure residual.
+template <bool XM>
+__global__ void __launch_bounds__(256) k_density_adv(dimcv_state st, dimcv_params p, int32_t bound,
+                                                  const int32_t* __restrict__ done,
+                                                  double* __restrict__ partials, float offset,
+                                                  Packed pk) {
+  __shared__ double lds[256];
+  if (jacobi_gated(st, done)) return;  // uniform per block
+  const double res = density_adv_body<XM>(st, p, bound, offset, pk, xcd_block());
   if (partials) residual_partial(res, partials, lds);
 }
 
 // dimcv_sph.py:495-549 (v_i updated in place, neighbour by neighbour);
 // kappa_j = (da_j - rho0) alpha_j first, as in k_div_iter.
 template <bool XM>
-__global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
-                                                       const int32_t* __restrict__ done,
-                                                       Packed pk) {
-  if (jacobi_gated(st, done)) return;
-  FLUID_ENTRY();
+__device__ __forceinline__ void pressure_iter_body(dimcv_state& st, const dimcv_params& p, int32_t bound,
+                                                   const Packed& pk, int64_t lb) {
+  FLUID_ENTRY_LB(lb);
   const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float dt = p.dt;
   v3 xi = ld3(st.x, i);
@@ -463,6 +487,114 @@ __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_par
         }
       });
   st3(st.v, i, vi);
+}
+
+template <bool XM>
+__global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_params p, int32_t bound,
+                                                       const int32_t* __restrict__ done,
+                                                       Packed pk) {
+  if (jacobi_gated(st, done)) return;
+  pressure_iter_body<XM>(st, p, bound, pk, xcd_block());
+}
+
+// A batched Jacobi solve to convergence in one persistent launch (which 0:
+// divergence, dimcv_sph.py:376-393; 1: pressure, :597-614), after its first
+// iteration's two passes, which run as ordinary full-occupancy launches and
+// leave the block residual partials.  The launch tests every scene (its
+// canonical FP64 residual and convergence test) and, while a scene iterates
+// on, runs further iterations: the velocity update (k_div_iter /
+// k_pressure_iter), the density change (k_density_change / k_density_adv)
+// with its block partials, and the tests again.  Grid barriers separate the
+// phases, and the blocks leave together when no scene iterates on.  Blocks
+// stride over the logical 256-entry blocks; a block of a converged scene is
+// skipped as the launched passes skip it, so every scene's arithmetic and
+// iteration count are those of separate launches.
+// bar: BM_BAR of the batch misc words (arrivals, generation, abort, go flags
+// for even and odd iterations; cleared by k_batch_jacobi_reset).
+template <int WHICH>
+__global__ void __launch_bounds__(256) k_batch_solve(dimcv_state st, dimcv_params p, int32_t bound, Packed pk,
+                                                     double* __restrict__ partials, float offset, double eta,
+                                                     int32_t max_iter, int32_t* __restrict__ bar) {
+  __shared__ double lds[256];
+  __shared__ int go;
+  int32_t gen = 0;
+  if (threadIdx.x == 0) gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t* done = st.scene_counters + DIMCV_SC_DONE;
+  int32_t* flag = bar + 3;
+  // XCD-aware: block b runs on XCD b % 8 (gridDim.x is a multiple of 8) and
+  // strides over that XCD's contiguous eighth of the logical blocks
+  const int64_t nlb = ((int64_t)bound + 255) / 256;
+  const int64_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
+  const int64_t lb0 = nlb * xcd / 8 + (blockIdx.x >> 3), lb1 = nlb * (xcd + 1) / 8;
+  for (int it = 0;; ++it) {
+    if (it > 0) {  // iteration it + 1 (the first ran before the launch)
+      for (int64_t lb = lb0; lb < lb1; lb += per) {
+        if (jacobi_gated_lb(st, done, lb)) continue;  // uniform per block
+        if (WHICH == 0) div_iter_body<true>(st, p, bound, pk, lb);
+        else pressure_iter_body<true>(st, p, bound, pk, lb);
+      }
+      if (!grid_sync(bar, gen)) return;
+      for (int64_t lb = lb0; lb < lb1; lb += per) {
+        if (jacobi_gated_lb(st, done, lb)) continue;
+        const double res = WHICH == 0 ? density_change_body<true>(st, p, bound, pk, lb)
+                                      : density_adv_body<true>(st, p, bound, offset, pk, lb);
+        const double sum = block_tree_sum256(res, lds);
+        if (threadIdx.x == 0) partials[lb] = sum;
+      }
+      if (!grid_sync(bar, gen)) return;
+    }
+    for (int sc = blockIdx.x; sc < st.n_scenes; sc += gridDim.x) {
+      if (done[(int64_t)sc * DIMCV_SC_COUNT]) continue;  // uniform
+      batch_reduce_scene(st, partials, DIMCV_SD_RESIDUAL, sc, lds);
+      if (threadIdx.x == 0 && batch_scene_check(st, eta, max_iter, WHICH, sc)) atomicOr(&flag[it & 1], 1);
+    }
+    // the other flag was last read before this iteration's barriers
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&flag[(it + 1) & 1], 0);
+    if (!grid_sync(bar, gen)) return;
+    if (threadIdx.x == 0) go = __hip_atomic_load(&flag[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (!go) break;
+  }
+}
+
+// Resident blocks of k_batch_solve<which> on the current device: the
+// occupancy bound, rounded down to a multiple of the 8 XCDs.
+static int solve_capacity(int which) {
+  static std::mutex mu;
+  static int cap[64][2];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (cap[dev][which] == 0) {
+    int per_cu = 0, cus = 0;
+    const void* f = which == 0 ? (const void*)k_batch_solve<0> : (const void*)k_batch_solve<1>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    cap[dev][which] = (per_cu * cus) & ~7;
+  }
+  return cap[dev][which];
+}
+
+int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                       const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
+                       int32_t* bar, hipStream_t s) {
+  const int cap = solve_capacity(which);
+  if (cap <= 0) {
+    set_error("persistent Jacobi solve: no resident block capacity on this device");
+    return DIMCV_ERR_HIP;
+  }
+  // a multiple of 8 blocks (idle blocks of a small batch only take part in
+  // the barriers)
+  const int64_t nlb = ((int64_t)fluid_bound + 255) / 256;
+  const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((nlb + 7) & ~int64_t(7), cap));
+  if (which == 0)
+    hipLaunchKernelGGL(k_batch_solve<0>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
+                       max_iter, bar);
+  else
+    hipLaunchKernelGGL(k_batch_solve<1>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
+                       max_iter, bar);
+  DIMCV_LAUNCHED("k_batch_solve");
+  return DIMCV_OK;
 }
 
 // dimcv_sph.py:797-801
@@ -803,7 +935,7 @@ __global__ void __launch_bounds__(256) k_kernel_density(dimcv_state st, dimcv_pa
 // dimcv_sph.py:753-760
 __global__ void k_life_time(dimcv_state st, dimcv_params p, int32_t bound) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
-  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
+  if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST] || step_failed(st)) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0 || !in_vortex_domain(p, ld3(st.x, i))) return;
   st.life_time[i] += p.dt;

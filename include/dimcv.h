@@ -442,9 +442,10 @@ typedef struct dimcv_step_opts {
   int32_t m;                /* fixed-M sample count; 0 selects the Bernoulli sample set */
   int32_t max_iter_v;       /* m_max_iterations_v (dimcv_sph.py:22) */
   int32_t max_iter_p;       /* m_max_iterations (:23) */
-  int32_t chunk;            /* Jacobi iterations issued per host convergence check (>= 1); 0: the
-                               first check after out_kv / out_kp iterations as passed in (the
-                               previous step's counts, at least 1), then after every iteration */
+  int32_t chunk;            /* dimcv_substep: Jacobi iterations issued per host convergence check
+                               (>= 1); 0: the first check after out_kv / out_kp iterations as
+                               passed in (the previous step's counts, at least 1), then after every
+                               iteration.  Unused by the batched step (no host checks). */
   double eta_v;             /* inv_dt * 0.1 * 0.01 * rho0 (:389) */
   double eta_p;             /* 0.05 * 0.01 * rho0 (:611) */
   float inv_dt;             /* f32(1 / dt) (:27, :381) */
@@ -558,8 +559,14 @@ int dimcv_segment_rk4(const dimcv_segments* sg, const float* v_minus, const floa
  * splitting of a scene are those of the same scene run alone: each scene's
  * results are bitwise equal to its solo run.  Batches are open boxes with no
  * solids, emitters, vortex domain or transit area, in fixed-M (inverse-CDF)
- * sampling mode.  Nothing here synchronises the host except
- * dimcv_batch_substep's convergence checks and dimcv_batch_status.
+ * sampling mode.  The Jacobi solves converge on the device: after each
+ * solve's first iteration (two ordinary launches), one persistent launch runs
+ * the per-scene convergence tests and any further iterations of every scene,
+ * its phases separated by grid barriers (its grid is at most the resident
+ * block count; a barrier wait that exceeds about a second aborts the launch
+ * and the step returns DIMCV_ERR_HIP).  Nothing here synchronises the host
+ * except the final status readback of dimcv_batch_substep and
+ * dimcv_batch_graph_step, and dimcv_batch_status.
  *
  * Capacities: n_max >= total particles + 256 * n_scenes; block_scene holds
  * n_max / 256 + 1 entries; n_cells (params) = sum of scene cells.
@@ -589,17 +596,20 @@ int dimcv_batch_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step
  * 64, 128, 256}. */
 int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block);
 
-/* Step graphs: the batched step (grid build, neighbour lists, substep)
- * captured into HIP graphs, one per segment between host convergence checks
- * (before/inside/after the two Jacobi solves), for both parities of the
- * sorted-buffer swap.  st_even / st_odd: the state before an even / odd step
+/* Step graphs: the whole batched step (grid build, neighbour lists, a guard
+ * that stops the state writers after a failed build, substep with both
+ * solves converging on the device) captured as one HIP graph, for each
+ * parity of the sorted-buffer swap: a step is one graph launch and one
+ * status readback.  st_even / st_odd: the state before an even / odd step
  * (buffers swapped).  `stream` must not be the null stream.  The uniforms are
  * read from opts->u_cdf at replay time, so the caller refreshes that buffer
  * before each step. */
 typedef struct dimcv_batch_graph dimcv_batch_graph;
 int dimcv_batch_graph_create(const dimcv_state* st_even, const dimcv_state* st_odd, const dimcv_params* p,
                              const dimcv_step_opts* opts, void* stream, dimcv_batch_graph** out);
-/* One step from the captured graphs; parity selects st_even / st_odd. */
+/* One step from the captured graph; parity selects st_even / st_odd.  Returns
+ * the build's errors (as dimcv_batch_status) after the step, whose state
+ * writers were skipped on the device. */
 int dimcv_batch_graph_step(dimcv_batch_graph* g, int32_t parity, int32_t* out_iters, void* stream);
 void dimcv_batch_graph_destroy(dimcv_batch_graph* g);
 

@@ -73,6 +73,35 @@ def test_batch_scenes_equal_solo_runs(far):
     assert batch.scene_counters_host()[0][6] == 12 * 10 * 8
 
 
+def test_iterating_solves_equal_solo_runs():
+    """A tighter divergence tolerance (max_error_V, dimcv_sph.py:24,389) makes
+    the scenes' divergence solves iterate, each to its own count, inside the
+    one persistent launch per solve; fields and both iteration counts stay
+    bitwise those of the solo runs, eagerly and from the captured graph."""
+    from batch import SceneBatch, solo_system
+    specs = _specs()
+    eager = SceneBatch(specs, far_field="fast")
+    graph = SceneBatch(specs, far_field="fast")
+    solos = [solo_system(sp, far_field="fast") for sp in specs]
+    for sv in [eager.solver, graph.solver] + [sv for _, sv in solos]:
+        sv.max_error_V = 1e-6
+    graph.capture()
+    kv = set()
+    for step in range(3):
+        u = eager.uniforms()
+        graph.uniforms()   # same seeded draws: keep the generators in step
+        eager.step(u)
+        graph.step_graph(u)
+        for ps, sv in solos:
+            sv.step()
+        for s, (ps, sv) in enumerate(solos):
+            _compare_scene(eager, s, ps, sv)
+            _compare_scene(graph, s, ps, sv)
+            kv.add(int(eager.last_iterations[s][0]))
+    graph.release_graph()
+    assert max(kv) > 1 and len(kv) > 1, kv
+
+
 def test_graph_steps_equal_eager_steps_and_autotune_keeps_state():
     from batch import SceneBatch, autotune
     specs = _specs()
@@ -132,3 +161,37 @@ def test_batched_particle_on_upper_face_keys_like_solo():
         if step == 0:   # the first build kept the z-face particle, dropped the x-face one
             for s, d in lost.items():
                 assert solos[s][0].particle_num[None] == n0[s] - d
+
+
+@pytest.mark.parametrize("path", ["eager", "graph"])
+def test_list_width_overflow_raises_and_keeps_the_scenes(path):
+    """A neighbour list wider than nbr_kmax: the eager step raises after the
+    list build, the graph step (one launch, device-side solves) after its
+    status readback with the state writers skipped on the device; either
+    way every scene keeps its state."""
+    from batch import SceneBatch
+    specs = _specs()
+    batch = SceneBatch(specs, far_field="fast")
+    batch.step()
+    # lists of the allocated width, read only up to 16 entries (~26 neighbours)
+    batch.ps.nbr_kmax = 16
+    batch.ps.params.nbr_kmax = 16
+    if path == "graph":
+        batch.capture()
+    def snapshot(s):
+        # the eager build may have re-sorted the slots: compare in x order
+        x, v, lt = (batch.scene_field(n, s).copy() for n in ("x", "v", "life_time"))
+        o = np.lexsort(x.T)
+        return x[o], v[o], lt[o]
+
+    before = [snapshot(s) for s in range(len(specs))]
+    with pytest.raises(RuntimeError, match="neighbour list width"):
+        if path == "graph":
+            batch.step_graph()
+        else:
+            batch.step()
+    for s in range(len(specs)):
+        for a, b, n in zip(before[s], snapshot(s), ("x", "v", "life_time")):
+            assert np.array_equal(a, b), (s, n)
+    if path == "graph":
+        batch.release_graph()
