@@ -9,7 +9,6 @@ circulations) must match the adjoint gradient to 1e-4 relative.  The scene
 is compressed so that the pressure solve acts, and the rollout spans two
 checkpoint segments.
 """
-import os
 
 import numpy as np
 import pytest
@@ -80,9 +79,10 @@ def test_rollout_adjoint_matches_central_differences():
     assert err < 1e-4, (err, ga, gf)
 
 
-@pytest.mark.skipif(os.environ.get("DIMCV_FULL_ADJOINT_CHECK") != "1",
-                    reason="full-size check (65,536 particles, 16 rings, 32 steps) runs on request")
 def test_full_size_rollout_adjoint_matches_central_differences():
+    """The adjoint workload at its full size (65,536 particles, 16 rings, 32
+    steps, checkpoint 8, M = 1024) against central differences of the FP64
+    rollout for two circulations (about 25 s on one GPU)."""
     from adjoint import RingRollout
     model = RingRollout(64, 32, 32, n_rings=16, steps=32, checkpoint=8, mc_samples=1024, seed=5, pad=2)
     model_err, err, ga, gf = check_gradient(model, components=[0, 7])
