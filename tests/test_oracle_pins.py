@@ -542,6 +542,17 @@ def test_smoothing_closed_forms():
     L.or_compute_all_d_vort_smoothed(s.sim)
     np.testing.assert_allclose(f32v(s, "d_vort_smoothed")[inner], (rho[:, None] / RHO0 * (D0 + x @ B.T))[inner],
                                rtol=1e-5)
+    # two particles with different dv: each adds the other's dv (not its own)
+    # weighted by W at their distance
+    rr = 0.7 * H
+    s, slot = make([[0.5, 0.5, 0.5], [0.5 + rr, 0.5, 0.5]])
+    d0, d1 = np.array([1.0, -2.0, 0.5]), np.array([-3.0, 0.25, 4.0])
+    s.field("d_vort")[slot[0]] = d0
+    s.field("d_vort")[slot[1]] = d1
+    L.or_compute_all_d_vort_smoothed(s.sim)
+    r01 = float(np.float32(0.5 + rr) - np.float32(0.5))
+    np.testing.assert_allclose(s.field("d_vort_smoothed")[slot[0]], MV * (W64(0.0) * d0 + W64(r01) * d1), rtol=2e-6)
+    np.testing.assert_allclose(s.field("d_vort_smoothed")[slot[1]], MV * (W64(0.0) * d1 + W64(r01) * d0), rtol=2e-6)
     # an isolated particle keeps only its own term
     s, slot = make([[0.5, 0.5, 0.5]])
     s.field("d_vort")[slot[0]] = [2.0, 0.0, -1.0]
@@ -734,10 +745,11 @@ def test_bounce_back_clamps_and_reflects():
                     [0.5, 0.5, pad],            # exactly on the lower pad: counts (<=)
                     [0.5, 0.5, hi],             # exactly on the upper bound: does not (>)
                     [0.99, 0.5, 0.97],          # upper x and z
-                    [0.02, 0.5, 0.5]])          # a solid beyond the face: untouched
+                    [0.02, 0.5, 0.5],           # a solid beyond the face: untouched
+                    [0.01, 0.99, 0.5]])         # lower x and upper y: normal (-1, 1, 0)
     vel = np.array([[1.0, 2.0, 3.0], [2.0, -1.0, 0.5], [-1.0, -3.0, 2.0], [0.3, 0.1, -2.0],
-                    [0.0, 0.0, 4.0], [1.0, 0.0, 1.0], [-5.0, 0.0, 0.0]])
-    mat = np.array([1, 1, 1, 1, 1, 1, 0])
+                    [0.0, 0.0, 4.0], [1.0, 0.0, 1.0], [-5.0, 0.0, 0.0], [-1.0, 2.0, 0.3]])
+    mat = np.array([1, 1, 1, 1, 1, 1, 0, 1])
     s, slot = make(pos, vel, material=mat, dynamic=mat)
     L.or_enforce_boundary_3d(s.sim)
     x = s.field("x")[slot].astype(np.float64)
@@ -759,6 +771,8 @@ def test_bounce_back_clamps_and_reflects():
     np.testing.assert_allclose(v[5], refl(vel[5], [1, 0, 1]), rtol=1e-6, atol=1e-7)
     np.testing.assert_array_equal(x[6], pos[6].astype(np.float32))
     np.testing.assert_array_equal(v[6], vel[6])
+    np.testing.assert_allclose(x[7], [f32(pad), f32(hi), 0.5], rtol=1e-7)
+    np.testing.assert_allclose(v[7], refl(vel[7], [-1, 1, 0]), rtol=1e-6, atol=1e-6)
 
 
 # ------------------------------------------------------------- B8 ---------
