@@ -188,6 +188,8 @@ def lib():
         "dimcv_neighbor_build": (I32, [P, P, I32, P]),
         "dimcv_compute_static_boundary_volume": (I32, [P, P, I32, P]),
         "dimcv_compute_moving_boundary_volume": (I32, [P, P, I32, P]),
+        "dimcv_rigid_moments": (I32, [P, P, I32, I32, P, P, I32, P, P]),
+        "dimcv_rigid_apply": (I32, [P, P, I32, P, I32, P]),
         "dimcv_multiply_time_step": (I32, [P, P, P, F32, I32, P]),
         "dimcv_compute_density_error": (I32, [P, P, F32, I32, P, P]),
         "dimcv_compute_non_pressure_forces": (I32, [P, P, I32, I32, P]),

@@ -574,6 +574,72 @@ int launch_rigid_reaction(const dimcv_state* st, const dimcv_params* p, int32_t 
   return DIMCV_OK;
 }
 
+// compute_com / solve_constraints (sph_base.py:183-222) of one dynamic rigid
+// object: FP64 moments over its particles, per block with block_tree_sum256
+// into part[b * 9 + c], then summed over blocks in block order (one block).
+// mode 0: (sum m, sum m x), m = m_V0 density (f32); mode 1: sum m p q^T
+// (row-major), p = x - cm, q = x_0 - rest_cm.
+__global__ void __launch_bounds__(256) k_rigid_moments(dimcv_state st, dimcv_params p, int32_t bound,
+                                                        int32_t obj, int mode, v3 cm, v3 rest,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < bound && i < st.counters[DIMCV_C_PARTICLE_NUM] && st.material[i] == 0 && st.is_dynamic[i] &&
+      st.object_id[i] == obj) {
+    const float m = p.m_V0 * st.density[i];
+    const v3 x = ld3(st.x, i);
+    if (mode == 0) {
+      c[0] = m;
+      c[1] = (double)m * x.x;
+      c[2] = (double)m * x.y;
+      c[3] = (double)m * x.z;
+    } else {
+      const v3 pp = vsub(x, cm), q = vsub(ld3(st.x_0, i), rest);
+      const float a[3] = {pp.x, pp.y, pp.z}, b[3] = {q.x, q.y, q.z};
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) c[3 * r + k] = (double)m * a[r] * b[k];
+    }
+  }
+  const int nc = mode == 0 ? 4 : 9;
+  for (int k = 0; k < nc; ++k) {
+    const double sum = block_tree_sum256(c[k], lds);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * 9 + k] = sum;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rigid_moments_sum(const double* __restrict__ part, int64_t nb,
+                                                            double* __restrict__ out) {
+  __shared__ double lds[256];
+  for (int k = 0; k < 9; ++k) {
+    double acc = 0.0;
+    for (int64_t b0 = 0; b0 < nb; b0 += 256) {
+      const int64_t b = b0 + threadIdx.x;
+      const double s = block_tree_sum256(b < nb ? part[b * 9 + k] : 0.0, lds);
+      acc += s;  // thread 0's running total over groups, in block order
+    }
+    if (threadIdx.x == 0) out[k] = acc;
+  }
+}
+
+// sph_base.py:215-220: x += (goal - x) * 1.0, goal = cm + R (x_0 - rest_cm)
+// (each component of R q summed over k = 0, 1, 2 in f32).
+struct RigidT {
+  float cm[3], R[9], rest[3];
+};
+__global__ void k_rigid_apply(dimcv_state st, int32_t bound, int32_t obj, RigidT t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bound || i >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
+  if (st.material[i] != 0 || !st.is_dynamic[i] || st.object_id[i] != obj) return;
+  const v3 q = vsub(ld3(st.x_0, i), mk(t.rest[0], t.rest[1], t.rest[2]));
+  const v3 rq = mk(t.R[0] * q.x + t.R[1] * q.y + t.R[2] * q.z, t.R[3] * q.x + t.R[4] * q.y + t.R[5] * q.z,
+                   t.R[6] * q.x + t.R[7] * q.y + t.R[8] * q.z);
+  const v3 goal = vadd(mk(t.cm[0], t.cm[1], t.cm[2]), rq);
+  const v3 x = ld3(st.x, i);
+  const v3 corr = smul(1.0f, vsub(goal, x));
+  st3(st.x, i, vadd(x, corr));
+}
+
 // sph_base.py:91-112: Akinci boundary volume over solid neighbours.
 __global__ void k_boundary_volume(dimcv_state st, dimcv_params p, int32_t bound, int dynamic_sel) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -920,6 +986,55 @@ int dimcv_compute_moving_boundary_volume(const dimcv_state* st, const dimcv_para
     hipLaunchKernelGGL(k_boundary_volume, dim3(nblocks(particle_bound)), dim3(256), 0, (hipStream_t)stream,
                        *st, *p, particle_bound, 1);
   DIMCV_LAUNCHED("k_boundary_volume");
+  return DIMCV_OK;
+}
+
+int dimcv_rigid_moments(const dimcv_state* st, const dimcv_params* p, int32_t object_id, int32_t mode,
+                        const float* cm, const float* rest_cm, int32_t particle_bound, double* out, void* stream) {
+  if (!st || !p || !out || (mode != 0 && mode != 1) || (mode == 1 && (!cm || !rest_cm))) {
+    set_error("dimcv_rigid_moments: bad arguments");
+    return DIMCV_ERR_ARG;
+  }
+  for (int k = 0; k < 9; ++k) out[k] = 0.0;
+  if (particle_bound <= 0) return DIMCV_OK;
+  Workspace ws;
+  int rc = bind_workspace(st, p, &ws, 0);
+  if (rc) return rc;
+  const int64_t nb = nblocks(particle_bound);
+  // block partials (9 doubles per block) and the result, in the far-field
+  // partial region (at least min(16 n_max, 2^21) x 32 bytes)
+  if ((nb * 9 + 9) * 8 > ws.far_partial_cap * 32) {
+    set_error("dimcv_rigid_moments: workspace too small");
+    return DIMCV_ERR_WORKSPACE;
+  }
+  double* part = (double*)ws.far_partials;
+  double* dout = part + nb * 9;
+  const v3 c = mode == 1 ? v3{cm[0], cm[1], cm[2]} : v3{0.0f, 0.0f, 0.0f};
+  const v3 r = mode == 1 ? v3{rest_cm[0], rest_cm[1], rest_cm[2]} : v3{0.0f, 0.0f, 0.0f};
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_rigid_moments, dim3((unsigned)nb), dim3(256), 0, s, *st, *p, particle_bound, object_id, mode,
+                     c, r, part);
+  hipLaunchKernelGGL(k_rigid_moments_sum, dim3(1), dim3(256), 0, s, (const double*)part, nb, dout);
+  DIMCV_LAUNCHED("k_rigid_moments");
+  DIMCV_HIP(hipMemcpyAsync(out, dout, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipStreamSynchronize(s));
+  return DIMCV_OK;
+}
+
+int dimcv_rigid_apply(const dimcv_state* st, const dimcv_params* p, int32_t object_id, const float* T,
+                      int32_t particle_bound, void* stream) {
+  if (!st || !p || !T) {
+    set_error("dimcv_rigid_apply: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (particle_bound <= 0) return DIMCV_OK;
+  RigidT t;
+  for (int k = 0; k < 3; ++k) t.cm[k] = T[k];
+  for (int k = 0; k < 9; ++k) t.R[k] = T[3 + k];
+  for (int k = 0; k < 3; ++k) t.rest[k] = T[12 + k];
+  hipLaunchKernelGGL(k_rigid_apply, dim3(nblocks(particle_bound)), dim3(256), 0, (hipStream_t)stream, *st,
+                     particle_bound, object_id, t);
+  DIMCV_LAUNCHED("k_rigid_apply");
   return DIMCV_OK;
 }
 

@@ -102,6 +102,10 @@ class ParticleSystem:
 
         self.particle_max_num = fluid_particle_num + rigid_particle_num
         self.num_rigid_bodies = len(rigid_blocks)
+        if self.num_rigid_bodies > 0:
+            # rest centres of mass of the shape matching (particle_system.py:126-132), host side
+            self.rigid_rest_cm = np.zeros((self.num_rigid_bodies + len(fluid_blocks) + len(self.fluid_emitters), 3),
+                                          dtype=np.float32)
         self.reserved_capacity = self.cfg.get_cfg("reservedCapacity")
         if self.reserved_capacity is not None:
             self.particle_max_num += self.reserved_capacity

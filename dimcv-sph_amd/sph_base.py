@@ -10,6 +10,8 @@ Shape matching of dynamic rigid bodies (:87-89, :182-222) is not supported.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 import _native as N
@@ -80,6 +82,48 @@ class SPHBase:
         # sph_base.py:105-112, over the dynamic rigid particles (none: no work)
         if self.ps.has_dynamic_rigid:
             self.ps.call("dimcv_compute_moving_boundary_volume", self.ps._pn, self.ps.stream())
+
+    # ------------------------------------------------- rigid shape matching --
+    def compute_com(self, object_id):
+        """sph_base.py:183-194: centre of mass of object_id's dynamic rigid
+        particles, m = m_V0 density; FP64 sums in one fixed order (the
+        reference's Taichi reduction order is unspecified), rounded to f32."""
+        out = (ctypes.c_double * 9)()
+        self.ps.call("dimcv_rigid_moments", int(object_id), 0, None, None, self.ps._pn, out, self.ps.stream())
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return (np.array(out[1:4]) / out[0]).astype(np.float32)   # 0/0 -> nan, as the reference
+
+    def compute_com_kernel(self, object_id: int):
+        return self.compute_com(object_id)
+
+    def compute_rigid_rest_cm(self, object_id: int):
+        """sph_base.py:83-85 (rest centre of mass, from the current positions)."""
+        self.ps.rigid_rest_cm[object_id] = self.compute_com(object_id)
+
+    def solve_constraints(self, object_id: int):
+        """sph_base.py:200-222: shape matching of object_id.  A = sum m p q^T
+        (FP64, see compute_com); R = the rotation of A's polar decomposition
+        (SVD with both factors turned into rotations, so det R = +1; the
+        identity when every entry is below 1e-6); every particle moves to
+        cm + R (x_0 - rest_cm).  Returns R (f32)."""
+        cm = self.compute_com(object_id)
+        rest = np.asarray(self.ps.rigid_rest_cm[object_id], dtype=np.float32)
+        out = (ctypes.c_double * 9)()
+        c3, r3 = (ctypes.c_float * 3)(*cm), (ctypes.c_float * 3)(*rest)
+        self.ps.call("dimcv_rigid_moments", int(object_id), 1, c3, r3, self.ps._pn, out, self.ps.stream())
+        A = np.array(out[:9], dtype=np.float64).reshape(3, 3)
+        U, sig, Vt = np.linalg.svd(A)
+        if np.linalg.det(U) < 0:
+            U[:, 2] = -U[:, 2]
+        if np.linalg.det(Vt) < 0:
+            Vt[2, :] = -Vt[2, :]
+        R = (U @ Vt).astype(np.float32)
+        if np.all(np.abs(R) < 1e-6):
+            R = np.eye(3, dtype=np.float32)
+        T = (ctypes.c_float * 15)(*np.concatenate([cm, R.ravel(), rest]).astype(np.float32))
+        self.ps.call("dimcv_rigid_apply", int(object_id), T, self.ps._pn, self.ps.stream())
+        self.ps._nbr_valid = False
+        return R
 
     def substep(self):
         pass

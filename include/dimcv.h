@@ -355,6 +355,23 @@ int dimcv_compute_static_boundary_volume(const dimcv_state* st, const dimcv_para
 int dimcv_compute_moving_boundary_volume(const dimcv_state* st, const dimcv_params* p,
                                          int32_t particle_bound, void* stream);
 
+/* Shape matching of a dynamic rigid object (sph_base.py:183-222,
+ * compute_com and solve_constraints; the reference's step never calls
+ * them).  dimcv_rigid_moments: FP64 moments over the object's dynamic rigid
+ * particles, summed per block and then over blocks in block order (one
+ * fixed order, where the reference's Taichi reduction order is unspecified);
+ * mode 0: out[0] = sum m, out[1..3] = sum m x, m = f32(m_V0 * density);
+ * mode 1: out[0..8] = sum m (x - cm)(x_0 - rest_cm)^T, row-major, cm and
+ * rest_cm host float[3].  out: host double[9]; uses the workspace's far-field
+ * partial region; synchronises `stream`.
+ * dimcv_rigid_apply: x += (goal - x) * 1, goal = cm + R (x_0 - rest_cm), for
+ * the same particles; T: host float[15] = cm[3], R[9] row-major, rest_cm[3]. */
+int dimcv_rigid_moments(const dimcv_state* st, const dimcv_params* p, int32_t object_id, int32_t mode,
+                        const float* cm, const float* rest_cm, int32_t particle_bound, double* out,
+                        void* stream);
+int dimcv_rigid_apply(const dimcv_state* st, const dimcv_params* p, int32_t object_id, const float* T,
+                      int32_t particle_bound, void* stream);
+
 /* ---- SPH / DFSPH passes (dimcv_sph.py).  `fluid_bound` / `particle_bound` are
  * host-side upper bounds of the fluid-list length / particle count used to
  * size the launch; the kernels read the exact counts from counters[]. ------- */

@@ -58,3 +58,45 @@ def test_dynamic_rigid_block_bitexact(mc):
         acc = ref.field("acceleration")[:n][rig]
         reacted += int((np.abs(acc[:, 2] + np.float32(9.8)) > 1e-3).sum())
     assert reacted > 100    # the reactions acted on many rigid particles
+
+
+def test_shape_matching_recovers_the_rigid_transform():
+    """compute_com / solve_constraints (sph_base.py:183-222) on a cubic
+    dynamic block: moved to c + Rot S q (q = x_0 - rest_cm, S a symmetric
+    stretch), its centre of mass is c (the lattice's first moment vanishes),
+    A = sum m p q^T = Rot S Q with Q isotropic, so the polar rotation is Rot
+    and every particle returns to c + Rot q.  The rest centres are indexed by
+    object id over rigid + fluid objects (particle_system.py:126-132), so
+    the block takes id 1 after the fluid block's 0."""
+    from config_builder import SimConfig
+    from particle_system import ParticleSystem
+    scene = rigid_drop_scene()
+    scene["RigidBlocks"] = [{"objectId": 1, "start": [0.2, 0.2, 0.2], "end": [0.3, 0.3, 0.3],
+                             "translation": [0, 0, 0], "scale": [1, 1, 1], "velocity": [0, 0, 0],
+                             "density": 500.0, "color": [200, 50, 50], "isDynamic": True}]
+    ps = ParticleSystem(SimConfig(scene=scene))
+    solver = ps.build_solver()
+    solver.initialize()
+    solver.compute_rigid_rest_cm(1)
+    n = ps.particle_num[None]
+    obj = ps.object_id.to_numpy()[:n] == 1
+    assert obj.sum() == 125
+    x0 = ps.x_0.to_numpy()[:n][obj].astype(np.float64)
+    rest = ps.rigid_rest_cm[1].astype(np.float64)
+    np.testing.assert_allclose(rest, x0.mean(axis=0), rtol=1e-6)
+    np.testing.assert_allclose(rest, [0.24, 0.24, 0.24], rtol=1e-5)
+    th = 0.3
+    rot = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    stretch = np.diag([1.08, 0.95, 1.02])
+    c = np.array([0.31, 0.27, 0.22])
+    q = x0 - rest
+    x = ps.x.to_numpy()
+    x[:n][obj] = (c + q @ (rot @ stretch).T).astype(np.float32)
+    ps.x.from_numpy(x)
+    np.testing.assert_allclose(solver.compute_com(1), c, rtol=2e-6)
+    R = solver.solve_constraints(1)
+    np.testing.assert_allclose(R, rot, atol=2e-6)
+    got = ps.x.to_numpy()[:n][obj].astype(np.float64)
+    np.testing.assert_allclose(got, c + q @ rot.T, atol=3e-6)
+    # the other particles are untouched
+    np.testing.assert_array_equal(ps.x.to_numpy()[:n][~obj], x[:n][~obj])
