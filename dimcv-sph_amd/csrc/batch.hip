@@ -50,7 +50,7 @@ int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, 
 
 static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
-int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs* bw, void* base) {
+int64_t batch_layout(int64_t n_scenes, int64_t n_max, BatchWs* bw, void* base) {
   char* b = (char*)base;
   int64_t off = 0;
   auto take = [&](int64_t bytes) -> void* {
@@ -68,8 +68,6 @@ int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs*
   w.misc = (int32_t*)take(4 * BM_COUNT);
   w.btot = (double*)take(8 * w.tile_bound);
   w.bpre = (double*)take(8 * w.tile_bound);
-  w.part_cap = part_cap > 0 ? part_cap : 1;
-  w.part = (FarPartial*)take((int64_t)sizeof(FarPartial) * w.part_cap);
   w.bytes = off;
   if (bw) *bw = w;
   return off;
@@ -85,16 +83,13 @@ int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw) {
     set_error("batched scenes are open boxes without vortex domain, transit area or emitters");
     return DIMCV_ERR_ARG;
   }
-  const int64_t need0 = batch_layout(st->n_scenes, p->n_max, 1, nullptr, nullptr);
-  if (st->batch_workspace_bytes < need0) {
+  const int64_t need = batch_layout(st->n_scenes, p->n_max, nullptr, nullptr);
+  if (st->batch_workspace_bytes < need) {
     set_error("batch workspace too small: %lld < %lld bytes", (long long)st->batch_workspace_bytes,
-              (long long)need0);
+              (long long)need);
     return DIMCV_ERR_WORKSPACE;
   }
-  // the partials are the last region
-  const int64_t cap = (st->batch_workspace_bytes - (need0 - align256((int64_t)sizeof(FarPartial)))) /
-                      (int64_t)sizeof(FarPartial);
-  batch_layout(st->n_scenes, p->n_max, cap, bw, st->batch_workspace);
+  batch_layout(st->n_scenes, p->n_max, bw, st->batch_workspace);
   return DIMCV_OK;
 }
 
@@ -136,21 +131,20 @@ __global__ void __launch_bounds__(256) k_batch_scene_setup(dimcv_state st, dimcv
     c[DIMCV_SC_SPLITS] = far_split_count_mc(hi - lo);
   }
   __syncthreads();
-  // far-field order: larger per-item source chunks first, then scene index
+  // far-field order: larger per-item cost (the scene's sample count) first,
+  // then scene index
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
-    const int64_t cost = (st.scenes[s].m + c[DIMCV_SC_SPLITS] - 1) / c[DIMCV_SC_SPLITS];
+    const int64_t cost = st.scenes[s].m;
     int rank = 0;
     for (int t = 0; t < S; ++t) {
-      const int32_t* ct = st.scene_counters + (int64_t)t * DIMCV_SC_COUNT;
-      const int64_t cost_t = (st.scenes[t].m + ct[DIMCV_SC_SPLITS] - 1) / ct[DIMCV_SC_SPLITS];
+      const int64_t cost_t = st.scenes[t].m;
       rank += (cost_t > cost || (cost_t == cost && t < s)) ? 1 : 0;
     }
     bw.order[rank] = s;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  int64_t fs = 0, tiles = 0, part = 0;
+  int64_t fs = 0, tiles = 0;
   int err = 0;
   for (int s = 0; s < S; ++s) {
     int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
@@ -159,17 +153,7 @@ __global__ void __launch_bounds__(256) k_batch_scene_setup(dimcv_state st, dimcv
     fs += (F + 255) & ~int64_t(255);
     c[DIMCV_SC_TILE_START] = (int32_t)tiles;
     tiles += (F + DIMCV_SCAN_TILE - 1) / DIMCV_SCAN_TILE;
-    c[DIMCV_SC_PART_BASE] = 0;
-    if (c[DIMCV_SC_SPLITS] > 1) {
-      const int64_t need = (int64_t)c[DIMCV_SC_SPLITS] * F;
-      if (part + need > bw.part_cap) {
-        c[DIMCV_SC_SPLITS] = 1;
-        err |= 2;
-      } else {
-        c[DIMCV_SC_PART_BASE] = (int32_t)part;
-        part += need;
-      }
-    }
+    c[DIMCV_SC_PART_BASE] = 0;  // the batched far field keeps no partial sums
   }
   if (fs > p.n_max) err |= 1;
   int64_t items = 0;
@@ -178,7 +162,7 @@ __global__ void __launch_bounds__(256) k_batch_scene_setup(dimcv_state st, dimcv
     int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
     bw.item_start[r] = (int32_t)items;
     c[DIMCV_SC_ITEM_START] = (int32_t)items;
-    items += ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256 * c[DIMCV_SC_SPLITS];
+    items += ((int64_t)c[DIMCV_SC_FLUID] + 255) / 256;
   }
   bw.item_start[S] = (int32_t)items;
   bw.misc[BM_ITEMS] = (int32_t)items;
@@ -380,7 +364,7 @@ static int check_counters(const int32_t* hc, const int32_t* misc, const dimcv_pa
     return DIMCV_ERR_NBR;
   }
   if (misc && misc[BM_ERROR]) {
-    set_error("batch capacity exceeded (flags %d: 1 fluid list, 2 far-field partials)", misc[BM_ERROR]);
+    set_error("batch capacity exceeded (flags %d: 1 fluid list)", misc[BM_ERROR]);
     return DIMCV_ERR_RANGE;
   }
   if (misc && misc[BM_BAR + 2]) {
@@ -410,12 +394,7 @@ extern "C" {
 
 int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_particles, int64_t n_max) {
   if (n_scenes <= 0 || !scene_particles) return -1;
-  int64_t cap = 0;
-  for (int32_t s = 0; s < n_scenes; ++s) {
-    const int64_t n = scene_particles[s] > 0 ? scene_particles[s] : 0;
-    cap += 16 * n < DIMCV_FAR_PARTIAL_MAX ? 16 * n : DIMCV_FAR_PARTIAL_MAX;
-  }
-  return batch_layout(n_scenes, n_max, cap, nullptr, nullptr);
+  return batch_layout(n_scenes, n_max, nullptr, nullptr);
 }
 
 int dimcv_batch_set_launch(int32_t far_workgroups, int32_t nbr_block) {

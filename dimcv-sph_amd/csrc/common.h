@@ -589,13 +589,13 @@ struct FarPartial {
 };
 
 // Batch workspace (dimcv_batch_workspace_bytes): unpadded fluid list, far
-// field work order, misc counters, far-field partials (last region).
+// field work order, misc counters, CDF tile totals.
 enum BatchMisc {
   BM_UCOUNT = 0,       // unpadded fluid-list length
   BM_ITEMS = 1,        // far-field work items
   BM_QUEUE = 2,        // far-field work queue head
   BM_TILES = 3,        // CDF scan tiles
-  BM_ERROR = 4,        // error flags (1: fluid list, 2: far partials, 4: Jacobi barrier timed out)
+  BM_ERROR = 4,        // capacity error flags (1: fluid list)
   BM_BAR = 5,          // persistent Jacobi solve: barrier arrivals, generation, abort, two go flags
   BM_COUNT = 16
 };
@@ -607,8 +607,6 @@ struct BatchWs {
   double* btot;         // [tile_bound]: CDF tile totals
   double* bpre;         // [tile_bound]: their per-scene scans
   int64_t tile_bound;   // n_max / 4096 + n_scenes + 1
-  FarPartial* part;     // [part_cap]
-  int64_t part_cap;
   int64_t bytes;
 };
 
@@ -622,7 +620,7 @@ __device__ __forceinline__ int last_le(const int32_t* __restrict__ a, int n, int
   }
   return lo - 1;
 }
-int64_t batch_layout(int64_t n_scenes, int64_t n_max, int64_t part_cap, BatchWs* bw, void* base);
+int64_t batch_layout(int64_t n_scenes, int64_t n_max, BatchWs* bw, void* base);
 int bind_batch(const dimcv_state* st, const dimcv_params* p, BatchWs* bw);
 
 // the pressure update's reaction on dynamic rigid particles (grid.hip); no

@@ -776,17 +776,18 @@ __global__ void k_batch_cdf_draw(dimcv_state st, const double* __restrict__ L, c
 }
 
 // Persistent far field over a work queue.  Work item q = (scene, 256-target
-// block, source split) in the order set up by the scene set-up (largest
-// per-item cost first); a workgroup takes the next item until none is left.
-// A target's sum is the one of k_vv_far_fast (splits = 1) or k_far_split
-// (splits > 1) for the scene alone: the same 64-lane waves over the scene's
-// fluid entries and the same source ranges.
+// block) in the order set up by the scene set-up (largest sample count
+// first); a workgroup takes the next item until none is left.  A target's
+// sum is the one of the scene run alone: k_vv_far_fast (splits = 1), or
+// k_far_split + k_far_combine (splits > 1), whose source ranges are summed
+// here one after another in the same 64-lane waves and added in split order
+// in registers.  A batch fills the GPU with its scenes' target blocks, so it
+// needs no partial sums and no combine pass.
 template <bool KAHAN>
 __global__ void __launch_bounds__(256) k_batch_far(dimcv_state st, dimcv_params p, const float* __restrict__ near3,
                                                    const float* __restrict__ scale_s, const float4* __restrict__ src,
                                                    int32_t* __restrict__ misc, const int32_t* __restrict__ order,
-                                                   const int32_t* __restrict__ item_start,
-                                                   FarPartial* __restrict__ part) {
+                                                   const int32_t* __restrict__ item_start) {
   __shared__ int q_sh;
   const int total = __builtin_amdgcn_readfirstlane(misc[BM_ITEMS]);
   const float T = p.far_r2_max_rejected;
@@ -803,9 +804,8 @@ __global__ void __launch_bounds__(256) k_batch_far(dimcv_state st, dimcv_params 
     const int r = last_le(item_start, st.n_scenes, 1, q);
     const int s = order[r];
     const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
-    const int local = q - item_start[r];
+    const int tb = q - item_start[r];
     const int splits = c[DIMCV_SC_SPLITS];
-    const int tb = local / splits, k = local - tb * splits;
     const int F = c[DIMCV_SC_FLUID];
     const int lf = tb * 256 + (int)threadIdx.x;
     if (lf < F) {
@@ -821,56 +821,32 @@ __global__ void __launch_bounds__(256) k_batch_far(dimcv_state st, dimcv_params 
           far_sum<KAHAN, false>(sp, M, xi, T, L0, acc, added);
           vv_finish(st, p, i, ld3(near3, f), vmuls(acc, scale_s[f]), added);
         } else {
+          // the solo run's source splits one after another, each summed
+          // from fresh accumulators, then added in split order exactly as
+          // k_far_combine adds the stored partials
           const int chunk = (M + splits - 1) / splits;
-          const int j0 = min(k * chunk, M), j1 = min(j0 + chunk, M);
           float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
-          int rej = 0;
-          far_sum_range<KAHAN, false>(sp, j0, j1, xi, T, L0, A, C, rej);
-          FarPartial& o = part[(int64_t)c[DIMCV_SC_PART_BASE] + (int64_t)k * F + lf];
-          for (int d = 0; d < 3; ++d) {
-            o.a[d] = A[d];
-            o.c[d] = C[d];
+          int rej_all = 0;
+          for (int k = 0; k < splits; ++k) {
+            const int j0 = min(k * chunk, M), j1 = min(j0 + chunk, M);
+            float a[3] = {0.0f, 0.0f, 0.0f}, cc[3] = {0.0f, 0.0f, 0.0f};
+            int rej = 0;
+            far_sum_range<KAHAN, false>(sp, j0, j1, xi, T, L0, a, cc, rej);
+            for (int d = 0; d < 3; ++d) {
+              if (KAHAN) {
+                acc_add<true>(A[d], C[d], a[d]);
+                acc_add<true>(A[d], C[d], -cc[d]);
+              } else {
+                A[d] += a[d];
+              }
+            }
+            rej_all += rej;
           }
-          o.rej = rej;
+          vv_finish(st, p, i, ld3(near3, f), vmuls(mk(A[0], A[1], A[2]), scale_s[f]), M - rej_all);
         }
       }
     }
   }
-}
-
-// k_far_combine for the split scenes of a batch.
-template <bool KAHAN>
-__global__ void k_batch_far_combine(dimcv_state st, dimcv_params p, int64_t bound, const FarPartial* __restrict__ part,
-                                    const float* __restrict__ near3, const float* __restrict__ scale_s) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= bound || q >= st.counters[DIMCV_C_FLUID_LIST]) return;
-  const int s = st.block_scene[q >> 8];
-  if (s < 0) return;
-  const int32_t* c = st.scene_counters + (int64_t)s * DIMCV_SC_COUNT;
-  const int splits = c[DIMCV_SC_SPLITS];
-  if (splits == 1) return;
-  const int64_t F = c[DIMCV_SC_FLUID];
-  const int64_t lf = q - c[DIMCV_SC_FLUID_START];
-  if (lf >= F) return;
-  const int64_t i = st.fluid_list[q];
-  const v3 xi = ld3(st.x, i);
-  if (!in_vortex_domain(p, xi)) return;
-  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
-  int rej = 0;
-  for (int s2 = 0; s2 < splits; ++s2) {
-    const FarPartial& o = part[(int64_t)c[DIMCV_SC_PART_BASE] + (int64_t)s2 * F + lf];
-    for (int d = 0; d < 3; ++d) {
-      if (KAHAN) {
-        acc_add<true>(A[d], C[d], o.a[d]);
-        acc_add<true>(A[d], C[d], -o.c[d]);
-      } else {
-        A[d] += o.a[d];
-      }
-    }
-    rej += o.rej;
-  }
-  const int M = c[DIMCV_SC_NUM_SAMPLES];
-  vv_finish(st, p, i, ld3(near3, q), vmuls(mk(A[0], A[1], A[2]), scale_s[q]), M - rej);
 }
 
 static int g_far_wgs = 2048;
@@ -901,11 +877,8 @@ int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspac
   } else {
     const bool kahan = p->far_mode == DIMCV_FAR_KAHAN;
     auto far = kahan ? k_batch_far<true> : k_batch_far<false>;
-    auto comb = kahan ? k_batch_far_combine<true> : k_batch_far_combine<false>;
     hipLaunchKernelGGL(far, dim3((unsigned)g_far_wgs), dim3(256), 0, s, *st, *p, ws.near3, ws.scale_s, ws.samples,
-                       bw.misc, bw.order, bw.item_start, bw.part);
-    hipLaunchKernelGGL(comb, dim3(nblocks(n)), dim3(256), 0, s, *st, *p, n, (const FarPartial*)bw.part, ws.near3,
-                       ws.scale_s);
+                       bw.misc, bw.order, bw.item_start);
   }
   DIMCV_LAUNCHED("batched vort2vel");
   return DIMCV_OK;

@@ -587,7 +587,8 @@ int dimcv_segment_rk4(const dimcv_segments* sg, const float* v_minus, const floa
  *
  * Capacities: n_max >= total particles + 256 * n_scenes; block_scene holds
  * n_max / 256 + 1 entries; n_cells (params) = sum of scene cells.
- * scene_particles: host array of the scenes' particle counts (upper bounds). */
+ * scene_particles: host array of the scenes' particle counts (must be given;
+ * the layout depends only on n_scenes and n_max). */
 int64_t dimcv_batch_workspace_bytes(int32_t n_scenes, const int32_t* scene_particles, int64_t n_max);
 /* Grid build of every scene (activity, keys, sort, cell table, gather into
  * the *_buffer fields including scene_id, padded fluid list, per-scene
