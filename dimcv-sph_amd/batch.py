@@ -14,9 +14,11 @@ each has its own box, viscosity, ring set, seed and sample count M (default
 2% of its particles).  Per-scene uniforms come from ``numpy`` generators
 seeded with the scene seed, as the solo solver's ``rngSeed`` generator does.
 
-A step runs eagerly (``step``) or from HIP graphs (``capture`` then
-``step_graph``): the step is captured as five graphs separated by the host's
-Jacobi convergence checks, for both parities of the sorted-buffer swap.
+A step runs eagerly (``step``) or from a HIP graph (``capture`` then
+``step_graph``): the Jacobi solves converge on the device (a persistent
+launch tests every scene and iterates the ones that have not converged), so
+the whole step is captured as one graph per parity of the sorted-buffer swap
+and a step is one graph launch plus one status readback.
 ``autotune`` picks the far-field persistent grid size and the neighbour-build
 workgroup size; both only change launch shapes, never results.
 """
@@ -91,7 +93,7 @@ def solo_system(spec: RingSpec, far_field: str = "fast", r: float = 0.01, dt: fl
 class SceneBatch:
 
     def __init__(self, specs, far_field: str = "fast", r: float = 0.01, dt: float = 2e-4,
-                 device: str = "cuda", jacobi_chunk: int = 2):
+                 device: str = "cuda"):
         from config_builder import SimConfig
         from particle_system import ParticleSystem
 
@@ -101,7 +103,6 @@ class SceneBatch:
             raise ValueError("a batch needs at least one scene")
         self.n_scenes = S
         self.far_field = far_field
-        self.jacobi_chunk = int(jacobi_chunk)
         infos = [make_ring_scene(sp.nx, sp.ny, sp.nz, n_rings=sp.n_rings, r=r, seed=sp.seed,
                                  mc_samples=sp.m, far_field=far_field, dt=dt, viscosity=sp.viscosity,
                                  trim=sp.trim, device=device) for sp in self.specs]
@@ -245,7 +246,6 @@ class SceneBatch:
         o.u_cdf, o.m = u_ptr, self.m_total
         o.max_iter_v = sv.m_max_iterations_v
         o.max_iter_p = sv.m_max_iterations
-        o.chunk = self.jacobi_chunk
         o.eta_v = sv.inv_dt * sv.max_error_V * 0.01 * sv.density_0
         o.eta_p = sv.max_error * 0.01 * sv.density_0
         o.inv_dt = np.float32(sv.inv_dt)
