@@ -469,9 +469,9 @@ __host__ __device__ __forceinline__ int far_split_count(int64_t targets) {
 // scene sums in the order of a solo one): from 2^19 targets, twice the waves
 // of far_split_count (two halves at 1M targets run 1.16 ms against 1.22 ms
 // for one pass, the tail of the last round of waves being the difference),
-// clamped so that splits * targets <= 2^21, the partial capacity both steps
-// reserve (the solo workspace min(16 n_max, 2^21), a batched scene min(16 n_s,
-// 2^21)).
+// clamped so that splits * targets <= 2^21, the partial capacity of the solo
+// workspace (min(16 n_max, 2^21)); the batched step sums a scene's splits in
+// registers and keeps no partials.
 #define DIMCV_FAR_PARTIAL_MAX (1LL << 21)
 __host__ __device__ __forceinline__ int far_split_count_mc(int64_t targets) {
   int64_t s = far_split_count(targets >= (1 << 19) ? (targets + 1) / 2 : targets);
