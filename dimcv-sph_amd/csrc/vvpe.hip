@@ -290,35 +290,48 @@ __device__ __forceinline__ void far_sum(const float4* __restrict__ src, int M, v
   added = M - rej;
 }
 
-// Split form for small target counts: workgroup (b, s) sums sources
-// [s*chunk, (s+1)*chunk) for targets of block b and stores the partial
-// (A, compensation, rejections); k_far_combine adds the partials in split
-// order.
+// Split form: workgroup (b, s) sums sources [s*chunk, (s+1)*chunk) for the
+// targets of block b and stores the partial: (A, rejections) as one float4
+// (naive sums), or (A, compensation, rejections) as a FarPartial (Kahan).
+// Split 0 marks a target outside the vortex domain (rejections -1), so that
+// k_far_combine does not read its position again.  k_far_combine adds the
+// partials in split order.
+template <bool KAHAN>
+__device__ __forceinline__ void far_put_partial(void* part, int64_t idx, const float* A, const float* C, int rej) {
+  if (KAHAN) {
+    FarPartial& o = ((FarPartial*)part)[idx];
+    for (int d = 0; d < 3; ++d) {
+      o.a[d] = A[d];
+      o.c[d] = C[d];
+    }
+    o.rej = rej;
+  } else {
+    ((float4*)part)[idx] = make_float4(A[0], A[1], A[2], __int_as_float(rej));
+  }
+}
 
 template <bool KAHAN, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params p, int32_t bound,
                                                    const float4* __restrict__ src, const int32_t* __restrict__ msrc,
-                                                   int cap, int splits, FarPartial* __restrict__ part) {
+                                                   int cap, int splits, void* __restrict__ part) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
   v3 xi = ld3(st.x, i);
-  if (!in_vortex_domain(p, xi)) return;
+  const int sidx = blockIdx.y;
+  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
+  if (!in_vortex_domain(p, xi)) {
+    if (sidx == 0) far_put_partial<KAHAN>(part, f, A, C, -1);
+    return;
+  }
   const int M = min(*msrc, cap);
   const int chunk = (M + splits - 1) / splits;
-  const int sidx = blockIdx.y;
   const int j0 = min(sidx * chunk, M), j1 = min(j0 + chunk, M);
-  float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
   int rej = 0;
   far_sum_range<KAHAN, PERIODIC>(src, j0, j1, xi, p.far_r2_max_rejected,
                                  mk(p.box_size[0], p.box_size[1], p.box_size[2]), A, C, rej);
-  FarPartial& o = part[(int64_t)sidx * bound + f];
-  for (int d = 0; d < 3; ++d) {
-    o.a[d] = A[d];
-    o.c[d] = C[d];
-  }
-  o.rej = rej;
+  far_put_partial<KAHAN>(part, (int64_t)sidx * bound + f, A, C, rej);
 }
 
 // Combine the partials: naive sums in split order, or Kahan sums carrying
@@ -326,27 +339,42 @@ __global__ void __launch_bounds__(256) k_far_split(dimcv_state st, dimcv_params 
 // v); mode 1: direct sum into v_vort.
 template <bool KAHAN>
 __global__ void k_far_combine(dimcv_state st, dimcv_params p, int32_t bound, const int32_t* __restrict__ msrc, int cap,
-                              int splits, const FarPartial* __restrict__ part, const float* __restrict__ near3,
+                              int splits, const void* __restrict__ part, const float* __restrict__ near3,
                               const float* __restrict__ scale_s, int mode) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
-  v3 xi = ld3(st.x, i);
-  if (!in_vortex_domain(p, xi)) return;
   float A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f};
   int rej = 0;
   for (int s2 = 0; s2 < splits; ++s2) {
-    const FarPartial& q = part[(int64_t)s2 * bound + f];
+    const int64_t idx = (int64_t)s2 * bound + f;
+    float a[3], c[3] = {0.0f, 0.0f, 0.0f};
+    int r;
+    if (KAHAN) {
+      const FarPartial& q = ((const FarPartial*)part)[idx];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = q.a[d];
+        c[d] = q.c[d];
+      }
+      r = q.rej;
+    } else {
+      const float4 q = ((const float4*)part)[idx];
+      a[0] = q.x;
+      a[1] = q.y;
+      a[2] = q.z;
+      r = __float_as_int(q.w);
+    }
+    if (r < 0) return;  // outside the vortex domain (split 0's mark)
     for (int d = 0; d < 3; ++d) {
       if (KAHAN) {
-        acc_add<true>(A[d], C[d], q.a[d]);
-        acc_add<true>(A[d], C[d], -q.c[d]);
+        acc_add<true>(A[d], C[d], a[d]);
+        acc_add<true>(A[d], C[d], -c[d]);
       } else {
-        A[d] += q.a[d];
+        A[d] += a[d];
       }
     }
-    rej += q.rej;
+    rej += r;
   }
   const int M = min(*msrc, cap);
   v3 acc = mk(A[0], A[1], A[2]);
@@ -664,14 +692,14 @@ static int far_splits_mc(int64_t targets, const Workspace& ws) {
 static int launch_far_split(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, const float4* src,
                             const int32_t* msrc, int cap, int splits, bool kahan, Workspace& ws, int mode,
                             hipStream_t s) {
-  FarPartial* part = (FarPartial*)ws.far_partials;
+  void* part = ws.far_partials;
   dim3 g(nblocks(fluid_bound), splits);
   auto kern = kahan ? (p->periodic ? k_far_split<true, true> : k_far_split<true, false>)
                     : (p->periodic ? k_far_split<false, true> : k_far_split<false, false>);
   hipLaunchKernelGGL(kern, g, dim3(256), 0, s, *st, *p, fluid_bound, src, msrc, cap, splits, part);
   auto comb = kahan ? k_far_combine<true> : k_far_combine<false>;
   hipLaunchKernelGGL(comb, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, msrc, cap, splits,
-                     (const FarPartial*)part, ws.near3, ws.scale_s, mode);
+                     (const void*)part, ws.near3, ws.scale_s, mode);
   DIMCV_LAUNCHED("far field (split)");
   return DIMCV_OK;
 }
