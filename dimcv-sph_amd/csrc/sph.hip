@@ -258,6 +258,7 @@ __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int w
     // a failed build leaves the solve done before its first iteration
     st.counters[DIMCV_C_JACOBI_DONE] = step_failed(st) ? 1 : 0;
     st.counters[DIMCV_C_JACOBI_ITERS] = 0;
+    st.counters[DIMCV_C_BAR_ARRIVE] = 0;  // barrier of a persistent solve (k_solve_single)
   }
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
@@ -594,6 +595,108 @@ int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, 
     hipLaunchKernelGGL(k_batch_solve<1>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
                        max_iter, bar);
   DIMCV_LAUNCHED("k_batch_solve");
+  return DIMCV_OK;
+}
+
+// The single-scene Jacobi solve's remaining iterations in one persistent
+// launch (the device-side form of dimcv_step's convergence checks): the
+// first iterations ran as ordinary launches and left the solve's state in
+// counters[DIMCV_C_JACOBI_DONE / _ITERS]; if the solve has converged every
+// block returns at once, otherwise the launch iterates (velocity update,
+// density pass with block partials, the canonical FP64 residual tree of
+// k_reduce_tail and its test, by block 0) behind grid barriers until the
+// test sets the flag.  Same arithmetic and order as the launched passes.
+// bar = counters + DIMCV_C_BAR_ARRIVE (arrivals, generation, abort).
+template <int WHICH>
+__global__ void __launch_bounds__(256) k_solve_single(dimcv_state st, dimcv_params p, int32_t bound, Packed pk,
+                                                      double* __restrict__ partials, float offset, double eta,
+                                                      int32_t max_iter) {
+  __shared__ double lds[256];
+  __shared__ int go;
+  int32_t* bar = st.counters + DIMCV_C_BAR_ARRIVE;
+  if (threadIdx.x == 0) go = st.counters[DIMCV_C_JACOBI_DONE] == 0;
+  __syncthreads();
+  if (!go) return;  // uniform: converged before the launch
+  int32_t gen = 0;
+  if (threadIdx.x == 0) gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t nlb = ((int64_t)bound + 255) / 256;
+  const int64_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
+  const int64_t lb0 = nlb * xcd / 8 + (blockIdx.x >> 3), lb1 = nlb * (xcd + 1) / 8;
+  for (;;) {
+    for (int64_t lb = lb0; lb < lb1; lb += per) {
+      if (WHICH == 0) div_iter_body<true>(st, p, bound, pk, lb);
+      else pressure_iter_body<true>(st, p, bound, pk, lb);
+    }
+    if (!grid_sync(bar, gen)) return;
+    for (int64_t lb = lb0; lb < lb1; lb += per) {
+      const double res = WHICH == 0 ? density_change_body<true>(st, p, bound, pk, lb)
+                                    : density_adv_body<true>(st, p, bound, offset, pk, lb);
+      const double sum = block_tree_sum256(res, lds);
+      if (threadIdx.x == 0) partials[lb] = sum;
+    }
+    if (!grid_sync(bar, gen)) return;
+    if (blockIdx.x == 0) {
+      // k_reduce_tail's tree (levels of 256 consecutive values), in place
+      int64_t m = nlb;
+      while (m > 1) {
+        const int64_t groups = (m + 255) / 256;
+        for (int64_t g = 0; g < groups; ++g) {
+          const int64_t k = g * 256 + threadIdx.x;
+          const double sum = block_tree_sum256(k < m ? partials[k] : 0.0, lds);
+          if (threadIdx.x == 0) partials[g] = sum;
+          __syncthreads();
+        }
+        m = groups;
+      }
+      if (threadIdx.x == 0) {
+        const double r = nlb > 0 ? partials[0] : 0.0;
+        st.dscal[DIMCV_D_RESIDUAL] = r;
+        const int32_t it = ++st.counters[DIMCV_C_JACOBI_ITERS];
+        const double avg = r / (double)st.counters[DIMCV_C_FLUID_NUM];
+        if (avg <= eta || it >= max_iter) st.counters[DIMCV_C_JACOBI_DONE] = 1;
+      }
+    }
+    if (!grid_sync(bar, gen)) return;
+    if (threadIdx.x == 0)
+      go = __hip_atomic_load(&st.counters[DIMCV_C_JACOBI_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __syncthreads();
+    if (!go) break;
+  }
+}
+
+static int solve_single_capacity(int which) {
+  static std::mutex mu;
+  static int cap[64][2];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (cap[dev][which] == 0) {
+    int per_cu = 0, cus = 0;
+    const void* f = which == 0 ? (const void*)k_solve_single<0> : (const void*)k_solve_single<1>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    cap[dev][which] = (per_cu * cus) & ~7;
+  }
+  return cap[dev][which];
+}
+
+int launch_solve_single(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                        const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
+                        hipStream_t s) {
+  const int cap = solve_single_capacity(which);
+  if (cap <= 0) {
+    set_error("persistent Jacobi solve: no resident block capacity on this device");
+    return DIMCV_ERR_HIP;
+  }
+  const int64_t nlb = ((int64_t)fluid_bound + 255) / 256;
+  const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((nlb + 7) & ~int64_t(7), cap));
+  if (which == 0)
+    hipLaunchKernelGGL(k_solve_single<0>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
+                       max_iter);
+  else
+    hipLaunchKernelGGL(k_solve_single<1>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
+                       max_iter);
+  DIMCV_LAUNCHED("k_solve_single");
   return DIMCV_OK;
 }
 

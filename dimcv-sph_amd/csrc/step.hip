@@ -30,6 +30,9 @@ void set_far_timing(hipEvent_t a, hipEvent_t b);
 int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials = nullptr, const Packed* pk = nullptr);
+int launch_solve_single(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
+                        const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
+                        hipStream_t s);
 
 // Events for the optional far-field timing (created on first use, kept for
 // the life of the process).
@@ -45,6 +48,7 @@ __global__ void k_jacobi_reset(int32_t* counters) {
 // truncated) or no fluid (the reference divides by fluid_particle_num).
 // Nonzero makes the substep's state writers return (step_failed).
 __global__ void k_step_guard(int32_t* counters, int32_t nbr_kmax) {
+  counters[DIMCV_C_BAR_ABORT] = 0;
   int32_t e = 0;
   if (counters[DIMCV_C_KEY_ERROR] != 0) e |= 1;
   if (counters[DIMCV_C_NBR_MAX] > nbr_kmax) e |= 2;
@@ -54,6 +58,10 @@ __global__ void k_step_guard(int32_t* counters, int32_t nbr_kmax) {
 
 // Host side of the same test, from a copy of the counters.
 static int step_status(const int32_t* hc, const dimcv_params* p) {
+  if (hc[DIMCV_C_BAR_ABORT] != 0) {
+    set_error("dimcv_step: a grid barrier of the device-side Jacobi loop timed out (blocks not all resident)");
+    return DIMCV_ERR_HIP;
+  }
   if (hc[DIMCV_C_KEY_ERROR] != 0) {
     set_error("%d active solid particles lie past the cell table", hc[DIMCV_C_KEY_ERROR]);
     return DIMCV_ERR_KEY;
@@ -110,6 +118,8 @@ static int jacobi_issue(int which, const dimcv_state* st, const dimcv_params* p,
 // kept for the life of the process.
 static int32_t* g_check_host = nullptr;
 static hipEvent_t g_check_ev = nullptr;
+// The divergence solve's counters of a device-side step (pinned).
+static int32_t* g_div_host = nullptr;
 
 static int read_counters(const dimcv_state* st, hipStream_t s, int32_t* out) {
   if (!g_check_host) {
@@ -143,6 +153,16 @@ static int jacobi_wait(int which, const dimcv_state* st, const dimcv_params* p, 
   }
   *iters_out = c[DIMCV_C_JACOBI_ITERS];
   return DIMCV_OK;
+}
+
+// The rest of a solve on the device (dimcv_step with chunk < 0): one
+// persistent launch that returns at once when the issued iterations have
+// converged.
+static int jacobi_device(int which, const dimcv_state* st, const dimcv_params* p, const dimcv_step_opts* o,
+                         int32_t fluid_bound, const Workspace& ws, hipStream_t s) {
+  const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
+  return launch_solve_single(which, st, p, fluid_bound, pk, ws.partials, which == 0 ? 0.0f : o->density0,
+                             which == 0 ? o->eta_v : o->eta_p, which == 0 ? o->max_iter_v : o->max_iter_p, s);
 }
 
 static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
@@ -427,6 +447,29 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   const HeadKey& key = sp.key;
   const int32_t pb = particle_bound;
   const int n_first_p = sp.n_first_p;
+  if (o->chunk < 0 && !p->dyn_rigid) {
+    // device-side convergence: the rest of the step is enqueued without a
+    // host check (each solve's remaining iterations, if any, in a persistent
+    // launch); the divergence solve's counters are copied aside before the
+    // pressure solve resets them, and the host waits once, at the end.  A
+    // failed build runs the step with every writer of x, v, is_sample and
+    // life_time returning (the solves start converged).
+    if (!g_div_host) DIMCV_HIP(hipHostMalloc((void**)&g_div_host, DIMCV_C_COUNT * sizeof(int32_t),
+                                             hipHostMallocDefault));
+    CHECK(jacobi_device(0, st, p, o, pb, ws, s));
+    DIMCV_HIP(hipMemcpyAsync(g_div_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
+    CHECK(seg_far(st, p, o, pb, s));
+    CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
+                    s));
+    CHECK(jacobi_device(1, st, p, o, pb, ws, s));
+    CHECK(seg_tail(st, p, o, pb, s));
+    CHECK(read_counters(st, s, host_counters));
+    o->out_kv = g_div_host[DIMCV_C_JACOBI_ITERS];
+    o->out_kp = host_counters[DIMCV_C_JACOBI_ITERS];
+    CHECK(far_time(p, o));
+    return step_status(host_counters, p);
+  }
   // the divergence solve's checks also bring the build's outcome: a failed
   // build ends the step here (its solve was done before it began, and no
   // writer of x, v, is_sample or life_time has run)

@@ -436,10 +436,12 @@ class DIMCVSPHSolver(SPHBase):
             o.u1, o.u2, o.m = t1.data_ptr(), t2.data_ptr(), 0
         o.max_iter_v = self.m_max_iterations_v
         o.max_iter_p = self.m_max_iterations
-        # jacobiChunk: iterations per host convergence check; 0 (the default)
-        # checks first after the previous step's iteration count
+        # jacobiChunk: iterations per host convergence check; 0 checks first
+        # after the previous step's iteration count; -1 (the default) lets the
+        # one-call fixed-M step (dimcv_step) converge on the device with no
+        # host check inside the step (dimcv_substep treats it as 0)
         chunk = ps.cfg.get_cfg("jacobiChunk")
-        o.chunk = int(chunk) if chunk is not None else 0
+        o.chunk = int(chunk) if chunk is not None else -1
         o.out_kv = max(1, int(self.last_iterations_v or 1))
         o.out_kp = max(1, int(self.last_iterations_p or 1))
         o.eta_v = self.inv_dt * self.max_error_V * 0.01 * self.density_0

@@ -67,6 +67,9 @@ typedef enum dimcv_counter {
                                  the table, 2 list width over nbr_kmax, 4 no fluid); the substep's writers of
                                  x, v, is_sample and life_time then return, so a failing step leaves them as
                                  the build left them.  Cleared by every grid build. */
+  DIMCV_C_BAR_ARRIVE = 13,    /* native step, device-side Jacobi loop: grid-barrier arrivals */
+  DIMCV_C_BAR_GEN = 14,       /* ... barrier generation */
+  DIMCV_C_BAR_ABORT = 15,     /* ... nonzero when a barrier wait timed out */
   DIMCV_C_COUNT = 16
 } dimcv_counter;
 
@@ -459,10 +462,14 @@ typedef struct dimcv_step_opts {
   int32_t m;                /* fixed-M sample count; 0 selects the Bernoulli sample set */
   int32_t max_iter_v;       /* m_max_iterations_v (dimcv_sph.py:22) */
   int32_t max_iter_p;       /* m_max_iterations (:23) */
-  int32_t chunk;            /* dimcv_substep: Jacobi iterations issued per host convergence check
-                               (>= 1); 0: the first check after out_kv / out_kp iterations as
-                               passed in (the previous step's counts, at least 1), then after every
-                               iteration.  Unused by the batched step (no host checks). */
+  int32_t chunk;            /* Jacobi iterations issued per host convergence check (>= 1); 0: the
+                               first check after out_kv / out_kp iterations as passed in (the
+                               previous step's counts, at least 1), then after every iteration.
+                               Negative: dimcv_step converges on the device (after out_kv / out_kp
+                               launched iterations, one persistent launch per solve runs any
+                               further ones; the host waits once, at the end of the step);
+                               dimcv_substep treats it as 0 and scenes with dynamic rigid
+                               particles keep the host checks.  Unused by the batched step. */
   double eta_v;             /* inv_dt * 0.1 * 0.01 * rho0 (:389) */
   double eta_p;             /* 0.05 * 0.01 * rho0 (:611) */
   float inv_dt;             /* f32(1 / dt) (:27, :381) */

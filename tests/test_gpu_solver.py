@@ -138,11 +138,13 @@ def test_divergence_solve_iterates_bitexact(max_error_v, steps):
         assert 100 in kvs, kvs
 
 
-@pytest.mark.parametrize("chunk", [None, 2])
+@pytest.mark.parametrize("chunk", [0, 2, None], ids=["host-adaptive", "host-every-2", "device-loop"])
 def test_divergence_solve_iterates_fixed_m_step(chunk):
     """The same for the fixed-M one-call step (dimcv_step_begin/finish) on a
-    ring lattice: the first convergence check after the previous step's
-    iteration count (chunk unset), or every 2 iterations."""
+    ring lattice: the first host convergence check after the previous step's
+    iteration count (chunk 0), every 2 iterations, or no host check at all
+    (chunk unset = -1: the remaining iterations in one persistent launch per
+    solve, the counters read once at the end of the step)."""
     ps, solver, ref, info = H.make_ring_pair(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="exact")
     if chunk is not None:
         ps.cfg.config["Configuration"]["jacobiChunk"] = chunk
