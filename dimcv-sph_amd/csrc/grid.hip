@@ -414,7 +414,9 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
 // Candidates [beg, end) of one run, SCAN_U at a time: their positions and
 // material flags are loaded (indices clamped into the run) before any is
 // tested, so a lane waits for one memory latency per SCAN_U candidates
-// rather than per candidate.  f(j, is_fluid) is called in slot order.
+// rather than per candidate.  f(j, is_fluid, accepted) is called for every
+// candidate in slot order (the emitter below records the accepted ones
+// without a branch per candidate).
 #define SCAN_U 8
 // Candidate positions and material flags: from the separate fields, or from
 // 16-byte (x, y, z, material bits) records (one aligned load per candidate).
@@ -452,7 +454,7 @@ __device__ __forceinline__ void scan_run(const Cand& cand, int beg, int end, int
     for (int u = 0; u < SCAN_U; ++u) {
       const v3 r = vsub(xi, xj[u]);
       const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-      if (j + u < end && r2 < lim && j + u != i) f(j + u, mt[u] == 1);
+      f(j + u, mt[u] == 1, (j + u < end) & (r2 < lim) & (j + u != i));  // no short-circuit branches
     }
   }
 }
@@ -486,10 +488,41 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const Cand& can
 // first NBR_STAGE entries of each lane are therefore staged in LDS and
 // written row by row by the whole wave (one coalesced 256-byte row store);
 // entries past NBR_STAGE are written directly.
-#define NBR_STAGE 32
+//
+// Every candidate is stored to the stage, accepted or not, at row cnt (which
+// only an accepted candidate advances) or, from cnt = NBR_STAGE on, at the
+// dump row NBR_STAGE: a rejected candidate's word is overwritten by the next
+// accepted one or lies past the list, so recording costs a few vector
+// instructions per candidate and no branch (only the rare direct write of an
+// entry past the stage branches).
+#define NBR_STAGE 31
+
+struct NbrEmitter {
+  uint32_t (*stage)[64];  // the wave's NBR_STAGE + 1 rows
+  uint32_t* __restrict__ nbr;
+  int64_t stride;
+  int64_t f;
+  int kmax, lane, cnt;
+  __device__ __forceinline__ void operator()(int j, bool fluid, bool ok) {
+    const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
+    stage[min(cnt, NBR_STAGE)][lane] = e;
+    if (ok & (cnt >= NBR_STAGE) & (cnt < kmax)) nbr[(int64_t)cnt * stride + f] = e;
+    cnt += ok ? 1 : 0;
+  }
+};
+
+// Raises counters[DIMCV_C_NBR_MAX] to the wave's longest list: one atomic per
+// wave (its lane 0) instead of one per lane on the same word.  Called by every
+// lane of the wave (cnt = 0 for lanes without an entry).
+__device__ __forceinline__ void nbr_wave_max(dimcv_state& st, int lane, int cnt) {
+  int m = cnt;
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if (lane == 0 && m > 0) atomicMax(&st.counters[DIMCV_C_NBR_MAX], m);
+}
 
 __device__ __forceinline__ void nbr_flush(dimcv_state& st, const dimcv_params& p, uint32_t (*stage)[64], int lane,
                                           int64_t f, int cnt) {
+  nbr_wave_max(st, lane, cnt);
   const int c = min(cnt, min(NBR_STAGE, p.nbr_kmax));
   int cw = c;
   for (int o = 32; o > 0; o >>= 1) cw = max(cw, __shfl_xor(cw, o, 64));
@@ -500,27 +533,18 @@ __device__ __forceinline__ void nbr_flush(dimcv_state& st, const dimcv_params& p
 
 template <class Cand>
 __global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound, Cand cand) {
-  __shared__ uint32_t stage[4][NBR_STAGE][64];
+  __shared__ uint32_t stage[4][NBR_STAGE + 1][64];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int cnt = 0;
+  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0};
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
     const v3 xi = ld3(st.x, i);
-    const int kmax = p.nbr_kmax;
-    const int64_t stride = p.n_max;
-    auto emit = [&](int j, bool fluid) {
-      const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
-      if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
-      else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
-      ++cnt;
-    };
-    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) { emit(j, st.material[j] == 1); });
+    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) { emit(j, st.material[j] == 1, true); });
     else scan_runs(p, cand, st.grid_particles_num, i, xi, emit);
-    st.nbr_count[f] = cnt;
-    atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+    st.nbr_count[f] = emit.cnt;
   }
-  nbr_flush(st, p, stage[w], lane, f, cnt);
+  nbr_flush(st, p, stage[w], lane, f, emit.cnt);
 }
 
 // dimcv_sph.py:536-549: the pressure update's reaction on a dynamic rigid
@@ -759,10 +783,10 @@ __global__ void k_batch_keys(dimcv_state st, dimcv_params p, int64_t n_in, uint3
 // that cell), as in a solo build.  Entries are staged as in k_nbr_build
 // (workgroups of at most 256 threads).
 __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_params p, int64_t bound) {
-  __shared__ uint32_t stage[4][NBR_STAGE][64];
+  __shared__ uint32_t stage[4][NBR_STAGE + 1][64];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int cnt = 0;
+  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0};
   const int64_t i = (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) ? st.fluid_list[f] : -2;
   if (i == -1) st.nbr_count[f] = 0;
   if (i >= 0) {
@@ -776,8 +800,6 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
     const int32_t* __restrict__ gpn = st.grid_particles_num + sc.cell_base;
     const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, nz - 1);
     const float lim = p.nbr_r2_limit;
-    const int kmax = p.nbr_kmax;
-    const int64_t stride = p.n_max;
     for (int o0 = -1; o0 <= 1; ++o0) {
       const int g0 = cc[0] + o0;
       if (g0 < 0 || g0 >= nx) continue;
@@ -788,18 +810,12 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
         const int lo = fl + zlo - 1;
         const int beg = lo >= 0 ? gpn[lo] : gpn[0];
         const int end = gpn[fl + zhi];
-        scan_run(CandFields{st.x, st.material}, beg, end, i, xi, lim, [&](int j, bool fluid) {
-          const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
-          if (cnt < NBR_STAGE) stage[w][cnt][lane] = e;
-          else if (cnt < kmax) st.nbr[(int64_t)cnt * stride + f] = e;
-          ++cnt;
-        });
+        scan_run(CandFields{st.x, st.material}, beg, end, i, xi, lim, emit);
       }
     }
-    st.nbr_count[f] = cnt;
-    atomicMax(&st.counters[DIMCV_C_NBR_MAX], cnt);
+    st.nbr_count[f] = emit.cnt;
   }
-  nbr_flush(st, p, stage[w], lane, f, cnt);
+  nbr_flush(st, p, stage[w], lane, f, emit.cnt);
 }
 
 static int g_nbr_block = 256;
