@@ -24,8 +24,12 @@ Also measured in the same run:
   * differentiation: forward and adjoint times of a 32-step, 65,536-particle
     ring rollout with 16 circulations as parameters.
 
+  * Jacobi iterations at scale: ms per divergence and per pressure
+    iteration at 1M particles, from steps whose solves iterate.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--no-direct] [--no-batching] [--batch-steps K] [--no-adjoint]
+                       [--no-jacobi]
 """
 from __future__ import annotations
 
@@ -325,6 +329,94 @@ def run_adjoint(steps=32, checkpoint=8):
     return out
 
 
+def _timed_steps(solver, n, before=None):
+    """n steps, each timed alone (the step returns after the device finishes);
+    before(k) runs untimed ahead of step k.  Returns [(ms, k_v, k_p)]."""
+    import torch
+    out = []
+    for k in range(n):
+        if before is not None:
+            before(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.step()
+        out.append(((time.perf_counter() - t0) * 1e3, solver.last_iterations_v, solver.last_iterations_p))
+    return out
+
+
+def _slope(rows, col):
+    """Least-squares ms per extra iteration of column col (1: k_v, 2: k_p)."""
+    import numpy as np
+    k = np.array([r[col] for r in rows], float)
+    t = np.array([r[0] for r in rows], float)
+    if np.ptp(k) == 0:
+        return None
+    a = np.vstack([k, np.ones_like(k)]).T
+    return float(np.linalg.lstsq(a, t, rcond=None)[0][0])
+
+
+def run_jacobi_scale(far_field="fast"):
+    """Cost of one Jacobi iteration at 1M particles (the default workload
+    converges both solves in one iteration).  Pressure: the 1M ring lattice
+    compressed to 0.92 of its spacing (rho about 1.28 x 800 > rho0), whose
+    pressure solve iterates from the 100-iteration cap down to 1 as the
+    compression relaxes.  Divergence: the uncompressed lattice with a
+    converging flow -a (x - c) added before each step, a swept so that the
+    divergence solve iterates a varying number of times (rho stays below rho0,
+    so the pressure solve converges at once).  The step time against the
+    iteration count gives ms per iteration (least squares; iterations past the
+    first run inside the solve's persistent launch), and the solve's
+    algorithmic bytes per iteration (95 and 99 B per particle) over it an HBM
+    rate."""
+    import numpy as np
+    import torch
+    import scenes
+    from config_builder import SimConfig
+    from particle_system import ParticleSystem
+
+    info = scenes.make_ring_scene(128, 128, 64, n_rings=8, seed=2, mc_samples=4096, far_field=far_field)
+    out = {"workload": "1M ring lattice: pressure solve on the lattice compressed to 0.92 spacing; divergence "
+                       "solve (maxErrorV 1e-3) with a converging flow -a (x - c) added before each step, a swept"}
+    # pressure iterations
+    ps = ParticleSystem(SimConfig(scene=info["scene"]))
+    scenes.add_ring_particles(ps, (info["pos"] * np.float32(0.92)).astype(np.float32), info["vel"])
+    solver = ps.build_solver()
+    solver.initialize()
+    n = ps.fluid_particle_num[None]
+    rows_p = _timed_steps(solver, 10)
+    del solver, ps
+    torch.cuda.empty_cache()
+    # divergence iterations (tolerance maxErrorV 1e-3 %, so that a moderate
+    # converging flow takes several iterations to remove)
+    ps, solver, _ = scenes.build_ring_system(128, 128, 64, n_rings=8, seed=2, mc_samples=4096, far_field=far_field)
+    solver.max_error_V = 1e-3
+    solver.step()
+    rates = [0.0, 1.0, 3.0, 10.0, 0.0, 3.0, 10.0, 1.0, 10.0, 3.0]
+
+    def converge(k):
+        if rates[k] > 0.0:
+            x = ps.x.t[: ps.particle_num[None]]
+            ps.v.t[: x.shape[0]] -= rates[k] * (x - x.mean(0))
+
+    rows_v = _timed_steps(solver, len(rates), converge)
+    del solver, ps
+    torch.cuda.empty_cache()
+    ms_p, ms_v = _slope(rows_p, 2), _slope(rows_v, 1)
+    out.update({
+        "n_particles": n,
+        "pressure_steps": [{"ms": round(t, 3), "k_v": kv, "k_p": kp} for t, kv, kp in rows_p],
+        "divergence_steps": [{"ms": round(t, 3), "k_v": kv, "k_p": kp, "converging_rate": a}
+                             for (t, kv, kp), a in zip(rows_v, rates)],
+        "ms_per_pressure_iteration": None if ms_p is None else round(ms_p, 5),
+        "ms_per_divergence_iteration": None if ms_v is None else round(ms_v, 5),
+        "pressure_iteration_gbs": None if not ms_p else round(99 * n / (ms_p * 1e-3) / 1e9, 1),
+        "divergence_iteration_gbs": None if not ms_v else round(95 * n / (ms_v * 1e-3) / 1e9, 1),
+        "note": "iteration bytes: 95 B (divergence: density change + velocity update) and 99 B (pressure: "
+                "density_adv + velocity update) per particle, read and written once"})
+    log(f"jacobi at scale: {json.dumps(out)}")
+    return out
+
+
 def grid_sph_rate(res, ms_per_step):
     """Algorithmic HBM bytes of the grid and SPH passes per step (each
     particle's fields read and written once per pass):
@@ -378,6 +470,7 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the timed steps, run a 256 MiB elementwise copy (PMC scale reference)")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -398,6 +491,7 @@ def main():
                   "precision_262144": run_recovery_comparison(64, 64, 64, [1024, 4096, 16384, 65536, 262144])}
     batching = None if args.no_batching else run_batching(steps=args.batch_steps)
     adjoint = None if args.no_adjoint else run_adjoint()
+    jacobi = None if args.no_jacobi else run_jacobi_scale(args.far_field)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(info, steps=5)
     value = res["n_fluid"] * args.steps / res["sec"]
     traffic = None
@@ -440,6 +534,7 @@ def main():
         "mc_vs_direct": direct,
         "batching": batching,
         "adjoint": adjoint,
+        "jacobi_at_scale": jacobi,
     }
     print(json.dumps(line), flush=True)
 

@@ -33,7 +33,7 @@ int launch_block_partials(int which, const dimcv_state* st, const dimcv_params* 
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s);
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
-                       float* kap, hipStream_t s, bool reset_jacobi = false);
+                       float* kap, hipStream_t s, bool reset_jacobi = false, int32_t* bar_tree = nullptr);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
                        hipStream_t s, const Packed* pk);
 int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,

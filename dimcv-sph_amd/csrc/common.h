@@ -67,6 +67,7 @@ struct Workspace {
   float4* xw;             // per slot (x, y, z, w), w = m (fluid) or m_V rho0 (the Jacobi passes)
   float* kap;             // per slot kappa of the current Jacobi iteration
   int32_t* tiny;          // step flag: some coordinate is nonzero and below 2^-76 in magnitude
+  int32_t* bar_tree;      // group words of the persistent solves' grid barrier (grid_sync_tree)
   float4* samples;        // packed far-field sources: (x, y, z, -) (cx, cy, cz, -)
   int64_t max_samples;
   int64_t bytes;
@@ -573,6 +574,79 @@ __device__ inline bool grid_sync(int32_t* bar, int32_t& gen) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     gen = g + 1;
+    ok = r;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Grid barrier of the single-scene persistent solve as a two-level tree:
+// blocks arrive on their group's word (DIMCV_BAR_GROUP consecutive blocks per
+// group, one group per 64 bytes of tree), the last block of a group arrives
+// on the top word, the last group's arrival opens the next generation, and
+// each group's last block then opens its group.  As in grid_sync, every
+// block releases its stores before it arrives and acquires after it leaves,
+// and every wait is bounded; the tree spreads the same-address atomics and
+// the polling over the group words (through one word, a barrier of ~1,000
+// blocks cost tens of microseconds).
+// bar: top arrivals, generation, abort; tree[16 g]: arrivals of group g,
+// tree[16 g + 1]: its generation (only incremented); gen / ggen: this
+// block's copies of the two generations (grid_sync_tree_init at launch).
+// Group arrival words must be 0 when a launch starts (cleared with the
+// solve's reset, and left 0 by every completed barrier).
+#define DIMCV_BAR_GROUP 16
+#define DIMCV_BAR_GROUPS 256
+
+__device__ inline void grid_sync_tree_init(const int32_t* bar, const int32_t* tree, int32_t& gen, int32_t& ggen) {
+  if (threadIdx.x == 0) {
+    gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ggen = __hip_atomic_load(&tree[16 * (blockIdx.x / DIMCV_BAR_GROUP) + 1], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Waits until *word differs from old; false when the barrier was aborted or
+// the wait ran past GRID_SPIN_LIMIT polls (then it raises the abort flag).
+__device__ inline bool bar_wait(const int32_t* word, int32_t old, int32_t* bar) {
+  for (uint32_t spin = 0;; ++spin) {
+    if (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != old) return true;
+    if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (spin >= GRID_SPIN_LIMIT) {
+      atomicExch(&bar[2], 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+__device__ inline bool grid_sync_tree(int32_t* bar, int32_t* tree, int32_t& gen, int32_t& ggen) {
+  __shared__ int ok;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores have reached the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int r = 1;
+    const int g = blockIdx.x / DIMCV_BAR_GROUP;
+    const int ng = (gridDim.x + DIMCV_BAR_GROUP - 1) / DIMCV_BAR_GROUP;
+    const int gsize = min((int)gridDim.x - g * DIMCV_BAR_GROUP, DIMCV_BAR_GROUP);
+    int32_t* gw = tree + 16 * g;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's stores, from the L2 to memory
+    if (atomicAdd(&gw[0], 1) == gsize - 1) {
+      atomicExch(&gw[0], 0);
+      if (atomicAdd(&bar[0], 1) == ng - 1) {
+        atomicExch(&bar[0], 0);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        atomicAdd(&bar[1], 1);
+      } else {
+        r = bar_wait(&bar[1], gen, bar) ? 1 : 0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the group word's reset before the group opens
+      if (r) atomicAdd(&gw[1], 1);
+    } else {
+      r = bar_wait(&gw[1], ggen, bar) ? 1 : 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    gen += 1;
+    ggen += 1;
     ok = r;
   }
   __syncthreads();

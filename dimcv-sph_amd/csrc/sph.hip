@@ -252,7 +252,7 @@ int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particl
 // With reset, it also clears the Jacobi counters (done flag, iteration count)
 // for the solve that follows it.
 __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int which, int32_t bound,
-                              float* __restrict__ kap, int reset) {
+                              float* __restrict__ kap, int reset, int32_t* __restrict__ tree) {
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   if (reset && f == 0) {
     // a failed build leaves the solve done before its first iteration
@@ -260,6 +260,8 @@ __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int w
     st.counters[DIMCV_C_JACOBI_ITERS] = 0;
     st.counters[DIMCV_C_BAR_ARRIVE] = 0;  // barrier of a persistent solve (k_solve_single)
   }
+  // the group arrival words of that barrier (the launch has >= 256 threads)
+  if (reset && tree && f < DIMCV_BAR_GROUPS) tree[16 * f] = 0;
   if (f >= bound || f >= st.counters[DIMCV_C_FLUID_LIST]) return;
   const int64_t i = st.fluid_list[f];
   if (i < 0) return;
@@ -270,11 +272,11 @@ __global__ void k_scale_kappa(dimcv_state st, dimcv_params p, float scale, int w
 }
 
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
-                       float* kap, hipStream_t s, bool reset_jacobi) {
+                       float* kap, hipStream_t s, bool reset_jacobi, int32_t* bar_tree) {
   // with reset_jacobi, at least one block runs (its first thread clears the counters)
   if (fluid_bound <= 0 && !reset_jacobi) return DIMCV_OK;
   hipLaunchKernelGGL(k_scale_kappa, dim3(fluid_bound > 0 ? nblocks(fluid_bound) : 1), dim3(256), 0, s, *st, *p,
-                     scale, which, fluid_bound, kap, reset_jacobi ? 1 : 0);
+                     scale, which, fluid_bound, kap, reset_jacobi ? 1 : 0, reset_jacobi ? bar_tree : nullptr);
   DIMCV_LAUNCHED("k_scale_kappa");
   return DIMCV_OK;
 }
@@ -602,23 +604,58 @@ int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, 
 // launch (the device-side form of dimcv_step's convergence checks): the
 // first iterations ran as ordinary launches and left the solve's state in
 // counters[DIMCV_C_JACOBI_DONE / _ITERS]; if the solve has converged every
-// block returns at once, otherwise the launch iterates (velocity update,
-// density pass with block partials, the canonical FP64 residual tree of
-// k_reduce_tail and its test, by block 0) behind grid barriers until the
-// test sets the flag.  Same arithmetic and order as the launched passes.
-// bar = counters + DIMCV_C_BAR_ARRIVE (arrivals, generation, abort).
+// block returns at once, otherwise the launch iterates: the velocity update,
+// a grid barrier, the density pass with its block partials, a grid barrier,
+// then every block forms the residual's canonical tree over all the block
+// partials itself (the tree of k_reduce_tail, so the same value in every
+// block) and applies the test, so all blocks leave the loop together with no
+// third barrier; block 0 records the residual and the counters.  Same
+// arithmetic and order as the launched passes.
+// bar = counters + DIMCV_C_BAR_ARRIVE (arrivals, generation, abort); tree:
+// the barrier's group words (grid_sync_tree).
+//
+// The canonical tree of a[0, n) (levels of 256 consecutive values, as
+// k_reduce_tail and k_reduce_levels form it), by one block: each group of 256
+// level-1 sums is kept in l1 and summed into l2; n <= 2^24.
+__device__ double block_canonical_sum(const double* __restrict__ a, int64_t n, double* lds, double* l1, double* l2) {
+  const int t = threadIdx.x;
+  if (n <= 0) return 0.0;
+  const int64_t G = (n + 255) / 256;  // level-1 groups
+  const int64_t C = (G + 255) / 256;  // level-2 groups
+  for (int64_t c = 0; c < C; ++c) {
+    const int64_t q1 = G - c * 256 < 256 ? G - c * 256 : 256;
+    for (int64_t q = 0; q < q1; ++q) {
+      const int64_t k = (c * 256 + q) * 256 + t;
+      const double sum = block_tree_sum256(k < n ? a[k] : 0.0, lds);
+      if (t == 0) l1[q] = sum;
+    }
+    __syncthreads();
+    if (G == 1) return l1[0];
+    const double s2 = block_tree_sum256(t < q1 ? l1[t] : 0.0, lds);
+    if (t == 0) l2[c] = s2;
+    __syncthreads();
+  }
+  if (C == 1) return l2[0];
+  return block_tree_sum256(t < C ? l2[t] : 0.0, lds);
+}
+
 template <int WHICH>
 __global__ void __launch_bounds__(256) k_solve_single(dimcv_state st, dimcv_params p, int32_t bound, Packed pk,
                                                       double* __restrict__ partials, float offset, double eta,
-                                                      int32_t max_iter) {
-  __shared__ double lds[256];
-  __shared__ int go;
+                                                      int32_t max_iter, int32_t* __restrict__ tree) {
+  __shared__ double lds[256], l1[256], l2[256];
+  __shared__ int go, it0, fluid;
   int32_t* bar = st.counters + DIMCV_C_BAR_ARRIVE;
-  if (threadIdx.x == 0) go = st.counters[DIMCV_C_JACOBI_DONE] == 0;
+  if (threadIdx.x == 0) {
+    go = st.counters[DIMCV_C_JACOBI_DONE] == 0;
+    it0 = st.counters[DIMCV_C_JACOBI_ITERS];
+    fluid = st.counters[DIMCV_C_FLUID_NUM];
+  }
   __syncthreads();
   if (!go) return;  // uniform: converged before the launch
-  int32_t gen = 0;
-  if (threadIdx.x == 0) gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t gen = 0, ggen = 0;
+  grid_sync_tree_init(bar, tree, gen, ggen);
+  int32_t it = it0;
   const int64_t nlb = ((int64_t)bound + 255) / 256;
   const int64_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
   const int64_t lb0 = nlb * xcd / 8 + (blockIdx.x >> 3), lb1 = nlb * (xcd + 1) / 8;
@@ -627,40 +664,25 @@ __global__ void __launch_bounds__(256) k_solve_single(dimcv_state st, dimcv_para
       if (WHICH == 0) div_iter_body<true>(st, p, bound, pk, lb);
       else pressure_iter_body<true>(st, p, bound, pk, lb);
     }
-    if (!grid_sync(bar, gen)) return;
+    if (!grid_sync_tree(bar, tree, gen, ggen)) return;
     for (int64_t lb = lb0; lb < lb1; lb += per) {
       const double res = WHICH == 0 ? density_change_body<true>(st, p, bound, pk, lb)
                                     : density_adv_body<true>(st, p, bound, offset, pk, lb);
       const double sum = block_tree_sum256(res, lds);
       if (threadIdx.x == 0) partials[lb] = sum;
     }
-    if (!grid_sync(bar, gen)) return;
-    if (blockIdx.x == 0) {
-      // k_reduce_tail's tree (levels of 256 consecutive values), in place
-      int64_t m = nlb;
-      while (m > 1) {
-        const int64_t groups = (m + 255) / 256;
-        for (int64_t g = 0; g < groups; ++g) {
-          const int64_t k = g * 256 + threadIdx.x;
-          const double sum = block_tree_sum256(k < m ? partials[k] : 0.0, lds);
-          if (threadIdx.x == 0) partials[g] = sum;
-          __syncthreads();
-        }
-        m = groups;
-      }
-      if (threadIdx.x == 0) {
-        const double r = nlb > 0 ? partials[0] : 0.0;
-        st.dscal[DIMCV_D_RESIDUAL] = r;
-        const int32_t it = ++st.counters[DIMCV_C_JACOBI_ITERS];
-        const double avg = r / (double)st.counters[DIMCV_C_FLUID_NUM];
-        if (avg <= eta || it >= max_iter) st.counters[DIMCV_C_JACOBI_DONE] = 1;
-      }
+    if (!grid_sync_tree(bar, tree, gen, ggen)) return;
+    const double r = block_canonical_sum(partials, nlb, lds, l1, l2);
+    ++it;
+    const bool done = r / (double)fluid <= eta || it >= max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st.dscal[DIMCV_D_RESIDUAL] = r;
+      st.counters[DIMCV_C_JACOBI_ITERS] = it;
+      if (done) st.counters[DIMCV_C_JACOBI_DONE] = 1;
     }
-    if (!grid_sync(bar, gen)) return;
-    if (threadIdx.x == 0)
-      go = __hip_atomic_load(&st.counters[DIMCV_C_JACOBI_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-    __syncthreads();
-    if (!go) break;
+    if (done) break;  // the same decision in every block
+    // the next velocity update overwrites nothing this reduction reads
+    // (partials are rewritten only after the next barrier)
   }
 }
 
@@ -682,7 +704,7 @@ static int solve_single_capacity(int which) {
 
 int launch_solve_single(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                         const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
-                        hipStream_t s) {
+                        int32_t* bar_tree, hipStream_t s) {
   const int cap = solve_single_capacity(which);
   if (cap <= 0) {
     set_error("persistent Jacobi solve: no resident block capacity on this device");
@@ -690,12 +712,16 @@ int launch_solve_single(int which, const dimcv_state* st, const dimcv_params* p,
   }
   const int64_t nlb = ((int64_t)fluid_bound + 255) / 256;
   const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((nlb + 7) & ~int64_t(7), cap));
+  if (g > DIMCV_BAR_GROUP * DIMCV_BAR_GROUPS) {
+    set_error("persistent Jacobi solve: %u blocks exceed the barrier's %d groups", g, DIMCV_BAR_GROUPS);
+    return DIMCV_ERR_HIP;
+  }
   if (which == 0)
     hipLaunchKernelGGL(k_solve_single<0>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
-                       max_iter);
+                       max_iter, bar_tree);
   else
     hipLaunchKernelGGL(k_solve_single<1>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
-                       max_iter);
+                       max_iter, bar_tree);
   DIMCV_LAUNCHED("k_solve_single");
   return DIMCV_OK;
 }

@@ -21,7 +21,7 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
                    hipStream_t s);
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
-                       float* kap, hipStream_t s, bool reset_jacobi = false);
+                       float* kap, hipStream_t s, bool reset_jacobi = false, int32_t* bar_tree = nullptr);
 int launch_copy_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_predict_life(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
@@ -32,7 +32,7 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
                  double* partials = nullptr, const Packed* pk = nullptr);
 int launch_solve_single(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                         const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
-                        hipStream_t s);
+                        int32_t* bar_tree, hipStream_t s);
 
 // Events for the optional far-field timing (created on first use, kept for
 // the life of the process).
@@ -162,7 +162,8 @@ static int jacobi_device(int which, const dimcv_state* st, const dimcv_params* p
                          int32_t fluid_bound, const Workspace& ws, hipStream_t s) {
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   return launch_solve_single(which, st, p, fluid_bound, pk, ws.partials, which == 0 ? 0.0f : o->density0,
-                             which == 0 ? o->eta_v : o->eta_p, which == 0 ? o->max_iter_v : o->max_iter_p, s);
+                             which == 0 ? o->eta_v : o->eta_p, which == 0 ? o->max_iter_v : o->max_iter_p,
+                             ws.bar_tree, s);
 }
 
 static int jacobi(int which, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
@@ -193,7 +194,7 @@ static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimc
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
   CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, &pk));
-  CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s, true));  // also resets the solve
+  CHECK(launch_scale_kappa(0, st, p, o->inv_dt, fluid_bound, ws.kap, s, true, ws.bar_tree));  // also resets the solve
   return jacobi_issue(0, st, p, o, fluid_bound, ws, s, n_first, false);
 }
 
@@ -249,7 +250,7 @@ static int seg_mid2(const dimcv_state* st, const dimcv_params* p, const dimcv_st
   CHECK(launch_predict_life(st, p, particle_bound, s));
   // pressure_solve (dimcv_sph.py:597-614)
   CHECK(launch_jacobi_pass(3, st, p, fluid_bound, nullptr, s, nullptr, 0.0f, &pk));
-  CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s, true));  // also resets the solve
+  CHECK(launch_scale_kappa(1, st, p, o->inv_dt2, fluid_bound, ws.kap, s, true, ws.bar_tree));  // also resets the solve
   return jacobi_issue(1, st, p, o, fluid_bound, ws, s, n_first_p, false);
 }
 

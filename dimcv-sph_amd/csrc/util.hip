@@ -55,6 +55,7 @@ int64_t workspace_layout(int64_t n_max, int64_t n_cells, int64_t max_samples, Wo
   w.xw = (float4*)take(16 * n);
   w.kap = (float*)take(4 * n);
   w.tiny = (int32_t*)take(256);
+  w.bar_tree = (int32_t*)take(4 * 16 * DIMCV_BAR_GROUPS);
   // split far-field partials: splits * targets of far_split_count for any
   // target count up to n (16 n when n < 65536, below 2^21 otherwise)
   w.far_partial_cap = n * 16 < DIMCV_FAR_PARTIAL_MAX ? n * 16 : DIMCV_FAR_PARTIAL_MAX;
