@@ -46,7 +46,7 @@ int batch_vort2vel_launch(const dimcv_state* st, const dimcv_params* p, Workspac
 void batch_set_far_workgroups(int w);
 int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
-                       int32_t* bar, hipStream_t s);
+                       int32_t* bar, int32_t* bar_tree, hipStream_t s);
 
 static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
@@ -209,7 +209,8 @@ __global__ void __launch_bounds__(256) k_batch_reduce(dimcv_state st, double* __
 
 // Before a solve: every scene iterates (none when the step failed), no
 // iterations counted, the persistent solve's barrier words cleared.
-__global__ void k_batch_jacobi_reset(dimcv_state st, int32_t* __restrict__ bar) {
+__global__ void k_batch_jacobi_reset(dimcv_state st, int32_t* __restrict__ bar, int32_t* __restrict__ tree) {
+  for (int g = threadIdx.x; g < DIMCV_BAR_GROUPS; g += blockDim.x) tree[16 * g] = 0;  // group arrivals
   const int32_t d = step_failed(st) ? 1 : 0;
   for (int s = threadIdx.x; s < st.n_scenes; s += blockDim.x) {
     st.scene_counters[(int64_t)s * DIMCV_SC_COUNT + DIMCV_SC_DONE] = d;
@@ -273,7 +274,7 @@ static int seg_div_begin(BatchCtx& c, hipStream_t s) {
   CHECK(launch_pack_xm(st, c.p, nb, c.pk, s));
   CHECK(launch_fused(3, st, c.p, nb, s, nullptr, &c.pk));
   CHECK(launch_scale_kappa(0, st, c.p, c.o->inv_dt, nb, c.ws.kap, s));
-  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR);
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR, c.ws.bar_tree);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
 }
@@ -291,7 +292,8 @@ static int seg_solve(BatchCtx& c, int which, hipStream_t s) {
                            which == 0 ? 0.0f : c.o->density0, &c.pk));
   return launch_batch_solve(which, c.st, c.p, (int32_t)c.p->n_max, c.pk, c.ws.partials,
                             which == 0 ? 0.0f : c.o->density0, which == 0 ? c.o->eta_v : c.o->eta_p,
-                            which == 0 ? c.o->max_iter_v : c.o->max_iter_p, c.bw.misc + BM_BAR, s);
+                            which == 0 ? c.o->max_iter_v : c.o->max_iter_p, c.bw.misc + BM_BAR, c.ws.bar_tree,
+                            s);
 }
 
 static int seg_mid(BatchCtx& c, hipStream_t s) {
@@ -312,7 +314,7 @@ static int seg_mid(BatchCtx& c, hipStream_t s) {
   CHECK(dimcv_predict_velocity(st, p, nb, s));
   CHECK(launch_jacobi_pass(3, st, p, nb, nullptr, s, nullptr, 0.0f, &c.pk));
   CHECK(launch_scale_kappa(1, st, p, c.o->inv_dt2, nb, c.ws.kap, s));
-  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR);
+  hipLaunchKernelGGL(k_batch_jacobi_reset, dim3(1), dim3(256), 0, s, *st, c.bw.misc + BM_BAR, c.ws.bar_tree);
   DIMCV_LAUNCHED("k_batch_jacobi_reset");
   return DIMCV_OK;
 }

@@ -534,66 +534,27 @@ __device__ inline bool batch_scene_check(const dimcv_state& st, double eta, int3
   return c[DIMCV_SC_DONE] == 0;
 }
 
-// Grid-wide barrier of a persistent launch whose blocks are all resident.
-// bar[0]: arrivals, bar[1]: generation, bar[2]: abort (bar[0] and bar[2]
-// cleared before the launch); gen: the generation this block last saw (held
-// by thread 0).  Every wave waits for its stores to reach the XCD's L2, then
-// thread 0 releases them at agent scope (the L2s of the XCDs are not coherent
-// with each other: the release writes the L2's dirty lines back) and
-// arrives; the last arrival opens the next generation, the others wait for
-// it, and each block acquires (invalidating its CU's L1 and its XCD's L2).  A wait is bounded (about a second): past it the block raises
+// Grid-wide barrier of a persistent launch whose blocks are all resident
+// (the single-scene and batched Jacobi solves), as a two-level tree: blocks
+// arrive on their group's word (DIMCV_BAR_GROUP consecutive blocks per group,
+// one group per 64 bytes of tree), the last block of a group arrives on the
+// top word, the last group's arrival opens the next generation, and each
+// group's last block then opens its group.  Through one word for every
+// arrival and poll, a barrier of ~1,000 blocks cost tens of microseconds.
+// Every wave waits for its stores to reach the XCD's L2, then thread 0
+// releases them at agent scope (the L2s of the XCDs are not coherent with
+// each other: the release writes the L2's dirty lines back) and arrives;
+// after leaving, each block acquires (invalidating its CU's L1 and its
+// XCD's L2).  A wait is bounded (about a second): past it the block raises
 // abort, and every waiting block returns false at once, so the launch always
 // ends; the caller reports the abort as an error.
-#define GRID_SPIN_LIMIT (1u << 20)
-__device__ inline bool grid_sync(int32_t* bar, int32_t& gen) {
-  __shared__ int ok;
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores have reached the L2
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int r = 1;
-    const int32_t g = gen;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's stores, from the L2 to memory
-    if (atomicAdd(&bar[0], 1) == (int32_t)gridDim.x - 1) {
-      atomicExch(&bar[0], 0);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      atomicAdd(&bar[1], 1);
-    } else {
-      for (uint32_t spin = 0;; ++spin) {
-        if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) break;
-        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          r = 0;
-          break;
-        }
-        if (spin >= GRID_SPIN_LIMIT) {
-          atomicExch(&bar[2], 1);
-          r = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(8);
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    gen = g + 1;
-    ok = r;
-  }
-  __syncthreads();
-  return ok != 0;
-}
-
-// Grid barrier of the single-scene persistent solve as a two-level tree:
-// blocks arrive on their group's word (DIMCV_BAR_GROUP consecutive blocks per
-// group, one group per 64 bytes of tree), the last block of a group arrives
-// on the top word, the last group's arrival opens the next generation, and
-// each group's last block then opens its group.  As in grid_sync, every
-// block releases its stores before it arrives and acquires after it leaves,
-// and every wait is bounded; the tree spreads the same-address atomics and
-// the polling over the group words (through one word, a barrier of ~1,000
-// blocks cost tens of microseconds).
 // bar: top arrivals, generation, abort; tree[16 g]: arrivals of group g,
 // tree[16 g + 1]: its generation (only incremented); gen / ggen: this
-// block's copies of the two generations (grid_sync_tree_init at launch).
-// Group arrival words must be 0 when a launch starts (cleared with the
-// solve's reset, and left 0 by every completed barrier).
+// block's copies of the two generations (grid_sync_tree_init at launch,
+// held by thread 0).  Top and group arrival words must be 0 when a launch
+// starts (cleared with the solve's reset, and left 0 by every completed
+// barrier).
+#define GRID_SPIN_LIMIT (1u << 20)
 #define DIMCV_BAR_GROUP 16
 #define DIMCV_BAR_GROUPS 256
 

@@ -513,15 +513,17 @@ __global__ void __launch_bounds__(256) k_pressure_iter(dimcv_state st, dimcv_par
 // skipped as the launched passes skip it, so every scene's arithmetic and
 // iteration count are those of separate launches.
 // bar: BM_BAR of the batch misc words (arrivals, generation, abort, go flags
-// for even and odd iterations; cleared by k_batch_jacobi_reset).
+// for even and odd iterations; cleared by k_batch_jacobi_reset); tree: the
+// barrier's group words (grid_sync_tree; arrivals cleared by the same reset).
 template <int WHICH>
 __global__ void __launch_bounds__(256) k_batch_solve(dimcv_state st, dimcv_params p, int32_t bound, Packed pk,
                                                      double* __restrict__ partials, float offset, double eta,
-                                                     int32_t max_iter, int32_t* __restrict__ bar) {
+                                                     int32_t max_iter, int32_t* __restrict__ bar,
+                                                     int32_t* __restrict__ tree) {
   __shared__ double lds[256];
   __shared__ int go;
-  int32_t gen = 0;
-  if (threadIdx.x == 0) gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t gen = 0, ggen = 0;
+  grid_sync_tree_init(bar, tree, gen, ggen);
   const int32_t* done = st.scene_counters + DIMCV_SC_DONE;
   int32_t* flag = bar + 3;
   // XCD-aware: block b runs on XCD b % 8 (gridDim.x is a multiple of 8) and
@@ -536,7 +538,7 @@ __global__ void __launch_bounds__(256) k_batch_solve(dimcv_state st, dimcv_param
         if (WHICH == 0) div_iter_body<true>(st, p, bound, pk, lb);
         else pressure_iter_body<true>(st, p, bound, pk, lb);
       }
-      if (!grid_sync(bar, gen)) return;
+      if (!grid_sync_tree(bar, tree, gen, ggen)) return;
       for (int64_t lb = lb0; lb < lb1; lb += per) {
         if (jacobi_gated_lb(st, done, lb)) continue;
         const double res = WHICH == 0 ? density_change_body<true>(st, p, bound, pk, lb)
@@ -544,7 +546,7 @@ __global__ void __launch_bounds__(256) k_batch_solve(dimcv_state st, dimcv_param
         const double sum = block_tree_sum256(res, lds);
         if (threadIdx.x == 0) partials[lb] = sum;
       }
-      if (!grid_sync(bar, gen)) return;
+      if (!grid_sync_tree(bar, tree, gen, ggen)) return;
     }
     for (int sc = blockIdx.x; sc < st.n_scenes; sc += gridDim.x) {
       if (done[(int64_t)sc * DIMCV_SC_COUNT]) continue;  // uniform
@@ -553,7 +555,7 @@ __global__ void __launch_bounds__(256) k_batch_solve(dimcv_state st, dimcv_param
     }
     // the other flag was last read before this iteration's barriers
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&flag[(it + 1) & 1], 0);
-    if (!grid_sync(bar, gen)) return;
+    if (!grid_sync_tree(bar, tree, gen, ggen)) return;
     if (threadIdx.x == 0) go = __hip_atomic_load(&flag[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (!go) break;
@@ -580,7 +582,7 @@ static int solve_capacity(int which) {
 
 int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound,
                        const Packed& pk, double* partials, float offset, double eta, int32_t max_iter,
-                       int32_t* bar, hipStream_t s) {
+                       int32_t* bar, int32_t* bar_tree, hipStream_t s) {
   const int cap = solve_capacity(which);
   if (cap <= 0) {
     set_error("persistent Jacobi solve: no resident block capacity on this device");
@@ -590,12 +592,16 @@ int launch_batch_solve(int which, const dimcv_state* st, const dimcv_params* p, 
   // the barriers)
   const int64_t nlb = ((int64_t)fluid_bound + 255) / 256;
   const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((nlb + 7) & ~int64_t(7), cap));
+  if (g > DIMCV_BAR_GROUP * DIMCV_BAR_GROUPS) {
+    set_error("persistent Jacobi solve: %u blocks exceed the barrier's %d groups", g, DIMCV_BAR_GROUPS);
+    return DIMCV_ERR_HIP;
+  }
   if (which == 0)
     hipLaunchKernelGGL(k_batch_solve<0>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
-                       max_iter, bar);
+                       max_iter, bar, bar_tree);
   else
     hipLaunchKernelGGL(k_batch_solve<1>, dim3(g), dim3(256), 0, s, *st, *p, fluid_bound, pk, partials, offset, eta,
-                       max_iter, bar);
+                       max_iter, bar, bar_tree);
   DIMCV_LAUNCHED("k_batch_solve");
   return DIMCV_OK;
 }
