@@ -81,6 +81,10 @@ def run_gpu(args):
     # HIP events around the far-field kernel, recorded by dimcv_substep on the
     # launch stream inside every timed step
     solver.time_far_field = True
+    # timed: pipelined steps (async_steps): each step() returns once the step
+    # is enqueued, and its counts arrive with the next step(), so the device
+    # runs the next grid build while the host prepares it
+    solver.async_steps = not args.sync_steps
     kv, kp, far = [], [], []
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -89,18 +93,50 @@ def run_gpu(args):
         kv.append(solver.last_iterations_v)
         kp.append(solver.last_iterations_p)
         far.append(solver.last_far_ms)
+    solver.flush()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    if solver.async_steps:
+        # each step() brought the previous step's counts: drop the first
+        # (the last untimed step's) and add the last timed step's
+        kv, kp, far = kv[1:], kp[1:], far[1:]
+        kv.append(solver.last_iterations_v)
+        kp.append(solver.last_iterations_p)
+        far.append(solver.last_far_ms)
+    # for reference: the synchronous step (each step() waits for the device)
+    # on the next steps (the flow evolves, so later steps run a little slower)
+    pipelined = solver.async_steps
+    async_after_ms = None
+    solver.async_steps = False
+    n_sync = min(args.steps, 20)
+    t0s = time.perf_counter()
+    for _ in range(n_sync):
+        solver.step()
+    torch.cuda.synchronize()
+    sync_ms = (time.perf_counter() - t0s) * 1e3 / n_sync
+    if pipelined:
+        solver.async_steps = True
+        t0a = time.perf_counter()
+        for _ in range(n_sync):
+            solver.step()
+        solver.flush()
+        torch.cuda.synchronize()
+        async_after_ms = (time.perf_counter() - t0a) * 1e3 / n_sync
+        solver.async_steps = False
+        log(f"after the timed steps: {n_sync} synchronous steps {sync_ms:.3f} ms/step, then {n_sync} "
+            f"pipelined {async_after_ms:.3f} ms/step")
     n_fluid = ps.fluid_particle_num[None]
     sec = t2 - t1
     far_ms = float(np.mean(far))
     m = solver.num_samples[None]
     pairs = n_fluid * m
     achieved = FLOPS_PER_PAIR_MC * pairs / (far_ms * 1e-3) / 1e12
-    log(f"timed {args.steps} steps: {sec * 1e3 / args.steps:.3f} ms/step, far field {far_ms:.3f} ms "
-        f"({achieved:.1f} TFLOP/s), k_v mean {np.mean(kv):.2f}, k_p mean {np.mean(kp):.2f}")
+    log(f"timed {args.steps} steps: {sec * 1e3 / args.steps:.3f} ms/step ({sync_ms:.3f} synchronous), "
+        f"far field {far_ms:.3f} ms ({achieved:.1f} TFLOP/s), k_v mean {np.mean(kv):.2f}, "
+        f"k_p mean {np.mean(kp):.2f}")
     res = {
-        "n_fluid": n_fluid, "sec": sec, "far_ms": far_ms, "achieved_tflops": achieved,
+        "n_fluid": n_fluid, "sec": sec, "far_ms": far_ms, "achieved_tflops": achieved, "sync_ms": sync_ms,
+        "async_after_ms": async_after_ms,
         "pairs": pairs, "kv": float(np.mean(kv)), "kp": float(np.mean(kp)), "m": m,
         "n_cells": int(ps.grid_num[0]) * int(ps.grid_num[1]) * int(ps.grid_num[2]),
     }
@@ -474,6 +510,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the timed steps, run a 256 MiB elementwise copy (PMC scale reference)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--sync-steps", action="store_true",
+                    help="time synchronous steps (each step() waits for the device) instead of pipelined ones")
     ap.add_argument("--far-field", default="fast", choices=["fast", "kahan"],
                     help="far-field evaluation of the timed 1M step")
     args = ap.parse_args()
@@ -515,7 +553,11 @@ def main():
         "config": {"workload": "1M-particle vortex-ring lattice, MC-VVPE M=4096",
                    "n_particles": res["n_fluid"], "lattice": [128, 128, 64], "rings": 8,
                    "mc_samples": res["m"], "dt": 2e-4, "far_field": args.far_field,
-                   "mean_k_v": res["kv"], "mean_k_p": res["kp"], "parallelism": "single GPU"},
+                   "mean_k_v": res["kv"], "mean_k_p": res["kp"], "parallelism": "single GPU",
+                   "steps_pipelined": not args.sync_steps,
+                   "ms_per_step_synchronous_next_20": round(res["sync_ms"], 4),
+                   "ms_per_step_pipelined_20_after_those": None if res["async_after_ms"] is None
+                   else round(res["async_after_ms"], 4)},
         "roofline": {"bound": "valu", "kernel": "MC-VVPE far field: k_far_split over two sample halves + k_far_combine (k_vv_far_fast when unsplit)",
                      "achieved": round(res["achieved_tflops"], 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(res["achieved_tflops"] / FP32_PEAK_TFLOPS, 4),

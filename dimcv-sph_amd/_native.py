@@ -206,6 +206,8 @@ def lib():
         "dimcv_step": (I32, [P, P, P, P, I32, P, P]),
         "dimcv_step_begin": (I32, [P, P, P, P, I32, P]),
         "dimcv_step_finish": (I32, [P, P, P, P, I32, P, P]),
+        "dimcv_step_finish_async": (I32, [P, P, P, P, I32, P]),
+        "dimcv_step_wait": (I32, [P, P, P]),
         "dimcv_batch_workspace_bytes": (I64, [I32, P, I64]),
         "dimcv_batch_grid_build": (I32, [P, P, P]),
         "dimcv_batch_neighbor_build": (I32, [P, P, P]),

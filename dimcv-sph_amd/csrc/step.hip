@@ -284,6 +284,8 @@ static int substep_tail(const dimcv_state* st, const dimcv_params* p, dimcv_step
   return far_time(p, o);
 }
 
+static int async_idle(const char* who);
+
 extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o,
                              int32_t particle_bound, int32_t fluid_bound, void* stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -291,6 +293,7 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
     set_error("dimcv_substep: null argument");
     return DIMCV_ERR_ARG;
   }
+  CHECK(async_idle("dimcv_substep"));
   if (o->fluid_count <= 0) {
     set_error("dimcv_substep: no fluid particles (the reference divides by fluid_particle_num)");
     return DIMCV_ERR_ARG;
@@ -432,6 +435,24 @@ extern "C" int dimcv_step_begin(const dimcv_state* pre, const dimcv_state* st, c
   }, sp.key, s);
 }
 
+// Asynchronous completion of a device-side step: dimcv_step_finish_async
+// enqueues the rest of the step and the copy of the counters into pinned
+// memory and returns; dimcv_step_wait, called before the next step's
+// _finish (or at any later time), waits for that copy and reports the step
+// as dimcv_step_finish would have.  The caller may enqueue the next step's
+// _begin in between, so the device does not idle while the host prepares
+// the next step.  One asynchronous step may be outstanding.
+static hipEvent_t g_async_ev = nullptr;
+static bool g_async_pending = false;
+
+static int async_idle(const char* who) {
+  if (g_async_pending) {
+    set_error("%s: an asynchronous step is outstanding (call dimcv_step_wait first)", who);
+    return DIMCV_ERR_ARG;
+  }
+  return DIMCV_OK;
+}
+
 // The rest of the step after dimcv_step_begin with the same arguments; the
 // counts come back in host_counters with the pressure solve's last check.
 extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
@@ -442,6 +463,7 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
     set_error("dimcv_step: null argument");
     return DIMCV_ERR_ARG;
   }
+  CHECK(async_idle("dimcv_step_finish"));
   StepPrep sp;
   CHECK(step_prepare(pre, st, p, o, particle_bound, &sp));
   const Workspace& ws = sp.ws;
@@ -483,6 +505,62 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   CHECK(jacobi_wait(1, st, p, o, pb, ws, s, &o->out_kp, host_counters));
   // one or two kernels: a direct launch starts sooner than a graph launch
   CHECK(seg_tail(st, p, o, pb, s));
+  CHECK(far_time(p, o));
+  return step_status(host_counters, p);
+}
+
+extern "C" int dimcv_step_finish_async(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                                       dimcv_step_opts* o, int32_t particle_bound, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  CHECK(async_idle("dimcv_step_finish_async"));
+  StepPrep sp;
+  CHECK(step_prepare(pre, st, p, o, particle_bound, &sp));
+  if (o->chunk >= 0 || p->dyn_rigid) {
+    set_error("dimcv_step_finish_async: only the device-side step (chunk < 0, no dynamic rigid particles)");
+    return DIMCV_ERR_ARG;
+  }
+  const Workspace& ws = sp.ws;
+  const HeadKey& key = sp.key;
+  const int32_t pb = particle_bound;
+  const int n_first_p = sp.n_first_p;
+  if (!g_div_host) DIMCV_HIP(hipHostMalloc((void**)&g_div_host, DIMCV_C_COUNT * sizeof(int32_t),
+                                           hipHostMallocDefault));
+  if (!g_check_host) {
+    DIMCV_HIP(hipHostMalloc((void**)&g_check_host, DIMCV_C_COUNT * sizeof(int32_t), hipHostMallocDefault));
+    DIMCV_HIP(hipEventCreateWithFlags(&g_check_ev, hipEventDisableTiming));
+  }
+  if (!g_async_ev) DIMCV_HIP(hipEventCreateWithFlags(&g_async_ev, hipEventDisableTiming));
+  CHECK(jacobi_device(0, st, p, o, pb, ws, s));
+  DIMCV_HIP(hipMemcpyAsync(g_div_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
+  CHECK(seg_far(st, p, o, pb, s));
+  CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
+                  s));
+  CHECK(jacobi_device(1, st, p, o, pb, ws, s));
+  CHECK(seg_tail(st, p, o, pb, s));
+  DIMCV_HIP(hipMemcpyAsync(g_check_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DIMCV_HIP(hipEventRecord(g_async_ev, s));
+  g_async_pending = true;
+  return DIMCV_OK;
+}
+
+extern "C" int dimcv_step_wait(const dimcv_params* p, dimcv_step_opts* o, int32_t* host_counters) {
+  if (!p || !o || !host_counters) {
+    set_error("dimcv_step_wait: null argument");
+    return DIMCV_ERR_ARG;
+  }
+  if (!g_async_pending) {
+    set_error("dimcv_step_wait: no asynchronous step is outstanding");
+    return DIMCV_ERR_ARG;
+  }
+  hipError_t e;
+  while ((e = hipEventQuery(g_async_ev)) == hipErrorNotReady) {
+  }
+  g_async_pending = false;
+  DIMCV_HIP(e);
+  std::copy(g_check_host, g_check_host + DIMCV_C_COUNT, host_counters);
+  o->out_kv = g_div_host[DIMCV_C_JACOBI_ITERS];
+  o->out_kp = host_counters[DIMCV_C_JACOBI_ITERS];
   CHECK(far_time(p, o));
   return step_status(host_counters, p);
 }

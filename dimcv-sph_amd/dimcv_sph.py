@@ -101,6 +101,14 @@ class DIMCVSPHSolver(SPHBase):
         self.last_iterations_p = 0
         self.time_far_field = False   # record HIP events around the far-field kernel
         self.last_far_ms = None
+        # pipelined fixed-M steps (no counterpart in the reference): step()
+        # returns once the step is enqueued; its counts and iteration numbers
+        # reach the host, and a failed step raises, at the next step() or at
+        # flush().  Host-side counts (ps._pn, last_iterations_*) lag by one
+        # step until then; device-side reads (fields, counters) are current.
+        self.async_steps = False
+        self._pending = None
+        self._step_parity = 0
 
         p = ps.params
         f32 = np.float32
@@ -320,6 +328,13 @@ class DIMCVSPHSolver(SPHBase):
             bufs = self._staged[key] = (host, dev, host.numpy())
         return bufs
 
+    def _cdf_key(self):
+        """Staging key of the step's inverse-CDF uniforms: with async_steps two
+        buffer pairs by step parity, so that a step's upload never overwrites
+        the uniforms of the step still running (whose buffers were last used
+        by the step before it, already waited for)."""
+        return f"u_cdf{self._step_parity}" if self.async_steps else "u_cdf"
+
     def _upload_staged(self, key, a, dtype):
         """Copy `a` into a device buffer kept for `key`, through a pinned host
         buffer (an asynchronous copy on the current stream).  The host buffer
@@ -418,7 +433,7 @@ class DIMCVSPHSolver(SPHBase):
         if self.mc_samples:
             m = int(self.mc_samples)
             if defer_cdf:
-                t = self._staged_buffers("u_cdf", m, torch.float64)[1]
+                t = self._staged_buffers(self._cdf_key(), m, torch.float64)[1]
             else:
                 if u_cdf is None:
                     u_cdf = self.rng.random(m)
@@ -465,6 +480,7 @@ class DIMCVSPHSolver(SPHBase):
         in the same order; the Jacobi loops converge on the device and the host
         checks every ``jacobiChunk`` iterations."""
         ps = self.ps
+        self.flush()
         o = self._step_opts(u1, u2, u_cdf)
         o.fluid_count = self._fluid_count()
         if o.fluid_count <= 0:
@@ -484,6 +500,9 @@ class DIMCVSPHSolver(SPHBase):
         uniforms are drawn per particle after the build)."""
         ps = self.ps
         m = int(self.mc_samples)
+        if not self.async_steps:
+            self.flush()
+        self._step_parity ^= 1
         o = self._step_opts(None, None, None, defer_cdf=True)
         pre = ps.state()
         ps.swap_sorted_fields()
@@ -501,15 +520,48 @@ class DIMCVSPHSolver(SPHBase):
             self._upload_event = torch.cuda.Event()
         main = torch.cuda.current_stream(ps.device)
         with torch.cuda.stream(self._upload_stream):
-            t = self._upload_staged("u_cdf", u, torch.float64)
+            t = self._upload_staged(self._cdf_key(), u, torch.float64)
             self._upload_event.record(self._upload_stream)
         main.wait_event(self._upload_event)
         assert t.data_ptr() == o.u_cdf
+        if self.async_steps:
+            # the previous step's counts (its errors are raised here), then
+            # this step's rest without a wait: the device already runs this
+            # step's build while the host reads them
+            self.flush()
+            N.check(lib.dimcv_step_finish_async(*args, stream), "dimcv_step_finish_async")
+            self._pending = o
+            return
         rc = lib.dimcv_step_finish(*args, ps._host_counters_ptr, stream)
         ps._pn = int(hc[N.C_PARTICLE_NUM])
         ps._nfl = int(hc[N.C_FLUID_LIST])
         N.check(rc, "dimcv_step_finish")
         self._step_outputs(o)
+
+    def flush(self):
+        """Complete the outstanding asynchronous step (async_steps): its counts,
+        iteration counts and far-field time become visible on the host, and a
+        failed step raises here."""
+        o, self._pending = self._pending, None
+        if o is None:
+            return
+        ps = self.ps
+        hc = ps._host_counters
+        rc = N.lib().dimcv_step_wait(ctypes.byref(ps.params), ctypes.byref(o), ps._host_counters_ptr)
+        ps._pn = int(hc[N.C_PARTICLE_NUM])
+        ps._nfl = int(hc[N.C_FLUID_LIST])
+        self._step_outputs(o)
+        N.check(rc, "dimcv_step_wait")
+
+    def check_async_steps(self):
+        """async_steps needs the one-call fixed-M step converging on the
+        device, and a particle count that only the step itself changes (its
+        launch bound is the count before the pending step)."""
+        ps = self.ps
+        if not self.mc_samples or ps.cfg.get_cfg("jacobiChunk") is not None or ps.num_emitters > 0 \
+                or ps.params.dyn_rigid:
+            raise RuntimeError("async_steps needs fixed-M sampling (mcSamples), no jacobiChunk, no emitters "
+                               "and no dynamic rigid particles")
 
     # --------------------------------------------------- comparators --
     def direct_biot_savart(self, precision: str = "f32"):

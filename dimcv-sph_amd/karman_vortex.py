@@ -114,6 +114,8 @@ class KarmanVortexSolver(DIMCVSPHSolver):
             f.write(body)
 
     def step(self, u1=None, u2=None, u_cdf=None):
+        if self.async_steps:
+            self.check_async_steps()
         if self.emit_on_device:
             # emission and its counts stay on the device (dimcv_emit)
             if self.cnt % self.emit_interval == 0:
@@ -130,10 +132,12 @@ class KarmanVortexSolver(DIMCVSPHSolver):
             # fixed-M sampling: grid build, lists and substep in one library call
             self.native_step(u_cdf)
         elif native:
+            self.flush()
             self.ps.initialize_particle_system()
             self.compute_moving_boundary_volume()
             self.native_substep(u1, u2, u_cdf)
         else:
+            self.flush()
             self.ps.initialize_particle_system()
             self.compute_moving_boundary_volume()
             self._uniforms = (u1, u2, u_cdf)

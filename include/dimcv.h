@@ -500,14 +500,14 @@ int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts*
  * trips between them).  `pre` is the state before the grid build; `st` is
  * the same state with every sorted field exchanged with its _buffer twin
  * (the state the caller uses after the build).  particle_bound is the
- * particle count before the build; all launches are bounded by it and the
- * kernels read the exact counts from the device.  opts->fluid_count is
- * ignored (the residual divisor is the device's fluid count).  The counter
- * array (DIMCV_C_COUNT) is copied to host_counters at the pressure solve's
- * last convergence check; the grid and list checks of dimcv_grid_build and
- * dimcv_neighbor_build (DIMCV_ERR_KEY, DIMCV_ERR_NBR) are made from it after
- * the step, so a failing step has already run on truncated lists or
- * out-of-grid keys when the error is returned. */
+ * particle count before the build (or any larger value); all launches are
+ * bounded by it and the kernels read the exact counts from the device.
+ * opts->fluid_count is ignored (the residual divisor is the device's fluid
+ * count).  The counter array (DIMCV_C_COUNT) is copied to host_counters at
+ * the end of the step.  A step whose build fails (an active solid keyed past
+ * the cell table, DIMCV_ERR_KEY; a list wider than nbr_kmax, DIMCV_ERR_NBR)
+ * returns the error with x, v, life_time and the sample set unchanged: a
+ * guard after the build makes every state writer of the substep skip. */
 int dimcv_step(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* opts,
                int32_t particle_bound, int32_t* host_counters, void* stream);
 /* dimcv_step in two halves, called in this order with the same arguments:
@@ -522,6 +522,23 @@ int dimcv_step_begin(const dimcv_state* pre, const dimcv_state* st, const dimcv_
                      dimcv_step_opts* opts, int32_t particle_bound, void* stream);
 int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
                       dimcv_step_opts* opts, int32_t particle_bound, int32_t* host_counters, void* stream);
+/* No counterpart in the reference (which steps synchronously): the second
+ * half of a device-side step (opts->chunk < 0, no dynamic rigid particles)
+ * without the wait at its end.  dimcv_step_finish_async enqueues the rest of
+ * the step and a copy of the counters and returns; dimcv_step_wait waits for
+ * that copy, fills host_counters, opts->out_kv / out_kp / out_far_ms and
+ * returns the step's status as dimcv_step_finish would.  The caller may
+ * enqueue the next step's dimcv_step_begin (whose particle_bound must still
+ * bound the particle count, e.g. the count before the pending step when no
+ * particles are inserted) before dimcv_step_wait, so the device runs the next
+ * build while the host prepares it; a failed step is reported by the wait.
+ * One asynchronous step may be outstanding: dimcv_step_finish,
+ * dimcv_step_finish_async and dimcv_substep return DIMCV_ERR_ARG while one
+ * is.  For drivers that pipeline steps (the bench's timed loop); the same
+ * kernels, so results are those of dimcv_step. */
+int dimcv_step_finish_async(const dimcv_state* pre, const dimcv_state* st, const dimcv_params* p,
+                            dimcv_step_opts* opts, int32_t particle_bound, void* stream);
+int dimcv_step_wait(const dimcv_params* p, dimcv_step_opts* opts, int32_t* host_counters);
 
 /* ---- comparators and initial conditions (build-defined) --------------------- */
 /* Direct O(N^2) Biot-Savart over every in-domain fluid particle: the sum the

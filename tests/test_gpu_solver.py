@@ -234,19 +234,22 @@ def test_bounce_back_bitexact(kind):
         assert np.sum(np.any((x == h) | (x == hi), axis=1)) > 100
 
 
-@pytest.mark.parametrize("kind", ["fixed-m-step", "karman-substep"])
+@pytest.mark.parametrize("kind", ["fixed-m-step", "fixed-m-step-async", "karman-substep"])
 def test_list_width_overflow_fails_before_writing_state(kind):
     """A list wider than neighborListWidth raises RuntimeError before any
     writer of x, v, is_sample or life_time runs: the positions and
     velocities after the failed step are those before it (the grid build,
-    the reference's initialize_particle_system, may only reorder slots)."""
+    the reference's initialize_particle_system, may only reorder slots).
+    With async_steps the failed step reports at the next flush()."""
     from config_builder import SimConfig
     from particle_system import ParticleSystem
     import scenes
-    if kind == "fixed-m-step":
+    if kind.startswith("fixed-m-step"):
         ps, solver, info = scenes.build_ring_system(12, 12, 10, n_rings=2, seed=3, mc_samples=128,
                                                     far_field="fast")
+        solver.async_steps = kind.endswith("async")
         solver.step(u_cdf=np.random.default_rng(1).random(128))
+        solver.flush()
         # lists of the allocated width, read only up to 16 entries: the
         # lattice has ~26 neighbours per particle
         ps.nbr_kmax = 16
@@ -277,8 +280,9 @@ def test_list_width_overflow_fails_before_writing_state(kind):
                     break
         before = snapshot()
         with pytest.raises(RuntimeError, match="neighbour list width"):
-            if kind == "fixed-m-step":
+            if kind.startswith("fixed-m-step"):
                 solver.step(u_cdf=rng.random(128))
+                solver.flush()
             else:
                 solver.cnt = 1     # no emission in this step
                 solver.step(rng.random(ps.particle_max_num, dtype=np.float32),
@@ -286,3 +290,46 @@ def test_list_width_overflow_fails_before_writing_state(kind):
         after = snapshot()
         for a, b, name in zip(before, after, ("x", "v", "life_time", "is_sample")):
             assert np.array_equal(a, b), (attempt, name)
+
+
+def test_async_steps_equal_synchronous_steps():
+    """Pipelined steps (solver.async_steps: each step() returns once enqueued,
+    its counts arrive at the next step() or flush()) against the oracle, bit
+    for bit, on a ring lattice whose pressure solve iterates (compressed to
+    0.92 spacing) and whose divergence solve iterates (maxErrorV 1e-7): the
+    same kernels as the synchronous step, with the uniforms of consecutive
+    steps in separate buffers."""
+    import scenes
+    from config_builder import SimConfig
+    from oracle.refsim import RefSim
+    from particle_system import ParticleSystem
+    info = scenes.make_ring_scene(12, 12, 10, n_rings=2, seed=3, mc_samples=128, far_field="exact")
+    pos = (info["pos"] * np.float32(0.92)).astype(np.float32)
+    ps = ParticleSystem(SimConfig(scene=copy.deepcopy(info["scene"])))
+    scenes.add_ring_particles(ps, pos, info["vel"])
+    solver = ps.build_solver()
+    solver.initialize()
+    ref = RefSim(copy.deepcopy(info["scene"]))
+    n = pos.shape[0]
+    ref.add_particles(0, n, pos, info["vel"], np.full(n, 1000.0, np.float32), np.zeros(n, np.float32),
+                      np.ones(n, np.int32), np.ones(n, np.int32),
+                      np.tile(np.array([[50, 100, 200]], np.int32), (n, 1)))
+    ref.initialize()
+    solver.max_error_V = 1e-7
+    ref.max_error_V = 1e-7
+    solver.async_steps = True
+    rng = np.random.default_rng(4)
+    kv, kp = [], []
+    for k in range(6):
+        u = rng.random(128)
+        solver.step(u_cdf=u)
+        ref.step(u_cdf=u)
+        kv.append(ref.last_kv)
+        kp.append(ref.last_kp)
+        if k >= 1:  # the previous step's counts arrived with this step
+            assert (solver.last_iterations_v, solver.last_iterations_p) == (kv[-2], kp[-2]), k
+    solver.flush()
+    assert (solver.last_iterations_v, solver.last_iterations_p) == (kv[-1], kp[-1])
+    bad = H.mismatches(ps, solver, ref)
+    assert not bad, bad
+    assert max(kv) > 1 and max(kp) > 1, (kv, kp)
