@@ -14,7 +14,7 @@ REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdimcv.so")
 SOURCES = ["csrc/util.hip", "csrc/grid.hip", "csrc/sph.hip", "csrc/vvpe.hip", "csrc/step.hip",
            "csrc/batch.hip", "csrc/adjoint.hip", "csrc/export.hip",
-           "csrc/segment.hip"]
+           "csrc/segment.hip", "csrc/probe.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # one IEEE operation per source operation: the SPH passes reproduce the
@@ -176,6 +176,8 @@ def lib():
         "dimcv_abi_version": (I32, []),
         "dimcv_last_error": (ctypes.c_char_p, []),
         "dimcv_workspace_bytes": (I64, [I64, I64, I64]),
+        "dimcv_valu_issue_probe_bytes": (I64, []),
+        "dimcv_valu_issue_probe": (I32, [P, I64, P, P]),
         "dimcv_add_particles": (I32, [P, P, I32, I32, P, P, P, P, P, P, P, P, P]),
         "dimcv_init_cylinder": (I32, [P, P, I32, F32, F32, F32, P]),
         "dimcv_emit": (I32, [P, P, P, I32, P]),

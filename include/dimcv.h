@@ -271,6 +271,18 @@ typedef struct dimcv_state {
 /* ---- library ---------------------------------------------------------------- */
 int dimcv_abi_version(void);
 const char* dimcv_last_error(void);
+
+/* Vector-ALU issue-rate calibration; no counterpart in the reference (it
+ * serves the roofline of the far field, a kernel bound by vector
+ * instruction issue).  Runs two dense streams of independent FP32 fused
+ * multiply-adds, 4 waves per SIMD, on the current device and synchronises:
+ * operands all in vector registers, then one operand a scalar register.
+ * out[0] / out[2]: the shader clock during each stream (GHz, from s_memtime
+ * against the 100 MHz s_memrealtime); out[1] / out[3]: wave-instructions
+ * issued per SIMD and quad-cycle at that clock.  scratch: device memory of
+ * at least dimcv_valu_issue_probe_bytes() (-1 without a device). */
+int64_t dimcv_valu_issue_probe_bytes(void);
+int dimcv_valu_issue_probe(void* scratch, int64_t scratch_bytes, double* out, void* stream);
 /* Workspace bytes needed for n_max particles, n_cells cells and up to
  * max_samples importance samples. */
 int64_t dimcv_workspace_bytes(int64_t n_max, int64_t n_cells, int64_t max_samples);

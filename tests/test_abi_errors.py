@@ -47,6 +47,7 @@ def test_null_arguments_are_rejected(L):
         ("dimcv_rigid_moments", (None, None, 0, 0, None, None, 0, None, None)),
         ("dimcv_rigid_apply", (None, None, 0, None, 0, None)),
         ("dimcv_cdf_sample_noalloc", (None, None, None, 0, 0, None)),
+        ("dimcv_valu_issue_probe", (None, 0, None, None)),
     ]
     for name, args in cases:
         rc = getattr(L, name)(*args)
