@@ -411,57 +411,68 @@ __device__ __forceinline__ void scan_cells(const dimcv_params& p, const float* _
 // keeps cell 0 empty exactly as the per-cell rule does.  The distance test
 // sqrt_rn(r.r) < h is evaluated as r.r < nbr_r2_limit, which is equivalent
 // for every f32 r.r (sqrt is monotone and correctly rounded).
-// Candidates [beg, end) of one run, SCAN_U at a time: their positions and
-// material flags are loaded (indices clamped into the run) before any is
-// tested, so a lane waits for one memory latency per SCAN_U candidates
-// rather than per candidate.  f(j, is_fluid, accepted) is called for every
-// candidate in slot order (the emitter below records the accepted ones
-// without a branch per candidate).
+// Candidates [beg, end) of one run, SCAN_U at a time: their positions are
+// loaded (indices clamped into the run) before any is tested, so a lane
+// waits for one memory latency per SCAN_U candidates rather than per
+// candidate.  f(j, accepted) is called for every candidate in slot order
+// (the emitter below records the accepted ones without a branch per
+// candidate).  Entries carry the neighbour's fluid flag in their top bit;
+// it is read from material when an entry is written to the list, so the
+// candidate loads are the 12-byte positions alone.
 #define SCAN_U 8
-// Candidate positions and material flags: from the separate fields, or from
-// 16-byte (x, y, z, material bits) records (one aligned load per candidate).
-struct CandFields {
-  const float* __restrict__ x;
-  const int32_t* __restrict__ material;
-  __device__ __forceinline__ void get(int j, v3& xj, int32_t& mt) const {
-    xj = ld3s(x, (uint32_t)j);
-    mt = material[j];
-  }
-};
-struct CandRecords {
-  const float4* __restrict__ rec;
-  __device__ __forceinline__ void get(int j, v3& xj, int32_t& mt) const {
-    const float4 q = *(const float4*)((const char*)rec + (uint32_t)j * 16u);
-    xj = mk(q.x, q.y, q.z);
-    mt = __float_as_int(q.w);
-  }
-};
-
-__global__ void k_pack_candidates(dimcv_state st, int32_t bound, float4* __restrict__ rec) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
-  rec[s] = make_float4(st.x[3 * s], st.x[3 * s + 1], st.x[3 * s + 2], __int_as_float(st.material[s]));
+#define NBR_STAGE 31  // staged list rows per lane (the neighbour build below)
+#define NBR_FLUID_BIT 0x80000000u
+__device__ __forceinline__ uint32_t nbr_entry(const int32_t* __restrict__ material, uint32_t j) {
+  return j | (material[j] == 1 ? NBR_FLUID_BIT : 0u);
 }
 
-template <class Cand, class F>
-__device__ __forceinline__ void scan_run(const Cand& cand, int beg, int end, int64_t i, v3 xi, float lim, F&& f) {
+// The acceptance test (j < end, r.r < limit, j != i) is formed from sign
+// bits in vector registers, not as compares into lane masks: (j - end),
+// (bits(r.r) - bits(limit)) (non-negative floats order as their bit
+// patterns) and (d | -d) with d = j - i each have the sign bit set exactly
+// when their condition holds, and none of the differences leaves 32 bits.
+// Compares would each write a scalar mask that the next instruction waits
+// for.
+// Tests candidates j .. j + SCAN_U - 1 of a run ending at end (positions
+// xj) and hands each to the emitter f in slot order.
+template <class F>
+__device__ __forceinline__ void test_trip(int j, int end, const v3* xj, int64_t i, v3 xi, uint32_t lim_bits,
+                                          F&& f) {
+  const uint32_t ui = (uint32_t)i, uend = (uint32_t)end;
+  uint32_t ok[SCAN_U];
+#pragma unroll
+  for (int u = 0; u < SCAN_U; ++u) {
+    const v3 r = vsub(xi, xj[u]);
+    const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const uint32_t jj = (uint32_t)(j + u), d = jj - ui;
+    ok[u] = ((jj - uend) & (__float_as_uint(r2) - lim_bits) & (d | (0u - d))) >> 31;
+  }
+  // While every lane of the wave has SCAN_U free stage rows left, the trip
+  // writes row cnt directly (no clamp to the dump row, no direct list
+  // write): one wave-wide test per trip instead of a branch per candidate.
+  if (!__any(f.cnt > NBR_STAGE - SCAN_U)) {
+#pragma unroll
+    for (int u = 0; u < SCAN_U; ++u) f.staged(j + u, ok[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < SCAN_U; ++u) f(j + u, ok[u]);
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void scan_run(const float* __restrict__ x, int beg, int end, int64_t i, v3 xi, float lim,
+                                         F&& f) {
   for (int j = beg; j < end; j += SCAN_U) {
     v3 xj[SCAN_U];
-    int32_t mt[SCAN_U];
 #pragma unroll
-    for (int u = 0; u < SCAN_U; ++u) cand.get(min(j + u, end - 1), xj[u], mt[u]);
-#pragma unroll
-    for (int u = 0; u < SCAN_U; ++u) {
-      const v3 r = vsub(xi, xj[u]);
-      const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-      f(j + u, mt[u] == 1, (j + u < end) & (r2 < lim) & (j + u != i));  // no short-circuit branches
-    }
+    for (int u = 0; u < SCAN_U; ++u) xj[u] = ld3s(x, (uint32_t)min(j + u, end - 1));
+    test_trip(j, end, xj, i, xi, __float_as_uint(lim), f);
   }
 }
 
-template <class Cand, class F>
-__device__ __forceinline__ void scan_runs(const dimcv_params& p, const Cand& cand, const int32_t* __restrict__ gpn,
-                                          int64_t i, v3 xi, F&& f) {
+template <class F>
+__device__ __forceinline__ void scan_runs(const dimcv_params& p, const float* __restrict__ x,
+                                          const int32_t* __restrict__ gpn, int64_t i, v3 xi, F&& f) {
   int cc[3];
   cell_of(p, xi, cc);
   const int nx = p.grid_num[0], ny = p.grid_num[1], nz = p.grid_num[2];
@@ -476,7 +487,7 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const Cand& can
       int fl = (g0 * ny + g1) * nz;
       int beg = gpn[max(fl + zlo - 1, 0)];
       int end = gpn[fl + zhi];
-      scan_run(cand, beg, end, i, xi, lim, f);
+      scan_run(x, beg, end, i, xi, lim, f);
     }
   }
 }
@@ -495,7 +506,6 @@ __device__ __forceinline__ void scan_runs(const dimcv_params& p, const Cand& can
 // accepted one or lies past the list, so recording costs a few vector
 // instructions per candidate and no branch (only the rare direct write of an
 // entry past the stage branches).
-#define NBR_STAGE 31
 
 struct NbrEmitter {
   uint32_t (*stage)[64];  // the wave's NBR_STAGE + 1 rows
@@ -503,11 +513,18 @@ struct NbrEmitter {
   int64_t stride;
   int64_t f;
   int kmax, lane, cnt;
-  __device__ __forceinline__ void operator()(int j, bool fluid, bool ok) {
-    const uint32_t e = (uint32_t)j | (fluid ? 0x80000000u : 0u);
-    stage[min(cnt, NBR_STAGE)][lane] = e;
-    if (ok & (cnt >= NBR_STAGE) & (cnt < kmax)) nbr[(int64_t)cnt * stride + f] = e;
-    cnt += ok ? 1 : 0;
+  const int32_t* __restrict__ material;
+  // ok is 0 or 1.  staged(): only while cnt + SCAN_U <= NBR_STAGE.  The
+  // stage holds slots; nbr_flush adds the fluid flags.
+  __device__ __forceinline__ void staged(int j, uint32_t ok) {
+    stage[cnt][lane] = (uint32_t)j;
+    cnt += (int)ok;
+  }
+  __device__ __forceinline__ void operator()(int j, uint32_t ok) {
+    stage[min(cnt, NBR_STAGE)][lane] = (uint32_t)j;
+    const uint32_t direct = ok & ((uint32_t)(NBR_STAGE - 1 - cnt) >> 31) & ((uint32_t)(cnt - kmax) >> 31);
+    if (direct) nbr[(int64_t)cnt * stride + f] = nbr_entry(material, (uint32_t)j);
+    cnt += (int)ok;
   }
 };
 
@@ -528,20 +545,21 @@ __device__ __forceinline__ void nbr_flush(dimcv_state& st, const dimcv_params& p
   for (int o = 32; o > 0; o >>= 1) cw = max(cw, __shfl_xor(cw, o, 64));
   const int64_t stride = p.n_max;
   for (int k = 0; k < cw; ++k)
-    if (k < c) st.nbr[(int64_t)k * stride + f] = stage[k][lane];
+    if (k < c) st.nbr[(int64_t)k * stride + f] = nbr_entry(st.material, stage[k][lane]);
 }
 
-template <class Cand>
-__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound, Cand cand) {
+__global__ void __launch_bounds__(256) k_nbr_build(dimcv_state st, dimcv_params p, int32_t bound) {
   __shared__ uint32_t stage[4][NBR_STAGE + 1][64];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0};
+  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0, st.material};
   if (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
     const v3 xi = ld3(st.x, i);
-    if (p.periodic) scan_cells(p, st.x, st.grid_particles_num, i, xi, [&](int j) { emit(j, st.material[j] == 1, true); });
-    else scan_runs(p, cand, st.grid_particles_num, i, xi, emit);
+    if (p.periodic)
+      scan_cells(p, st.x, st.grid_particles_num, i, xi,
+                 [&](int j) { emit(j, 1u); });
+    else scan_runs(p, st.x, st.grid_particles_num, i, xi, emit);
     st.nbr_count[f] = emit.cnt;
   }
   nbr_flush(st, p, stage[w], lane, f, emit.cnt);
@@ -786,7 +804,7 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
   __shared__ uint32_t stage[4][NBR_STAGE + 1][64];
   const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0};
+  NbrEmitter emit{stage[w], st.nbr, p.n_max, f, p.nbr_kmax, lane, 0, st.material};
   const int64_t i = (f < bound && f < st.counters[DIMCV_C_FLUID_LIST]) ? st.fluid_list[f] : -2;
   if (i == -1) st.nbr_count[f] = 0;
   if (i >= 0) {
@@ -810,7 +828,7 @@ __global__ void __launch_bounds__(256) k_batch_nbr_build(dimcv_state st, dimcv_p
         const int lo = fl + zlo - 1;
         const int beg = lo >= 0 ? gpn[lo] : gpn[0];
         const int end = gpn[fl + zhi];
-        scan_run(CandFields{st.x, st.material}, beg, end, i, xi, lim, emit);
+        scan_run(st.x, beg, end, i, xi, lim, emit);
       }
     }
     st.nbr_count[f] = emit.cnt;
@@ -850,22 +868,8 @@ int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t 
     set_error("n_max %lld exceeds DIMCV_MAX_SLOTS (%d)", (long long)p->n_max, DIMCV_MAX_SLOTS);
     return DIMCV_ERR_ARG;
   }
-  if (fluid_bound > 0) {
-    // with a workspace, candidates are read from packed (x, material) records
-    Workspace ws;
-    const bool packed = !p->periodic && st->workspace &&
-                        st->workspace_bytes >= workspace_layout(p->n_max, p->n_cells, 1, nullptr, nullptr) &&
-                        bind_workspace(st, p, &ws, 0) == DIMCV_OK;
-    const int32_t pb = (int32_t)std::min<int64_t>(p->n_max, INT32_MAX);
-    if (packed) {
-      hipLaunchKernelGGL(k_pack_candidates, dim3(nblocks(pb)), dim3(256), 0, s, *st, pb, ws.xw);
-      hipLaunchKernelGGL(k_nbr_build<CandRecords>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p,
-                         fluid_bound, CandRecords{ws.xw});
-    } else {
-      hipLaunchKernelGGL(k_nbr_build<CandFields>, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound,
-                         CandFields{st->x, st->material});
-    }
-  }
+  if (fluid_bound > 0)
+    hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound);
   DIMCV_LAUNCHED("k_nbr_build");
   return DIMCV_OK;
 }
