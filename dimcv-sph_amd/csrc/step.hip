@@ -318,7 +318,8 @@ extern "C" int dimcv_substep(const dimcv_state* st, const dimcv_params* p, dimcv
 // replayed on the caller's stream: one launch instead of ~40, whose small
 // kernels would otherwise wait for the host to issue them.  If capture is
 // unavailable the kernels are issued directly.  The cache holds the last
-// four argument sets; calls from several threads must not overlap.
+// eight argument sets (buffer parity x uniform buffers x timing); calls
+// from several threads must not overlap.
 struct HeadKey {
   std::vector<unsigned char> bytes;
   void add(const void* q, size_t n) {
@@ -331,14 +332,17 @@ struct SegGraph {
   hipGraphExec_t exec;
 };
 // one cache per captured segment of dimcv_step
-enum { SEG_HEAD, SEG_MID1, SEG_MID2, SEG_COUNT };
+enum { SEG_HEAD, SEG_MID1, SEG_MID2, SEG_FINISH, SEG_COUNT };
 static std::vector<SegGraph> g_seg_graphs[SEG_COUNT];
 static hipStream_t g_capture_stream = nullptr;
 static bool g_graphs_off = false;
+static bool g_seg_off[SEG_COUNT] = {};
 
+// With `direct_on_error`, a segment whose capture fails (an error inside the
+// capture included) is issued directly, now and from then on.
 template <class F>
-static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s) {
-  if (g_graphs_off) return issue(s);
+static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s, bool direct_on_error = false) {
+  if (g_graphs_off || g_seg_off[seg]) return issue(s);
   std::vector<SegGraph>& cache = g_seg_graphs[seg];
   for (const SegGraph& g : cache)
     if (g.key == key.bytes) {
@@ -356,13 +360,18 @@ static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s) {
     ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
   }
   if (graph) (void)hipGraphDestroy(graph);
+  if (direct_on_error && (rc || !ok)) {
+    (void)hipGetLastError();
+    g_seg_off[seg] = true;
+    return issue(s);
+  }
   if (rc) return rc;
   if (!ok) {  // no capture on this runtime: issue directly from now on
     (void)hipGetLastError();
     g_graphs_off = true;
     return issue(s);
   }
-  if (cache.size() >= 4) {
+  if (cache.size() >= 8) {
     (void)hipGraphExecDestroy(cache.front().exec);
     cache.erase(cache.begin());
   }
@@ -413,6 +422,32 @@ static int step_prepare(const dimcv_state* pre, const dimcv_state* st, const dim
   sp->key.add(&sp->n_first_v, sizeof(sp->n_first_v));
   sp->key.add(&sp->n_first_p, sizeof(sp->n_first_p));
   return DIMCV_OK;
+}
+
+// The device-side rest of a step (both solves converging on the device):
+// the divergence solve's remaining iterations, the copy of its counters to
+// g_div_host, the substep to the far field, the far field (between its
+// timing events), the pressure solve and the tail, replayed as one graph.
+static int finish_device(const dimcv_state* st, const dimcv_params* p, dimcv_step_opts* o, int32_t pb,
+                         const StepPrep& sp, hipStream_t s) {
+  if (!g_div_host) DIMCV_HIP(hipHostMalloc((void**)&g_div_host, DIMCV_C_COUNT * sizeof(int32_t),
+                                           hipHostMallocDefault));
+  if (o->time_far && !g_step_ev[0]) {
+    DIMCV_HIP(hipEventCreate(&g_step_ev[0]));
+    DIMCV_HIP(hipEventCreate(&g_step_ev[1]));
+  }
+  const Workspace& ws = sp.ws;
+  HeadKey key = sp.key;
+  key.add(&o->time_far, sizeof(o->time_far));
+  return run_graph(SEG_FINISH, [&](hipStream_t hs) -> int {
+    CHECK(jacobi_device(0, st, p, o, pb, ws, hs));
+    DIMCV_HIP(hipMemcpyAsync(g_div_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, hs));
+    CHECK(seg_mid1(st, p, o, pb, pb, hs, ws));
+    CHECK(seg_far(st, p, o, pb, hs));
+    CHECK(seg_mid2(st, p, o, pb, pb, hs, ws, sp.n_first_p));
+    CHECK(jacobi_device(1, st, p, o, pb, ws, hs));
+    return seg_tail(st, p, o, pb, hs);
+  }, key, s, true);
 }
 
 // The grid build into the sorted buffers of `pre` (= the fields of `st`),
@@ -477,16 +512,7 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
     // pressure solve resets them, and the host waits once, at the end.  A
     // failed build runs the step with every writer of x, v, is_sample and
     // life_time returning (the solves start converged).
-    if (!g_div_host) DIMCV_HIP(hipHostMalloc((void**)&g_div_host, DIMCV_C_COUNT * sizeof(int32_t),
-                                             hipHostMallocDefault));
-    CHECK(jacobi_device(0, st, p, o, pb, ws, s));
-    DIMCV_HIP(hipMemcpyAsync(g_div_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
-    CHECK(seg_far(st, p, o, pb, s));
-    CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
-                    s));
-    CHECK(jacobi_device(1, st, p, o, pb, ws, s));
-    CHECK(seg_tail(st, p, o, pb, s));
+    CHECK(finish_device(st, p, o, pb, sp, s));
     CHECK(read_counters(st, s, host_counters));
     o->out_kv = g_div_host[DIMCV_C_JACOBI_ITERS];
     o->out_kp = host_counters[DIMCV_C_JACOBI_ITERS];
@@ -519,25 +545,13 @@ extern "C" int dimcv_step_finish_async(const dimcv_state* pre, const dimcv_state
     set_error("dimcv_step_finish_async: only the device-side step (chunk < 0, no dynamic rigid particles)");
     return DIMCV_ERR_ARG;
   }
-  const Workspace& ws = sp.ws;
-  const HeadKey& key = sp.key;
   const int32_t pb = particle_bound;
-  const int n_first_p = sp.n_first_p;
-  if (!g_div_host) DIMCV_HIP(hipHostMalloc((void**)&g_div_host, DIMCV_C_COUNT * sizeof(int32_t),
-                                           hipHostMallocDefault));
   if (!g_check_host) {
     DIMCV_HIP(hipHostMalloc((void**)&g_check_host, DIMCV_C_COUNT * sizeof(int32_t), hipHostMallocDefault));
     DIMCV_HIP(hipEventCreateWithFlags(&g_check_ev, hipEventDisableTiming));
   }
   if (!g_async_ev) DIMCV_HIP(hipEventCreateWithFlags(&g_async_ev, hipEventDisableTiming));
-  CHECK(jacobi_device(0, st, p, o, pb, ws, s));
-  DIMCV_HIP(hipMemcpyAsync(g_div_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  CHECK(run_graph(SEG_MID1, [&](hipStream_t hs) { return seg_mid1(st, p, o, pb, pb, hs, ws); }, key, s));
-  CHECK(seg_far(st, p, o, pb, s));
-  CHECK(run_graph(SEG_MID2, [&](hipStream_t hs) { return seg_mid2(st, p, o, pb, pb, hs, ws, n_first_p); }, key,
-                  s));
-  CHECK(jacobi_device(1, st, p, o, pb, ws, s));
-  CHECK(seg_tail(st, p, o, pb, s));
+  CHECK(finish_device(st, p, o, pb, sp, s));
   DIMCV_HIP(hipMemcpyAsync(g_check_host, st->counters, DIMCV_C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DIMCV_HIP(hipEventRecord(g_async_ev, s));
   g_async_pending = true;

@@ -1046,6 +1046,14 @@ namespace dimcv {
 // Optional HIP events recorded around the far-field kernel launch (used by
 // dimcv_substep's timing option; null when not timing).
 static hipEvent_t g_far_ev[2] = {nullptr, nullptr};
+// A timing event on s; inside a stream capture it becomes an event-record
+// node of the graph, recorded at each replay.
+static hipError_t record_far_event(hipEvent_t ev, hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive)
+    return hipEventRecordWithFlags(ev, s, hipEventRecordExternal);
+  return hipEventRecord(ev, s);
+}
 void set_far_timing(hipEvent_t a, hipEvent_t b) {
   g_far_ev[0] = a;
   g_far_ev[1] = b;
@@ -1073,7 +1081,7 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
                        st->counters + DIMCV_C_NUM_SAMPLES, (int64_t)max_samples, 1, ws.samples);
     const int splits = far_splits_mc(fluid_bound, ws);
     const bool kahan = p->far_mode == DIMCV_FAR_KAHAN;
-    if (g_far_ev[0]) DIMCV_HIP(hipEventRecord(g_far_ev[0], s));
+    if (g_far_ev[0]) DIMCV_HIP(record_far_event(g_far_ev[0], s));
     if (splits == 1) {
       auto kern = kahan ? (p->periodic ? k_vv_far_fast<true, true> : k_vv_far_fast<true, false>)
                         : (p->periodic ? k_vv_far_fast<false, true> : k_vv_far_fast<false, false>);
@@ -1084,7 +1092,7 @@ int dimcv_vort2vel_far(const dimcv_state* st, const dimcv_params* p, int32_t flu
                             splits, kahan, ws, 0, s);
       if (rc) return rc;
     }
-    if (g_far_ev[1]) DIMCV_HIP(hipEventRecord(g_far_ev[1], s));
+    if (g_far_ev[1]) DIMCV_HIP(record_far_event(g_far_ev[1], s));
   }
   DIMCV_LAUNCHED("vort2vel far field");
   return DIMCV_OK;
