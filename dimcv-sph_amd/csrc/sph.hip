@@ -236,9 +236,10 @@ __global__ void k_pack_xm(dimcv_state st, dimcv_params p, int32_t bound, float4*
   if (tiny_coord(x) || tiny_coord(y) || tiny_coord(z)) atomicOr(tiny, 1);
 }
 
+// zero_tiny false: the caller has cleared *pk.tiny already (k_step_guard).
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, (int32_t*)pk.tiny);
+                   hipStream_t s, bool zero_tiny) {
+  if (zero_tiny) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, (int32_t*)pk.tiny);
   if (particle_bound <= 0) return DIMCV_OK;
   hipLaunchKernelGGL(k_pack_xm, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p, particle_bound,
                      (float4*)pk.xm, (float4*)pk.xw, (int32_t*)pk.tiny);
@@ -355,9 +356,12 @@ __global__ void __launch_bounds__(256) k_div_iter(dimcv_state st, dimcv_params p
 
 // dimcv_sph.py:150-208, fluid particles (surface tension sigma = 0 and the
 // boundary viscosity 0 contribute exact zeros and are not evaluated).
+// With factor, also multiply_time_step (dimcv_sph.py:395-399) of that field
+// (the DFSPH factor, which this pass does not read): factor[i] *= factor_scale.
 template <bool XM>
 __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv_params p, int32_t bound,
-                                                           Packed pk) {
+                                                           Packed pk, float* __restrict__ factor,
+                                                           float factor_scale) {
   FLUID_ENTRY();
   const bool sd = XM && *pk.tiny == 0;  // shared-reciprocal divisions in cubic_dW
   const float visc_coef = st.block_scene ? st.scenes[block_scene_of(st)].visc_coef : p.visc_coef;
@@ -375,6 +379,7 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
              }
            });
   st3(st.acceleration, i, dv);
+  if (factor) factor[i] *= factor_scale;
 }
 
 // dimcv_sph.py:197-204 for non-fluid slots: static rigid -> 0, otherwise g.
@@ -1114,7 +1119,7 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 // compute_non_pressure_forces (dimcv_sph.py:150-208): non-fluid slots, then
 // the fluid pass (gathering the packed records when pk is given).
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const Packed* pk) {
+                       hipStream_t s, const Packed* pk, float* factor, float factor_scale) {
   const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
   if (particle_bound > 0)
     hipLaunchKernelGGL(k_nonpressure_other, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p,
@@ -1122,8 +1127,8 @@ int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t par
   DIMCV_LAUNCHED("k_nonpressure_other");
   if (fluid_bound > 0) {
     const dim3 g(nblocks(fluid_bound)), b(256);
-    if (pk) hipLaunchKernelGGL(k_nonpressure_fluid<true>, g, b, 0, s, *st, *p, fluid_bound, q);
-    else hipLaunchKernelGGL(k_nonpressure_fluid<false>, g, b, 0, s, *st, *p, fluid_bound, q);
+    if (pk) hipLaunchKernelGGL(k_nonpressure_fluid<true>, g, b, 0, s, *st, *p, fluid_bound, q, factor, factor_scale);
+    else hipLaunchKernelGGL(k_nonpressure_fluid<false>, g, b, 0, s, *st, *p, fluid_bound, q, factor, factor_scale);
   }
   DIMCV_LAUNCHED("k_nonpressure_fluid");
   return DIMCV_OK;
@@ -1273,7 +1278,7 @@ int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params*
 
 int dimcv_compute_non_pressure_forces(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
                                       int32_t fluid_bound, void* stream) {
-  return dimcv::launch_nonpressure(st, p, particle_bound, fluid_bound, (hipStream_t)stream, nullptr);
+  return dimcv::launch_nonpressure(st, p, particle_bound, fluid_bound, (hipStream_t)stream, nullptr, nullptr, 1.0f);
 }
 
 int dimcv_predict_velocity(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {

@@ -19,13 +19,13 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
                        const int32_t* done, hipStream_t s, double* partials = nullptr,
                        float offset = 0.0f, const Packed* pk = nullptr);
 int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, const Packed& pk,
-                   hipStream_t s);
+                   hipStream_t s, bool zero_tiny = true);
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
                        float* kap, hipStream_t s, bool reset_jacobi = false, int32_t* bar_tree = nullptr);
 int launch_copy_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_predict_life(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const Packed* pk);
+                       hipStream_t s, const Packed* pk, float* factor = nullptr, float factor_scale = 1.0f);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
 int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
@@ -47,7 +47,8 @@ __global__ void k_jacobi_reset(int32_t* counters) {
 // keyed past the cell table, a list wider than nbr_kmax (the lists are
 // truncated) or no fluid (the reference divides by fluid_particle_num).
 // Nonzero makes the substep's state writers return (step_failed).
-__global__ void k_step_guard(int32_t* counters, int32_t nbr_kmax) {
+__global__ void k_step_guard(int32_t* counters, int32_t nbr_kmax, int32_t* tiny) {
+  *tiny = 0;  // the tiny-coordinate flag that k_pack_xm raises next
   counters[DIMCV_C_BAR_ABORT] = 0;
   int32_t e = 0;
   if (counters[DIMCV_C_KEY_ERROR] != 0) e |= 1;
@@ -189,8 +190,8 @@ static int substep_head(const dimcv_state* st, const dimcv_params* p, const dimc
                         int32_t particle_bound, int32_t fluid_bound, hipStream_t s, const Workspace& ws,
                         int n_first) {
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
-  hipLaunchKernelGGL(k_step_guard, dim3(1), dim3(1), 0, s, st->counters, p->nbr_kmax);
-  CHECK(launch_pack_xm(st, p, particle_bound, pk, s));
+  hipLaunchKernelGGL(k_step_guard, dim3(1), dim3(1), 0, s, st->counters, p->nbr_kmax, ws.tiny);
+  CHECK(launch_pack_xm(st, p, particle_bound, pk, s, false));
   // compute_densities + compute_DFSPH_factor + the first
   // compute_density_change of divergence_solve (dimcv_sph.py:376-393) in one pass
   CHECK(launch_fused(3, st, p, fluid_bound, s, nullptr, &pk));
@@ -205,8 +206,8 @@ static int seg_mid1(const dimcv_state* st, const dimcv_params* p, const dimcv_st
                     int32_t fluid_bound, hipStream_t s, const Workspace& ws) {
   void* stream = (void*)s;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
-  CHECK(dimcv_multiply_time_step(st, p, st->dfsph_factor, o->dt, fluid_bound, stream));
-  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk));
+  // multiply_time_step (dimcv_sph.py:395-399) inside the non-pressure pass
+  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk, st->dfsph_factor, o->dt));
   // dimcv() (dimcv_sph.py:812-819)
   // init() + compute_d_vorticity in one pass; then the KVN (and Bernoulli
   // sample update), which reads only grad_v; then the smoothing pass, which
@@ -380,6 +381,10 @@ static int run_graph(int seg, F&& issue, const HeadKey& key, hipStream_t s, bool
   return DIMCV_OK;
 }
 
+// A step whose solves converge on the device (no host checks): both of its
+// halves decide it the same way from the same arguments.
+static bool device_step(const dimcv_step_opts* o, const dimcv_params* p) { return o->chunk < 0 && !p->dyn_rigid; }
+
 // Arguments, workspace and graph key of one dimcv_step (the same for its
 // begin and finish halves).
 struct StepPrep {
@@ -505,7 +510,7 @@ extern "C" int dimcv_step_finish(const dimcv_state* pre, const dimcv_state* st, 
   const HeadKey& key = sp.key;
   const int32_t pb = particle_bound;
   const int n_first_p = sp.n_first_p;
-  if (o->chunk < 0 && !p->dyn_rigid) {
+  if (device_step(o, p)) {
     // device-side convergence: the rest of the step is enqueued without a
     // host check (each solve's remaining iterations, if any, in a persistent
     // launch); the divergence solve's counters are copied aside before the
@@ -541,7 +546,7 @@ extern "C" int dimcv_step_finish_async(const dimcv_state* pre, const dimcv_state
   CHECK(async_idle("dimcv_step_finish_async"));
   StepPrep sp;
   CHECK(step_prepare(pre, st, p, o, particle_bound, &sp));
-  if (o->chunk >= 0 || p->dyn_rigid) {
+  if (!device_step(o, p)) {
     set_error("dimcv_step_finish_async: only the device-side step (chunk < 0, no dynamic rigid particles)");
     return DIMCV_ERR_ARG;
   }
