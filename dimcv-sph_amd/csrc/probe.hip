@@ -56,7 +56,8 @@ static int probe_run(int kind, unsigned blocks, int iters, float* sink, unsigned
   DIMCV_HIP(hipEventCreate(&e[0]));
   DIMCV_HIP(hipEventCreate(&e[1]));
   auto kern = kind == 0 ? k_valu_probe<0> : k_valu_probe<1>;
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, sink, iters, 0.999f, clk);  // warm-up
+  for (int w = 0; w < 8; ++w)  // warm-up: the clock settles under the load (about 30 ms)
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, sink, iters, 0.999f, clk);
   DIMCV_HIP(hipEventRecord(e[0], s));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, sink, iters, 0.999f, clk);
   DIMCV_HIP(hipEventRecord(e[1], s));
