@@ -271,10 +271,37 @@ __global__ void k_scan_u32_add(uint32_t* __restrict__ out, int64_t n, const uint
     if (base + k < n) out[base + k] += add;
 }
 
+// n <= 4096: the whole scan in one workgroup of 1024 threads, four values
+// each (one launch instead of three; integer sums, so the same result).
+__global__ void __launch_bounds__(1024) k_scan_u32_one(const uint32_t* in, uint32_t* out, int64_t n,
+                                                       uint32_t* total) {
+  __shared__ uint32_t lw[16];
+  const int64_t base = (int64_t)threadIdx.x * 4;
+  uint32_t a[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = (base + k < n) ? in[base + k] : 0u;
+    s += a[k];
+  }
+  uint32_t tot;
+  uint32_t pre = block_excl_scan_u32(s, lw, &tot);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = pre;
+    pre += a[k];
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, uint32_t* total,
                        void* stream) {
   if (n <= 0) return DIMCV_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (n <= 4096) {
+    hipLaunchKernelGGL(k_scan_u32_one, dim3(1), dim3(1024), 0, s, in, out, n, total);
+    DIMCV_LAUNCHED("exclusive_scan_u32");
+    return DIMCV_OK;
+  }
   unsigned nb = nblocks(n, 1024);
   hipLaunchKernelGGL(k_scan_u32_local, dim3(nb), dim3(256), 0, s, in, out, n, tmp);
   hipLaunchKernelGGL(k_scan_u32_sums, dim3(1), dim3(256), 0, s, tmp, (int64_t)nb, total);
