@@ -248,24 +248,20 @@ __global__ void k_scan_u32_local(const uint32_t* in, uint32_t* out, int64_t n, u
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_u32_sums(uint32_t* bsum, int64_t nb, uint32_t* total) {
+// Block b adds the sum of the block totals before it (each block forms that
+// carry itself: integer sums, so every order gives the same value); the last
+// block also writes the total.
+__global__ void k_scan_u32_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ bsum,
+                               int64_t nb, uint32_t* total) {
   __shared__ uint32_t lw[4];
-  uint32_t carry = 0;
-  for (int64_t c = 0; c < nb; c += 256) {
-    int64_t i = c + threadIdx.x;
-    uint32_t v = i < nb ? bsum[i] : 0u;
-    uint32_t tot;
-    uint32_t pre = block_excl_scan_u32(v, lw, &tot);
-    if (i < nb) bsum[i] = carry + pre;
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
-}
-
-__global__ void k_scan_u32_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ bsum) {
+  uint32_t v = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += 256) v += bsum[i];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const uint32_t add = lw[0] + lw[1] + lw[2] + lw[3];
+  if (total && threadIdx.x == 0 && (int64_t)blockIdx.x == nb - 1) *total = add + bsum[blockIdx.x];
   int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
-  uint32_t add = bsum[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (base + k < n) out[base + k] += add;
@@ -304,8 +300,7 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* t
   }
   unsigned nb = nblocks(n, 1024);
   hipLaunchKernelGGL(k_scan_u32_local, dim3(nb), dim3(256), 0, s, in, out, n, tmp);
-  hipLaunchKernelGGL(k_scan_u32_sums, dim3(1), dim3(256), 0, s, tmp, (int64_t)nb, total);
-  hipLaunchKernelGGL(k_scan_u32_add, dim3(nb), dim3(256), 0, s, out, n, tmp);
+  hipLaunchKernelGGL(k_scan_u32_add, dim3(nb), dim3(256), 0, s, out, n, (const uint32_t*)tmp, (int64_t)nb, total);
   DIMCV_LAUNCHED("exclusive_scan_u32");
   return DIMCV_OK;
 }
