@@ -486,8 +486,10 @@ __global__ void __launch_bounds__(256) k_direct_f64(dimcv_state st, dimcv_params
 // 256 threads x 16 consecutive values: l0 serial per thread, l1 serial per 16
 // threads over thread totals, L2 serial over the 16 group totals;
 // L1[t] = L2[g-1] + l1[t], L[i] = L1[t-1] + l0[i].
-__device__ void block_local_scan_f64(const double* __restrict__ y, double* __restrict__ L, int64_t m,
-                                     double* tt, double* l1s, double* L2s, double* L1s) {
+// y(k) gives value k of the tile.
+template <class Y>
+__device__ void block_local_scan_f64(Y&& y, double* __restrict__ L, int64_t m, double* tt, double* l1s,
+                                     double* L2s, double* L1s) {
   const int t = threadIdx.x;
   const int nthreads = (int)((m + 15) / 16);
   double l0[16];
@@ -495,10 +497,10 @@ __device__ void block_local_scan_f64(const double* __restrict__ y, double* __res
   if (t < nthreads) {
     int64_t s = (int64_t)t * 16;
     int64_t e = min(s + 16, m);
-    acc = y[s];
+    acc = y(s);
     l0[0] = acc;
     for (int64_t k = s + 1; k < e; ++k) {
-      acc = acc + y[k];
+      acc = acc + y(k);
       l0[k - s] = acc;
     }
     tt[t] = acc;
@@ -537,27 +539,37 @@ __device__ void block_local_scan_f64(const double* __restrict__ y, double* __res
   __syncthreads();
 }
 
+__device__ void block_local_scan_f64(const double* __restrict__ y, double* __restrict__ L, int64_t m,
+                                     double* tt, double* l1s, double* L2s, double* L1s) {
+  block_local_scan_f64([&](int64_t k) { return y[k]; }, L, m, tt, l1s, L2s, L1s);
+}
+
 // weights: in-domain fluid kden (fluid-list order), 0 otherwise
-__global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, double* __restrict__ w) {
-  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
-  if (f >= bound) return;
+__device__ __forceinline__ double cdf_weight(const dimcv_state& st, const dimcv_params& p, int64_t f) {
   double v = 0.0;
   if (f < st.counters[DIMCV_C_FLUID_LIST]) {
     const int64_t i = st.fluid_list[f];
     if (i >= 0 && in_vortex_domain(p, ld3(st.x, i))) v = (double)st.kernel_density[i];
   }
-  w[f] = v;
+  return v;
 }
 
-__global__ void __launch_bounds__(256) k_scan_blocks(const double* __restrict__ x, double* __restrict__ L,
-                                                     const int32_t* __restrict__ nptr, int64_t cap,
-                                                     double* __restrict__ btot) {
+__global__ void k_cdf_weights(dimcv_state st, dimcv_params p, int32_t bound, double* __restrict__ w) {
+  const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+  if (f >= bound) return;
+  w[f] = cdf_weight(st, p, f);
+}
+
+// Canonical scan tiles of the CDF weights (cdf_weight), each weight formed
+// where the tile scan reads it.
+__global__ void __launch_bounds__(256) k_weights_scan_blocks(dimcv_state st, dimcv_params p, double* __restrict__ L,
+                                                             int64_t cap, double* __restrict__ btot) {
   __shared__ double tt[256], l1s[256], L2s[16], L1s[256];
-  const int64_t n = min((int64_t)*nptr, cap);
+  const int64_t n = min((int64_t)st.counters[DIMCV_C_FLUID_LIST], cap);
   const int64_t s = (int64_t)blockIdx.x * DIMCV_SCAN_TILE;
   if (s >= n) return;  // uniform per block
   const int64_t m = min((int64_t)DIMCV_SCAN_TILE, n - s);
-  block_local_scan_f64(x + s, L + s, m, tt, l1s, L2s, L1s);
+  block_local_scan_f64([&](int64_t k) { return cdf_weight(st, p, s + k); }, L + s, m, tt, l1s, L2s, L1s);
   if (threadIdx.x == 0) btot[blockIdx.x] = L[s + m - 1];
 }
 
@@ -935,9 +947,9 @@ static int cdf_launch(const dimcv_state* st, const dimcv_params* p, const double
   const int32_t* nF = st->counters + DIMCV_C_FLUID_LIST;
   const int64_t cap = fluid_bound;
   if (fluid_bound > 0) {
-    hipLaunchKernelGGL(k_cdf_weights, dim3(nblocks(fluid_bound)), dim3(256), 0, s, *st, *p, fluid_bound, weights);
+    (void)weights;
     unsigned nb = nblocks(fluid_bound, DIMCV_SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, weights, cdf, nF, cap, btot);
+    hipLaunchKernelGGL(k_weights_scan_blocks, dim3(nb), dim3(256), 0, s, *st, *p, cdf, cap, btot);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, s, btot, bpre, nF, cap);
     hipLaunchKernelGGL(k_scan_add, dim3(nblocks(fluid_bound)), dim3(256), 0, s, cdf, bpre, nF, cap,
                        st->dscal + DIMCV_D_CDF_TOTAL);
