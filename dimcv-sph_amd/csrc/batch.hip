@@ -35,7 +35,8 @@ int launch_pack_xm(const dimcv_state* st, const dimcv_params* p, int32_t particl
 int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, float scale, int32_t fluid_bound,
                        float* kap, hipStream_t s, bool reset_jacobi = false, int32_t* bar_tree = nullptr);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const Packed* pk, float* factor = nullptr, float factor_scale = 1.0f);
+                       hipStream_t s, const Packed* pk, float* factor = nullptr, float factor_scale = 1.0f,
+                       bool zero_fixed_m = false);
 int batch_grid_launch(const dimcv_state* st, const dimcv_params* p, Workspace& ws, int32_t* ulist,
                       int32_t* ucount, hipStream_t s);
 int batch_nbr_launch(const dimcv_state* st, const dimcv_params* p, hipStream_t s);

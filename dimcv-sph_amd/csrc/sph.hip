@@ -386,8 +386,16 @@ __global__ void __launch_bounds__(256) k_nonpressure_fluid(dimcv_state st, dimcv
 // One value and one store, chosen by selects: written as two stores under
 // if/else, the gfx950 build stored the static branch's zero for dynamic
 // rigid slots as well (the join overwrote the gravity registers).
-__global__ void k_nonpressure_other(dimcv_state st, dimcv_params p, int32_t bound) {
+// With zero_n > 0 it also does k_zero_nonfluid's fill of z for slots below
+// zero_n (every slot but the fluid ones below particle_num).
+__global__ void k_nonpressure_other(dimcv_state st, dimcv_params p, int32_t bound, ZeroSet z, int64_t zero_n) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < zero_n && !(s < st.counters[DIMCV_C_PARTICLE_NUM] && st.material[s] == 1)) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (z.a[q])
+        for (int k = 0; k < z.c[q]; ++k) z.a[q][s * z.c[q] + k] = 0.0f;
+  }
   if (s >= bound || s >= st.counters[DIMCV_C_PARTICLE_NUM]) return;
   const int32_t mat = st.material[s];
   if (mat == 1) return;
@@ -1118,12 +1126,19 @@ int launch_jacobi_pass(int which, const dimcv_state* st, const dimcv_params* p, 
 
 // compute_non_pressure_forces (dimcv_sph.py:150-208): non-fluid slots, then
 // the fluid pass (gathering the packed records when pk is given).
+// zero_fixed_m: also the non-fluid zero fill of the fixed-M vorticity pass
+// (launch_fused 7 then skips it).
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const Packed* pk, float* factor, float factor_scale) {
+                       hipStream_t s, const Packed* pk, float* factor, float factor_scale, bool zero_fixed_m) {
   const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
-  if (particle_bound > 0)
-    hipLaunchKernelGGL(k_nonpressure_other, dim3(nblocks(particle_bound)), dim3(256), 0, s, *st, *p,
-                       particle_bound);
+  const ZeroSet z = zero_fixed_m ? ZeroSet{{st->v_star, st->vorticity, st->grad_v, st->d_vort, st->vorticity_vis},
+                                           {3, 3, 9, 3, 3}}
+                                 : ZeroSet{{nullptr, nullptr, nullptr, nullptr, nullptr}, {0, 0, 0, 0, 0}};
+  const int64_t zero_n = zero_fixed_m ? p->n_max : 0;
+  const int64_t other_n = std::max<int64_t>(particle_bound, zero_n);
+  if (other_n > 0)
+    hipLaunchKernelGGL(k_nonpressure_other, dim3(nblocks(other_n)), dim3(256), 0, s, *st, *p, particle_bound, z,
+                       zero_n);
   DIMCV_LAUNCHED("k_nonpressure_other");
   if (fluid_bound > 0) {
     const dim3 g(nblocks(fluid_bound)), b(256);
@@ -1136,8 +1151,9 @@ int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t par
 
 // which: 0 density + factor; 3 density + factor + first density change; 1
 // v*, vorticity, grad v, d_vort; 5 as 1 with the fixed-M KVN, also zeroing
-// vorticity_vis's non-fluid slots; 4 smoothing + AKDE with kden partials; 6
-// as 4 after a 5 (no zero fill); 2 smoothing alone.
+// vorticity_vis's non-fluid slots; 7 as 5 after launch_nonpressure with
+// zero_fixed_m (which did the zero fill); 4 smoothing + AKDE with kden
+// partials; 6 as 4 after a 5 or 7 (no zero fill); 2 smoothing alone.
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
                  double* partials, const Packed* pk) {
   const Packed q = pk ? *pk : Packed{nullptr, nullptr, nullptr, nullptr};
@@ -1145,6 +1161,7 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
     int rc = zero_nonfluid(st, p, s, st->v_star, 3, st->vorticity, 3, st->grad_v, 9);
     if (!rc) rc = zero_nonfluid(st, p, s, st->d_vort, 3);
     if (rc) return rc;
+  } else if (which == 7) {  // as 5, its zero fill done by launch_nonpressure (zero_fixed_m)
   } else if (which == 5) {  // as 1, and vorticity_vis for a following which 6, in one launch
     int rc = zero_nonfluid(st, p, s, ZeroSet{{st->v_star, st->vorticity, st->grad_v, st->d_vort, st->vorticity_vis},
                                              {3, 3, 9, 3, 3}});
@@ -1173,6 +1190,7 @@ int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_
       else hipLaunchKernelGGL((k_density_factor<true, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
     case 5:
+    case 7:
       if (pk) hipLaunchKernelGGL((k_vort_gradv<15, true>), g, b, 0, s, *st, *p, fluid_bound, q);
       else hipLaunchKernelGGL((k_vort_gradv<15, false>), g, b, 0, s, *st, *p, fluid_bound, q);
       break;
@@ -1278,7 +1296,8 @@ int dimcv_divergence_solver_iteration(const dimcv_state* st, const dimcv_params*
 
 int dimcv_compute_non_pressure_forces(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound,
                                       int32_t fluid_bound, void* stream) {
-  return dimcv::launch_nonpressure(st, p, particle_bound, fluid_bound, (hipStream_t)stream, nullptr, nullptr, 1.0f);
+  return dimcv::launch_nonpressure(st, p, particle_bound, fluid_bound, (hipStream_t)stream, nullptr, nullptr, 1.0f,
+                                   false);
 }
 
 int dimcv_predict_velocity(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, void* stream) {

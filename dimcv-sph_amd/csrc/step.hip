@@ -25,7 +25,8 @@ int launch_scale_kappa(int which, const dimcv_state* st, const dimcv_params* p, 
 int launch_copy_advect(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_predict_life(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, hipStream_t s);
 int launch_nonpressure(const dimcv_state* st, const dimcv_params* p, int32_t particle_bound, int32_t fluid_bound,
-                       hipStream_t s, const Packed* pk, float* factor = nullptr, float factor_scale = 1.0f);
+                       hipStream_t s, const Packed* pk, float* factor = nullptr, float factor_scale = 1.0f,
+                       bool zero_fixed_m = false);
 void set_far_timing(hipEvent_t a, hipEvent_t b);
 int neighbor_build_launch(const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s);
 int launch_fused(int which, const dimcv_state* st, const dimcv_params* p, int32_t fluid_bound, hipStream_t s,
@@ -207,7 +208,9 @@ static int seg_mid1(const dimcv_state* st, const dimcv_params* p, const dimcv_st
   void* stream = (void*)s;
   const Packed pk{ws.xm, ws.xw, ws.kap, ws.tiny};
   // multiply_time_step (dimcv_sph.py:395-399) inside the non-pressure pass
-  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk, st->dfsph_factor, o->dt));
+  // with fixed-M sampling it also zero-fills the non-fluid slots the vorticity
+  // pass leaves (launch_fused 7)
+  CHECK(launch_nonpressure(st, p, particle_bound, fluid_bound, s, &pk, st->dfsph_factor, o->dt, o->m > 0));
   // dimcv() (dimcv_sph.py:812-819)
   // init() + compute_d_vorticity in one pass; then the KVN (and Bernoulli
   // sample update), which reads only grad_v; then the smoothing pass, which
@@ -216,7 +219,7 @@ static int seg_mid1(const dimcv_state* st, const dimcv_params* p, const dimcv_st
   // With fixed-M sampling the KVN is computed inside the vorticity pass, and
   // one zero fill covers the non-fluid slots of both passes.
   if (o->m > 0) {
-    CHECK(launch_fused(5, st, p, fluid_bound, s, nullptr, &pk));
+    CHECK(launch_fused(7, st, p, fluid_bound, s, nullptr, &pk));
   } else {
     CHECK(launch_fused(1, st, p, fluid_bound, s, nullptr, &pk));
     CHECK(dimcv_compute_kvn_sample(st, p, o->u1, o->u2, 1, fluid_bound, stream));
